@@ -1,0 +1,158 @@
+/* mscx.h -- C ABI of the MI355X Morse-Smale complex engine (libmscx.so).
+ *
+ * Drop-in boundary for the reference's GridDataset / GridMSComplex /
+ * GridDataManager compute path (nithins/mscomplex, dmsc).  Every function
+ * returns 0 on success and a negative status on failure; the message of the
+ * last failure on the calling thread is returned by mscx_last_error().
+ * Plain pointers and sizes only.  Cell ids are packed as x | (y << 16) with
+ * x, y the int16 cell coordinates of the reference's cellid_t (the same bytes
+ * as its two_tuple_t<int16_t> on little-endian hosts); (-1,-1) packs to
+ * 0xFFFFFFFF.  Arrays over a tile are laid out over its ext rect, x fastest:
+ * index = (x - ext[0]) + (y - ext[1]) * (ext[2] - ext[0] + 1).
+ * Rects are int32_t[4] = {bl.x, bl.y, tr.x, tr.y} in cell coordinates
+ * (vertex (i, j) is cell (2i, 2j)).
+ */
+#ifndef MSCX_H_INCLUDED
+#define MSCX_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSCX_OK 0
+#define MSCX_EINVAL -1    /* bad argument (reference: std::invalid_argument / range_error) */
+#define MSCX_EDEVICE -2   /* HIP runtime failure (reference: runtime_error from _CHECKCL_ERR_CODE) */
+#define MSCX_ELOGIC -3    /* invariant violated (reference: std::logic_error) */
+#define MSCX_ENOMEM -4    /* allocation failure or HBM budget exceeded */
+#define MSCX_ESTATE -5    /* call out of order (e.g. download before work) */
+
+typedef struct mscx_tile mscx_tile;       /* a GridDataset: one rect + ext window on the GPU */
+typedef struct mscx_complex mscx_complex; /* a GridMSComplex (host resident) */
+typedef struct mscx_grid mscx_grid;       /* a GridDataManager run over one grid */
+
+/* ---- runtime ------------------------------------------------------------
+ * Replaces GridDataset::init_opencl / stop_opencl (grid_dataset.cpp:186-226,
+ * grid_dataset.h:236-238).  mscx_init selects the HIP device; it is called
+ * implicitly by the first tile call with device 0. */
+int mscx_init(int device);
+int mscx_shutdown(void);
+const char *mscx_last_error(void);
+int mscx_version(void);
+
+/* ---- tiles: GridDataset ------------------------------------------------- */
+
+/* GridDataset(const rect_t &r, const rect_t &e) (grid_dataset.h:97,
+ * grid_dataset.cpp:1429-1441).  ext must contain rect; both have even
+ * corners; at most 16384 vertices per side (int16 cell ids). */
+int mscx_tile_create(const int32_t rect[4], const int32_t ext[4], mscx_tile **out);
+int mscx_tile_destroy(mscx_tile *t);
+
+/* GridDataset::init(float *) (grid_dataset.cpp:1461-1484): the ext window
+ * of vertex values, x fastest, (ext_w/2+1) * (ext_h/2+1) floats, copied to
+ * HBM.  NaN values are rejected with MSCX_EINVAL. */
+int mscx_tile_upload(mscx_tile *t, const float *x_fastest, size_t n);
+
+/* Device-resident variant of init(): dev_ptr points to the window's first
+ * vertex in HBM with row_pitch floats per row.  No copy, no NaN check. */
+int mscx_tile_bind_device(mscx_tile *t, const float *dev_ptr, int64_t row_pitch);
+
+/* GridDataset::work() / work_ocl() (grid_dataset.cpp:1734-1741, :353-399):
+ * gradient, critical collation, owner tracing and saddle incidence on the
+ * GPU, synchronously. */
+int mscx_tile_work(mscx_tile *t);
+
+/* Batched work(): one launch per stage over all n tiles. */
+int mscx_tiles_work(mscx_tile *const *tiles, int n);
+
+/* Number of critical cells K (m_critical_cells.size()). */
+int mscx_tile_num_critical(const mscx_tile *t, uint32_t *K);
+
+/* Host copies of m_cell_flags (int8), m_cell_pairs and m_cell_own (packed
+ * ids) over the ext rect, and m_critical_cells (K packed ids, raster order).
+ * Any pointer may be NULL. */
+int mscx_tile_download(const mscx_tile *t, int8_t *flags, uint32_t *pairs, uint32_t *own, uint32_t *crit);
+
+/* Raw per-cell gradient codes (bits 0-1 flags, bits 2-4 pair direction
+ * 0 none, 1 +x, 2 -x, 3 +y, 4 -y) over the ext rect, x fastest. */
+int mscx_tile_download_codes(const mscx_tile *t, uint8_t *codes);
+
+/* Per critical cell: function value (get_cell_fn), incident owners of a
+ * saddle (two facet owners then two cofacet owners, as critical indices or
+ * 0xFFFFFFFF), and the critical index of the pair of a boundary-cancellable
+ * cell (or 0xFFFFFFFF).  Arrays of K, K, 4K, K entries; any may be NULL. */
+int mscx_tile_download_critical(const mscx_tile *t, float *fn, uint32_t *conn4, uint32_t *pair_idx);
+
+/* ---- complexes: GridMSComplex -------------------------------------------- */
+
+/* GridDataset::writeout_connectivity (grid_dataset.cpp:1858-1907): builds a
+ * complex from a tile after mscx_tile_work. */
+int mscx_complex_from_tile(const mscx_tile *t, mscx_complex **out);
+
+/* GridMSComplex::merge_up (grid_mscomplex.cpp:24-156); the caller owns *out. */
+int mscx_complex_merge_up(const mscx_complex *a, const mscx_complex *b, mscx_complex **out);
+
+/* GridMSComplex::merge_down (grid_mscomplex.cpp:158-322). */
+int mscx_complex_merge_down(mscx_complex *parent, mscx_complex *a, mscx_complex *b);
+
+/* GridMSComplex::simplify_un_simplify (grid_mscomplex.cpp:528-535). */
+int mscx_complex_simplify_un_simplify(mscx_complex *c, double threshold);
+
+/* GridDataset::postMergeFillDiscs (grid_dataset.cpp:1044-1231): fills the
+ * manifold discs of the tile's complex from the tile's owner image. */
+int mscx_tile_fill_discs(mscx_tile *t, mscx_complex *c);
+
+/* GridMSComplex::write_discs (grid_mscomplex.cpp:542-590): appends raw
+ * packed cell ids to <prefix>asc_(x,y) / <prefix>des_(x,y). */
+int mscx_complex_write_discs(const mscx_complex *c, const char *prefix);
+
+/* Canonical dump of a complex:
+ *   [ncp, rect[4], ext[4]] then per critical point: cellid, fn bits,
+ *   pair_idx, flags (1 cancelled, 2 strangulation, 4 boundary-cancellable),
+ *   n_asc, n_des, n_asc_disc, n_des_disc, asc..., des..., asc_disc...,
+ *   des_disc...
+ * *len receives the word count; the words are written when cap suffices. */
+int mscx_complex_dump(const mscx_complex *c, uint32_t *buf, size_t cap, size_t *len);
+int mscx_complex_num_cps(const mscx_complex *c, uint32_t *n);
+int mscx_complex_destroy(mscx_complex *c);
+
+/* ---- whole-grid pipeline: GridDataManager -------------------------------- */
+
+/* GridDataManager(filename, size_x, size_y, num_levels, ..., simp_tresh, ...)
+ * compute order (grid_datamanager.cpp:596-662): tile the grid, compute every
+ * leaf on the GPU in one batched launch per stage, merge up, simplify the
+ * root at `threshold` (fraction of the critical-value range; 0 = none),
+ * merge down, and fill manifold discs.  levels = slab levels along y as in
+ * createPieces_quadtree (grid_datamanager.cpp:53-89); lx/ly > 0 selects a
+ * 2^lx x 2^ly 2-D tiling.  data is host (x fastest) unless
+ * MSCX_RUN_DEVICE_INPUT is set, then it is a device pointer. */
+#define MSCX_RUN_DEVICE_INPUT 1
+#define MSCX_RUN_NO_DISCS 2
+int mscx_grid_run(const float *data, int32_t size_x, int32_t size_y, int32_t levels, int32_t lx, int32_t ly,
+                  double threshold, uint32_t flags, mscx_grid **out);
+int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n);
+/* Borrowed pointer to piece i's complex (leaves first, root last). */
+int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out);
+/* Stage times in seconds: compute, merge-up, simplify, merge-down, discs. */
+int mscx_grid_times(const mscx_grid *g, double t5[5]);
+int mscx_grid_destroy(mscx_grid *g);
+
+/* ---- synthetic inputs (seeded, host) -------------------------------------- */
+
+/* Sum of n Gaussians, centres U[0,sx)xU[0,sy), sigma U[8,64], amplitude
+ * N(0,1), plus U[-noise, noise]. */
+int mscx_gen_gaussians(float *out, int32_t sx, int32_t sy, int32_t n, double noise, uint64_t seed);
+/* Fractal value noise: octaves, lacunarity, gain, lattice values U[0,1),
+ * base period `period` pixels, smoothstep-bilinear interpolation. */
+int mscx_gen_value_noise(float *out, int32_t sx, int32_t sy, int32_t octaves, double lacunarity, double gain,
+                         double period, uint64_t seed);
+/* White noise U[0,1), or quantised to `levels` levels when levels > 0. */
+int mscx_gen_white(float *out, int32_t sx, int32_t sy, int32_t levels, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSCX_H_INCLUDED */

@@ -1,0 +1,239 @@
+"""mscomplex_amd -- MI355X Morse-Smale complex engine for 2D float32 grids.
+
+Python mirror of the reference's grid path (dmsc): GridDataset,
+GridMSComplex and the GridDataManager compute order, backed by libmscx.so
+(hand-written HIP kernels for gfx950 plus a C++ host runtime).  There is no
+CPU fallback: every compute call goes through the HIP library.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import MscxError, check, lib, ptr
+
+__all__ = ["GridDataset", "GridMSComplex", "GridRun", "work_tiles", "gen_gaussians", "gen_value_noise",
+           "gen_white", "MscxError", "unpack", "pack", "grid_rect"]
+
+
+def pack(x, y):
+    return (int(x) & 0xFFFF) | ((int(y) & 0xFFFF) << 16)
+
+
+def unpack(p):
+    p = int(p)
+    x, y = p & 0xFFFF, p >> 16
+    return (x - 0x10000 if x >= 0x8000 else x, y - 0x10000 if y >= 0x8000 else y)
+
+
+def grid_rect(size_x, size_y):
+    """Rect of a whole size_x x size_y vertex grid in cell coordinates."""
+    return (0, 0, 2 * (size_x - 1), 2 * (size_y - 1))
+
+
+# ------------------------------------------------------------------ inputs
+
+def gen_gaussians(sx, sy, n=32, noise=1e-3, seed=0):
+    out = np.empty((sy, sx), np.float32)
+    check(lib().mscx_gen_gaussians(ptr(out), sx, sy, n, noise, seed))
+    return out
+
+
+def gen_value_noise(sx, sy, octaves=8, lacunarity=2.0, gain=0.5, period=None, seed=0):
+    if period is None:
+        period = max(sx, sy) / 4.0
+    out = np.empty((sy, sx), np.float32)
+    check(lib().mscx_gen_value_noise(ptr(out), sx, sy, octaves, lacunarity, gain, float(period), seed))
+    return out
+
+
+def gen_white(sx, sy, levels=0, seed=0):
+    out = np.empty((sy, sx), np.float32)
+    check(lib().mscx_gen_white(ptr(out), sx, sy, levels, seed))
+    return out
+
+
+# ------------------------------------------------------------------ tiles
+
+class GridDataset:
+    """One tile (rect + ext window) on the GPU; mirrors dmsc GridDataset
+    (grid_dataset.h:35-240).  rect/ext are (x0, y0, x1, y1) cell coords."""
+
+    def __init__(self, rect, ext=None):
+        ext = rect if ext is None else ext
+        self.rect = tuple(int(v) for v in rect)
+        self.ext = tuple(int(v) for v in ext)
+        h = ctypes.c_void_p()
+        r = (ctypes.c_int32 * 4)(*self.rect)
+        e = (ctypes.c_int32 * 4)(*self.ext)
+        check(lib().mscx_tile_create(r, e, ctypes.byref(h)))
+        self._h = h
+        self.cw = self.ext[2] - self.ext[0] + 1
+        self.ch = self.ext[3] - self.ext[1] + 1
+        self.vx = self.cw // 2 + 1
+        self.vy = self.ch // 2 + 1
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                lib().mscx_tile_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def init(self, data):
+        """GridDataset::init: the ext window of vertex values (vy x vx, x fastest)."""
+        a = np.ascontiguousarray(data, np.float32).reshape(-1)
+        check(lib().mscx_tile_upload(self._h, ptr(a), a.size))
+
+    def bind_device(self, dev_ptr, row_pitch):
+        check(lib().mscx_tile_bind_device(self._h, ctypes.c_void_p(dev_ptr), row_pitch))
+
+    def work(self):
+        check(lib().mscx_tile_work(self._h))
+
+    def num_critical(self):
+        k = ctypes.c_uint32()
+        check(lib().mscx_tile_num_critical(self._h, ctypes.byref(k)))
+        return k.value
+
+    def arrays(self, flags=True, pairs=True, own=True, crit=True):
+        """(flags int8, pairs u32, own u32) as (ch, cw) images over ext, and
+        the critical list (K packed ids)."""
+        n = self.cw * self.ch
+        f = np.empty(n, np.int8) if flags else None
+        p = np.empty(n, np.uint32) if pairs else None
+        o = np.empty(n, np.uint32) if own else None
+        c = np.empty(self.num_critical(), np.uint32) if crit else None
+        check(lib().mscx_tile_download(self._h, ptr(f), ptr(p), ptr(o), ptr(c)))
+        shp = (self.ch, self.cw)
+        return (None if f is None else f.reshape(shp), None if p is None else p.reshape(shp),
+                None if o is None else o.reshape(shp), c)
+
+    def codes(self):
+        c = np.empty(self.cw * self.ch, np.uint8)
+        check(lib().mscx_tile_download_codes(self._h, ptr(c)))
+        return c.reshape(self.ch, self.cw)
+
+    def critical_data(self):
+        k = self.num_critical()
+        fn = np.empty(k, np.float32)
+        conn = np.empty(4 * k, np.uint32)
+        pr = np.empty(k, np.uint32)
+        check(lib().mscx_tile_download_critical(self._h, ptr(fn), ptr(conn), ptr(pr)))
+        return fn, conn.reshape(k, 4), pr
+
+    def fill_discs(self, msc):
+        """GridDataset::postMergeFillDiscs into `msc`."""
+        check(lib().mscx_tile_fill_discs(self._h, msc.handle))
+
+
+def work_tiles(tiles):
+    """Batched work() over a list of GridDataset (one launch per stage)."""
+    arr = (ctypes.c_void_p * len(tiles))(*[t.handle.value for t in tiles])
+    check(lib().mscx_tiles_work(arr, len(tiles)))
+
+
+# ------------------------------------------------------------------ complexes
+
+class GridMSComplex:
+    """Host-resident MS complex; mirrors dmsc GridMSComplex."""
+
+    def __init__(self, handle, owned=True):
+        self._h = handle
+        self._owned = owned
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if getattr(self, "_owned", False) and h and h.value:
+            try:
+                lib().mscx_complex_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def from_dataset(cls, ds):
+        """GridDataset::writeout_connectivity into a new complex."""
+        h = ctypes.c_void_p()
+        check(lib().mscx_complex_from_tile(ds.handle, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def merge_up(cls, a, b):
+        h = ctypes.c_void_p()
+        check(lib().mscx_complex_merge_up(a.handle, b.handle, ctypes.byref(h)))
+        return cls(h)
+
+    def merge_down(self, a, b):
+        check(lib().mscx_complex_merge_down(self._h, a.handle, b.handle))
+
+    def simplify_un_simplify(self, threshold):
+        check(lib().mscx_complex_simplify_un_simplify(self._h, float(threshold)))
+
+    def write_discs(self, prefix):
+        check(lib().mscx_complex_write_discs(self._h, prefix.encode()))
+
+    def num_cps(self):
+        n = ctypes.c_uint32()
+        check(lib().mscx_complex_num_cps(self._h, ctypes.byref(n)))
+        return n.value
+
+    def dump(self):
+        """Canonical uint32 dump (format in include/mscx.h)."""
+        n = ctypes.c_size_t()
+        check(lib().mscx_complex_dump(self._h, None, 0, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint32)
+        check(lib().mscx_complex_dump(self._h, ptr(buf), buf.size, ctypes.byref(n)))
+        return buf
+
+
+class GridRun:
+    """A whole GridDataManager compute run over one grid (mscx_grid_run)."""
+
+    def __init__(self, data, levels=1, lx=0, ly=0, threshold=0.0, discs=True, device_ptr=None, size=None):
+        h = ctypes.c_void_p()
+        flags = 0 if discs else 2
+        if device_ptr is not None:
+            sx, sy = size
+            flags |= 1
+            p = ctypes.c_void_p(device_ptr)
+        else:
+            data = np.ascontiguousarray(data, np.float32)
+            sy, sx = data.shape
+            p = ptr(data)
+        check(lib().mscx_grid_run(p, sx, sy, levels, lx, ly, float(threshold), flags, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                lib().mscx_grid_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def num_pieces(self):
+        n = ctypes.c_int32()
+        check(lib().mscx_grid_num_pieces(self._h, ctypes.byref(n)))
+        return n.value
+
+    def piece(self, i):
+        h = ctypes.c_void_p()
+        check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
+        return GridMSComplex(h, owned=False)
+
+    def times(self):
+        t = np.zeros(5, np.float64)
+        check(lib().mscx_grid_times(self._h, ptr(t)))
+        return t

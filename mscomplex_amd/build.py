@@ -1,0 +1,68 @@
+"""Build libmscx.so (HIP kernels for gfx950 + C++ host runtime) in-tree.
+
+Usage: python -m mscomplex_amd.build   (or mscomplex_amd.build.build())
+"""
+import concurrent.futures as _cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmscx.so")
+ARCH = os.environ.get("MSCX_OFFLOAD_ARCH", "gfx950")
+
+# Packed-FP32 instructions are disabled for all device code.
+HIP_FLAGS = [
+    "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
+]
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
+              "-Wno-unused-command-line-argument"]
+
+DEVICE_SOURCES = ["kernels.hip"]
+HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "pipeline.cpp", "gen.cpp"]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.isabs(c) and os.path.exists(c) or not os.path.isabs(c)):
+            return c
+    return "hipcc"
+
+
+def _compile(src, device):
+    out = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if device:
+        cmd = [_hipcc(), *HIP_FLAGS, "-c", src, "-o", out]
+    else:
+        cmd = [_hipcc(), "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(verbose=False):
+    """Compile every source and link mscomplex_amd/libmscx.so (always rebuilds)."""
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = [(os.path.join(CSRC, s), True) for s in DEVICE_SOURCES]
+    jobs += [(os.path.join(CSRC, s), False) for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with _cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(lambda j: _compile(*j), jobs))
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,79 @@
+// Error handling and device runtime shared by the C-ABI implementation.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/mscx.h"
+
+namespace mscx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &m);
+
+#define MSCX_HIP(x)                                                                                       \
+  do {                                                                                                    \
+    hipError_t e_ = (x);                                                                                  \
+    if (e_ != hipSuccess)                                                                                 \
+      throw ::mscx::Error(MSCX_EDEVICE, std::string(#x) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define MSCX_REQUIRE(cond, code, msg) \
+  do {                                \
+    if (!(cond)) throw ::mscx::Error((code), (msg)); \
+  } while (0)
+
+// Runs f, converting exceptions to a status code and the thread's last error.
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return MSCX_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return MSCX_ENOMEM;
+  } catch (const std::logic_error &e) {
+    set_last_error(e.what());
+    return MSCX_ELOGIC;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return MSCX_EDEVICE;
+  }
+}
+
+// Device state: the selected device, the work stream and a mutex that
+// serialises batched launches issued from several host threads.
+struct Runtime {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+};
+Runtime &runtime();
+void ensure_init();
+
+// Growable device scratch and pinned host staging used by batched launches.
+struct Scratch {
+  void *dev = nullptr;
+  size_t dev_bytes = 0;
+  void *host = nullptr;
+  size_t host_bytes = 0;
+  void *reserve_dev(size_t n);
+  void *reserve_host(size_t n);
+  ~Scratch();
+};
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace mscx

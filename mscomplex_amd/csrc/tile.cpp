@@ -1,0 +1,338 @@
+// mscx_tile implementation and the batched per-stage launcher.
+#include "tile.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+
+namespace mscx {
+
+static Scratch g_batch;  // descriptors, block prefixes, K array, flags
+
+static void check_rects(const int32_t *rect, const int32_t *ext) {
+  for (int k = 0; k < 4; ++k) {
+    MSCX_REQUIRE(rect[k] >= 0 && ext[k] >= 0 && rect[k] <= 32766 && ext[k] <= 32766, MSCX_EINVAL,
+                 "rect/ext coordinates must lie in [0, 32766]");
+    MSCX_REQUIRE((rect[k] & 1) == 0 && (ext[k] & 1) == 0, MSCX_EINVAL,
+                 "rect/ext corners must be vertices (even cell coordinates)");
+  }
+  MSCX_REQUIRE(ext[2] - ext[0] >= 2 && ext[3] - ext[1] >= 2, MSCX_EINVAL, "ext must span at least 2x2 vertices");
+  MSCX_REQUIRE(rect[2] - rect[0] >= 2 && rect[3] - rect[1] >= 2, MSCX_EINVAL,
+               "rect must span at least 2x2 vertices");
+  MSCX_REQUIRE(rect[0] >= ext[0] && rect[1] >= ext[1] && rect[2] <= ext[2] && rect[3] <= ext[3], MSCX_EINVAL,
+               "ext must contain rect");
+}
+
+static void alloc_fixed(mscx_tile *t) {
+  TileDesc &d = t->d;
+  TileGeom &g = d.g;
+  g.vx = (t->ext[2] - t->ext[0]) / 2 + 1;
+  g.vy = (t->ext[3] - t->ext[1]) / 2 + 1;
+  g.cw = 2 * g.vx - 1;
+  g.ch = 2 * g.vy - 1;
+  g.rx0 = t->rect[0] - t->ext[0];
+  g.ry0 = t->rect[1] - t->ext[1];
+  g.rx1 = t->rect[2] - t->ext[0];
+  g.ry1 = t->rect[3] - t->ext[1];
+  d.ox = t->ext[0];
+  d.oy = t->ext[1];
+  d.nseg = (g.cw + SEG - 1) / SEG;
+  d.pitch = d.nseg * SEG;
+  d.own_pitch = d.pitch;
+  const int64_t nrs = (int64_t)g.ch * d.nseg;
+  const int64_t nparts = (nrs + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  const int64_t nv = (int64_t)g.vx * g.vy, nq = (int64_t)(g.vx - 1) * (g.vy - 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  size_t o_codes = take((size_t)g.ch * d.pitch);
+  size_t o_cnt = take(nrs * 4), o_offs = take(nrs * 4), o_part = take(nparts * 4 + 4);
+  size_t o_vptr = take(nv * 4), o_qptr = take(nq * 4 + 4);
+  size_t o_own = take((size_t)g.ch * d.own_pitch * 4);
+  t->fixed_bytes = off;
+  MSCX_HIP(hipMalloc(&t->fixed, off));
+  char *base = (char *)t->fixed;
+  d.codes = (uint8_t *)(base + o_codes);
+  d.seg_counts = (uint32_t *)(base + o_cnt);
+  d.seg_offs = (uint32_t *)(base + o_offs);
+  d.scan_part = (uint32_t *)(base + o_part);
+  d.vptr = (int32_t *)(base + o_vptr);
+  d.qptr = (int32_t *)(base + o_qptr);
+  d.own = (uint32_t *)(base + o_own);
+}
+
+static void ensure_kcap(mscx_tile *t, size_t K) {
+  if (K <= t->kcap && t->crit_mem) return;
+  if (t->crit_mem) MSCX_HIP(hipFree(t->crit_mem));
+  t->crit_mem = nullptr;
+  size_t cap = K < 64 ? 64 : K + K / 8;
+  // crit (4) + fn (4) + conn (16) + pair (4)
+  size_t o_fn = align_up(cap * 4, 256), o_conn = o_fn + align_up(cap * 4, 256),
+         o_pair = o_conn + align_up(cap * 16, 256), total = o_pair + align_up(cap * 4, 256);
+  MSCX_HIP(hipMalloc(&t->crit_mem, total));
+  char *b = (char *)t->crit_mem;
+  t->d.crit = (uint32_t *)b;
+  t->d.crit_fn = (float *)(b + o_fn);
+  t->d.crit_conn = (uint32_t *)(b + o_conn);
+  t->d.crit_pair = (uint32_t *)(b + o_pair);
+  t->kcap = cap;
+}
+
+void run_tiles(mscx_tile *const *tiles, int n) {
+  if (n <= 0) return;
+  ensure_init();
+  Runtime &rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mu);
+  hipStream_t s = rt.stream;
+  for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i] && tiles[i]->has_input, MSCX_ESTATE, "tile has no input bound");
+
+  // Layout of the batch scratch: descriptors | prefixes (KB_COUNT x (n+1)) | K[n] | changed
+  const size_t o_desc = 0;
+  const size_t o_pref = align_up(sizeof(TileDesc) * n, 256);
+  const size_t o_k = align_up(o_pref + sizeof(int) * KB_COUNT * (n + 1), 256);
+  const size_t o_chg = align_up(o_k + 4 * n, 256);
+  const size_t total = o_chg + 256;
+  char *dev = (char *)g_batch.reserve_dev(total);
+  char *host = (char *)g_batch.reserve_host(total);
+  TileDesc *hd = (TileDesc *)(host + o_desc);
+  int *hp = (int *)(host + o_pref);
+  uint32_t *hk = (uint32_t *)(host + o_k);
+  TileDesc *dd = (TileDesc *)(dev + o_desc);
+  int *dp = (int *)(dev + o_pref);
+  uint32_t *dk = (uint32_t *)(dev + o_k);
+  uint32_t *dchg = (uint32_t *)(dev + o_chg);
+  uint32_t *hchg = (uint32_t *)(host + o_chg);
+
+  for (int i = 0; i < n; ++i) {
+    hd[i] = tiles[i]->d;
+    hd[i].nkrit = dk + i;
+    hd[i].kcap = 0;
+  }
+  int nb[KB_COUNT];
+  for (int k = 0; k < KB_CRIT; ++k) nb[k] = plan_blocks(k, hd, n, hp + k * (n + 1));
+  MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
+
+  launch_gradient(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
+  launch_scan(dd, dp + KB_SCAN * (n + 1), n, nb[KB_SCAN], s);
+  MSCX_HIP(hipGetLastError());
+  MSCX_HIP(hipMemcpyAsync(hk, dk, 4 * n, hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipStreamSynchronize(s));
+
+  for (int i = 0; i < n; ++i) {
+    ensure_kcap(tiles[i], hk[i]);
+    tiles[i]->K = hk[i];
+    hd[i].crit = tiles[i]->d.crit;
+    hd[i].crit_fn = tiles[i]->d.crit_fn;
+    hd[i].crit_conn = tiles[i]->d.crit_conn;
+    hd[i].crit_pair = tiles[i]->d.crit_pair;
+    hd[i].kcap = hk[i];
+  }
+  nb[KB_CRIT] = plan_blocks(KB_CRIT, hd, n, hp + KB_CRIT * (n + 1));
+  MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
+
+  launch_compact(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
+  // Pointer jumping: rounds in groups of 4 between convergence checks.
+  uint32_t rounds = 0;
+  for (;;) {
+    MSCX_HIP(hipMemsetAsync(dchg, 0, 4, s));
+    for (int r = 0; r < 4; ++r) launch_jump(dd, dp + KB_JUMP * (n + 1), n, nb[KB_JUMP], dchg, s);
+    rounds += 4;
+    MSCX_HIP(hipMemcpyAsync(hchg, dchg, 4, hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipStreamSynchronize(s));
+    if (*hchg == 0) break;
+    MSCX_REQUIRE(rounds < 256, MSCX_ELOGIC, "trace did not converge");
+  }
+  launch_owner(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
+  launch_incidence(dd, dp + KB_CRIT * (n + 1), n, nb[KB_CRIT], s);
+  MSCX_HIP(hipGetLastError());
+  MSCX_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    tiles[i]->worked = true;
+    tiles[i]->jump_rounds = rounds;
+  }
+}
+
+void tile_critical(const mscx_tile *t, std::vector<uint32_t> &crit, std::vector<float> &fn,
+                   std::vector<uint32_t> &conn4, std::vector<uint32_t> &pair) {
+  MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
+  hipStream_t s = runtime().stream;
+  size_t K = t->K;
+  crit.resize(K);
+  fn.resize(K);
+  conn4.resize(4 * K);
+  pair.resize(K);
+  if (K) {
+    MSCX_HIP(hipMemcpyAsync(crit.data(), t->d.crit, K * 4, hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipMemcpyAsync(fn.data(), t->d.crit_fn, K * 4, hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipMemcpyAsync(conn4.data(), t->d.crit_conn, K * 16, hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipMemcpyAsync(pair.data(), t->d.crit_pair, K * 4, hipMemcpyDeviceToHost, s));
+  }
+  MSCX_HIP(hipStreamSynchronize(s));
+}
+
+void tile_codes(const mscx_tile *t, std::vector<uint8_t> &codes) {
+  MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
+  const TileGeom &g = t->d.g;
+  codes.resize((size_t)g.cw * g.ch);
+  MSCX_HIP(hipMemcpy2DAsync(codes.data(), g.cw, t->d.codes, t->d.pitch, g.cw, g.ch, hipMemcpyDeviceToHost,
+                            runtime().stream));
+  MSCX_HIP(hipStreamSynchronize(runtime().stream));
+}
+
+void tile_owners(const mscx_tile *t, std::vector<uint32_t> &own) {
+  MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
+  const TileGeom &g = t->d.g;
+  own.resize((size_t)g.cw * g.ch);
+  MSCX_HIP(hipMemcpy2DAsync(own.data(), (size_t)g.cw * 4, t->d.own, (size_t)t->d.own_pitch * 4, (size_t)g.cw * 4,
+                            g.ch, hipMemcpyDeviceToHost, runtime().stream));
+  MSCX_HIP(hipStreamSynchronize(runtime().stream));
+}
+
+}  // namespace mscx
+
+using namespace mscx;
+
+extern "C" {
+
+int mscx_tile_create(const int32_t rect[4], const int32_t ext[4], mscx_tile **out) {
+  return guarded([&] {
+    MSCX_REQUIRE(rect && ext && out, MSCX_EINVAL, "null argument");
+    check_rects(rect, ext);
+    ensure_init();
+    mscx_tile *t = new mscx_tile;
+    memcpy(t->rect, rect, 16);
+    memcpy(t->ext, ext, 16);
+    try {
+      alloc_fixed(t);
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+int mscx_tile_destroy(mscx_tile *t) {
+  return guarded([&] {
+    if (!t) return;
+    MSCX_HIP(hipStreamSynchronize(runtime().stream));
+    if (t->fixed) MSCX_HIP(hipFree(t->fixed));
+    if (t->verts_owned) MSCX_HIP(hipFree(t->verts_owned));
+    if (t->crit_mem) MSCX_HIP(hipFree(t->crit_mem));
+    delete t;
+  });
+}
+
+int mscx_tile_upload(mscx_tile *t, const float *data, size_t n) {
+  return guarded([&] {
+    MSCX_REQUIRE(t && data, MSCX_EINVAL, "null argument");
+    const TileGeom &g = t->d.g;
+    size_t need = (size_t)g.vx * g.vy;
+    MSCX_REQUIRE(n == need, MSCX_EINVAL, "input size does not match the ext window");
+    for (size_t i = 0; i < n; ++i) MSCX_REQUIRE(!std::isnan(data[i]), MSCX_EINVAL, "input contains NaN");
+    if (t->verts_cap < need) {
+      if (t->verts_owned) MSCX_HIP(hipFree(t->verts_owned));
+      t->verts_owned = nullptr;
+      MSCX_HIP(hipMalloc(&t->verts_owned, need * 4));
+      t->verts_cap = need;
+    }
+    MSCX_HIP(hipMemcpyAsync(t->verts_owned, data, need * 4, hipMemcpyHostToDevice, runtime().stream));
+    MSCX_HIP(hipStreamSynchronize(runtime().stream));
+    t->d.verts = t->verts_owned;
+    t->d.vpitch = g.vx;
+    t->has_input = true;
+    t->worked = false;
+  });
+}
+
+int mscx_tile_bind_device(mscx_tile *t, const float *dev_ptr, int64_t row_pitch) {
+  return guarded([&] {
+    MSCX_REQUIRE(t && dev_ptr, MSCX_EINVAL, "null argument");
+    MSCX_REQUIRE(row_pitch >= t->d.g.vx, MSCX_EINVAL, "row pitch smaller than the window width");
+    t->d.verts = dev_ptr;
+    t->d.vpitch = row_pitch;
+    t->has_input = true;
+    t->worked = false;
+  });
+}
+
+int mscx_tile_work(mscx_tile *t) {
+  return guarded([&] {
+    MSCX_REQUIRE(t, MSCX_EINVAL, "null tile");
+    run_tiles(&t, 1);
+  });
+}
+
+int mscx_tiles_work(mscx_tile *const *tiles, int n) {
+  return guarded([&] {
+    MSCX_REQUIRE(tiles && n >= 0, MSCX_EINVAL, "bad tile array");
+    run_tiles(tiles, n);
+  });
+}
+
+int mscx_tile_num_critical(const mscx_tile *t, uint32_t *K) {
+  return guarded([&] {
+    MSCX_REQUIRE(t && K, MSCX_EINVAL, "null argument");
+    MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
+    *K = t->K;
+  });
+}
+
+int mscx_tile_download_codes(const mscx_tile *t, uint8_t *codes) {
+  return guarded([&] {
+    MSCX_REQUIRE(t && codes, MSCX_EINVAL, "null argument");
+    std::vector<uint8_t> c;
+    tile_codes(t, c);
+    memcpy(codes, c.data(), c.size());
+  });
+}
+
+int mscx_tile_download(const mscx_tile *t, int8_t *flags, uint32_t *pairs, uint32_t *own, uint32_t *crit) {
+  return guarded([&] {
+    MSCX_REQUIRE(t, MSCX_EINVAL, "null tile");
+    const TileGeom &g = t->d.g;
+    if (flags || pairs) {
+      std::vector<uint8_t> c;
+      tile_codes(t, c);
+      for (int y = 0; y < g.ch; ++y)
+        for (int x = 0; x < g.cw; ++x) {
+          size_t i = (size_t)y * g.cw + x;
+          uint8_t code = c[i];
+          if (flags) flags[i] = (int8_t)(code & 3);
+          if (pairs) {
+            int d = code_dir(code);
+            pairs[i] = d ? ((uint32_t)(uint16_t)(t->ext[0] + x + dir_dx(d)) |
+                            ((uint32_t)(uint16_t)(t->ext[1] + y + dir_dy(d)) << 16))
+                         : 0xFFFFFFFFu;
+          }
+        }
+    }
+    if (own) {
+      std::vector<uint32_t> o;
+      tile_owners(t, o);
+      memcpy(own, o.data(), o.size() * 4);
+    }
+    if (crit && t->K) {
+      MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
+      MSCX_HIP(hipMemcpy(crit, t->d.crit, (size_t)t->K * 4, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int mscx_tile_download_critical(const mscx_tile *t, float *fn, uint32_t *conn4, uint32_t *pair_idx) {
+  return guarded([&] {
+    MSCX_REQUIRE(t, MSCX_EINVAL, "null tile");
+    std::vector<uint32_t> c, cn, p;
+    std::vector<float> f;
+    tile_critical(t, c, f, cn, p);
+    if (fn) memcpy(fn, f.data(), f.size() * 4);
+    if (conn4) memcpy(conn4, cn.data(), cn.size() * 4);
+    if (pair_idx) memcpy(pair_idx, p.data(), p.size() * 4);
+  });
+}
+
+}  // extern "C"

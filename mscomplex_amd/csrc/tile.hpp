@@ -1,0 +1,38 @@
+// mscx_tile: one GridDataset (rect + ext window) resident in HBM.
+#pragma once
+
+#include <vector>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+struct mscx_tile {
+  int32_t rect[4];
+  int32_t ext[4];
+  mscx::TileDesc d{};     // device pointers and geometry
+  void *fixed = nullptr;  // one allocation: codes, counts, offsets, trace pointers, owners
+  size_t fixed_bytes = 0;
+  float *verts_owned = nullptr;
+  size_t verts_cap = 0;
+  void *crit_mem = nullptr;  // crit, fn, conn (x4), pair
+  size_t kcap = 0;
+  uint32_t K = 0;
+  bool has_input = false;
+  bool worked = false;
+  uint32_t jump_rounds = 0;
+};
+
+namespace mscx {
+
+// Runs gradient, collation, trace, owners and incidence for n tiles with one
+// launch per stage, then synchronises.  Sets K / worked on every tile.
+void run_tiles(mscx_tile *const *tiles, int n);
+
+// Host copies of a tile's critical data after work (K entries each).
+void tile_critical(const mscx_tile *t, std::vector<uint32_t> &crit, std::vector<float> &fn,
+                   std::vector<uint32_t> &conn4, std::vector<uint32_t> &pair);
+// Host copies of a tile's codes and owners over ext, x fastest (unpitched).
+void tile_codes(const mscx_tile *t, std::vector<uint8_t> &codes);
+void tile_owners(const mscx_tile *t, std::vector<uint32_t> &own);
+
+}  // namespace mscx
