@@ -42,6 +42,23 @@ int mscx_shutdown(void);
 const char *mscx_last_error(void);
 int mscx_version(void);
 
+/* Per-stage device timing (no reference counterpart; replaces the
+ * reference's _LOG_TIMINGS wall-clock macros, grid_dataset.cpp:19-34).
+ * mscx_profile(1) resets and enables HIP-event timing of each stage on the
+ * work stream; mscx_profile_read returns accumulated milliseconds and event
+ * pair counts for stages 0 gradient, 1 scan, 2 compaction, 3 trace,
+ * 4 owners, 5 incidence, 6 host complex build. */
+int mscx_profile(int enable);
+int mscx_profile_read(double *ms, uint32_t *counts, int n);
+
+/* Device buffers (no reference counterpart: the reference keeps inputs in
+ * host memory).  Lets a caller keep grids resident in HBM and bind tiles to
+ * them with mscx_tile_bind_device. */
+int mscx_dev_alloc(size_t bytes, void **out);
+int mscx_dev_free(void *p);
+int mscx_dev_upload(void *dst, const void *src, size_t bytes);
+int mscx_dev_download(void *dst, const void *src, size_t bytes);
+
 /* ---- tiles: GridDataset ------------------------------------------------- */
 
 /* GridDataset(const rect_t &r, const rect_t &e) (grid_dataset.h:97,
@@ -91,6 +108,14 @@ int mscx_tile_download_critical(const mscx_tile *t, float *fn, uint32_t *conn4, 
  * complex from a tile after mscx_tile_work. */
 int mscx_complex_from_tile(const mscx_tile *t, mscx_complex **out);
 
+/* GridDataset::writeout_connectivity_ocl counterpart (grid_dataset.cpp:
+ * 1250-1283): builds a complex from per-critical-cell data in the layout of
+ * mscx_tile_download_critical (K packed ids in raster order, K values, 4K
+ * incident owner indices, K pair indices).  Host only. */
+int mscx_complex_from_critical(const int32_t rect[4], const int32_t ext[4], uint32_t K, const uint32_t *crit,
+                               const float *fn, const uint32_t *conn4, const uint32_t *pair_idx,
+                               mscx_complex **out);
+
 /* GridMSComplex::merge_up (grid_mscomplex.cpp:24-156); the caller owns *out. */
 int mscx_complex_merge_up(const mscx_complex *a, const mscx_complex *b, mscx_complex **out);
 
@@ -103,6 +128,11 @@ int mscx_complex_simplify_un_simplify(mscx_complex *c, double threshold);
 /* GridDataset::postMergeFillDiscs (grid_dataset.cpp:1044-1231): fills the
  * manifold discs of the tile's complex from the tile's owner image. */
 int mscx_tile_fill_discs(mscx_tile *t, mscx_complex *c);
+
+/* postMergeFillDiscs from host copies of a tile's gradient codes and owner
+ * image over its ext rect (x fastest); own is modified in place. */
+int mscx_complex_fill_discs_host(mscx_complex *c, const int32_t rect[4], const int32_t ext[4],
+                                 const uint8_t *codes, uint32_t *own);
 
 /* GridMSComplex::write_discs (grid_mscomplex.cpp:542-590): appends raw
  * packed cell ids to <prefix>asc_(x,y) / <prefix>des_(x,y). */

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib
 from ._lib import MscxError, check, lib, ptr
 
-__all__ = ["GridDataset", "GridMSComplex", "GridRun", "work_tiles", "gen_gaussians", "gen_value_noise",
+__all__ = ["DeviceBuffer", "profile", "profile_read", "GridDataset", "GridMSComplex", "GridRun", "work_tiles", "gen_gaussians", "gen_value_noise",
            "gen_white", "MscxError", "unpack", "pack", "grid_rect"]
 
 
@@ -51,6 +51,55 @@ def gen_white(sx, sy, levels=0, seed=0):
     out = np.empty((sy, sx), np.float32)
     check(lib().mscx_gen_white(ptr(out), sx, sy, levels, seed))
     return out
+
+
+# ------------------------------------------------------------------ device buffers
+
+class DeviceBuffer:
+    """A HBM allocation owned by libmscx (keeps grids resident on the GPU)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().mscx_dev_alloc(nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib().mscx_dev_upload(ctypes.c_void_p(b.ptr), ptr(a), a.nbytes))
+        return b
+
+    def download(self, out):
+        check(lib().mscx_dev_download(ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mscx_dev_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def profile(enable=True):
+    """Enable (and reset) per-stage HIP-event timing inside the library."""
+    check(lib().mscx_profile(1 if enable else 0))
+
+
+STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex")
+
+
+def profile_read():
+    ms = np.zeros(len(STAGES), np.float64)
+    cnt = np.zeros(len(STAGES), np.uint32)
+    check(lib().mscx_profile_read(ptr(ms), ptr(cnt), len(STAGES)))
+    return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(STAGES)}
 
 
 # ------------------------------------------------------------------ tiles
@@ -167,6 +216,31 @@ class GridMSComplex:
         h = ctypes.c_void_p()
         check(lib().mscx_complex_from_tile(ds.handle, ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def from_critical(cls, rect, ext, crit, fn, conn4, pair_idx):
+        """writeout_connectivity_ocl counterpart: a complex from per-critical
+        data (raster-ordered packed ids, values, 4 incident owner indices per
+        cell, pair indices)."""
+        crit = np.ascontiguousarray(crit, np.uint32)
+        fn = np.ascontiguousarray(fn, np.float32)
+        conn4 = np.ascontiguousarray(conn4, np.uint32).reshape(-1)
+        pair_idx = np.ascontiguousarray(pair_idx, np.uint32)
+        h = ctypes.c_void_p()
+        r = (ctypes.c_int32 * 4)(*rect)
+        e = (ctypes.c_int32 * 4)(*ext)
+        check(lib().mscx_complex_from_critical(r, e, crit.size, ptr(crit), ptr(fn), ptr(conn4), ptr(pair_idx),
+                                               ctypes.byref(h)))
+        return cls(h)
+
+    def fill_discs_host(self, rect, ext, codes, own):
+        """postMergeFillDiscs from host copies of a tile's codes and owners."""
+        codes = np.ascontiguousarray(codes, np.uint8)
+        own = np.ascontiguousarray(own, np.uint32).copy()
+        r = (ctypes.c_int32 * 4)(*rect)
+        e = (ctypes.c_int32 * 4)(*ext)
+        check(lib().mscx_complex_fill_discs_host(self._h, r, e, ptr(codes), ptr(own)))
+        return own
 
     @classmethod
     def merge_up(cls, a, b):

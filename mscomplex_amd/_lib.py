@@ -23,6 +23,12 @@ _SIGS = {
     "mscx_shutdown": (c_int, []),
     "mscx_last_error": (ctypes.c_char_p, []),
     "mscx_version": (c_int, []),
+    "mscx_profile": (c_int, [c_int]),
+    "mscx_profile_read": (c_int, [P, P, c_int]),
+    "mscx_dev_alloc": (c_int, [c_size_t, ctypes.POINTER(P)]),
+    "mscx_dev_free": (c_int, [P]),
+    "mscx_dev_upload": (c_int, [P, P, c_size_t]),
+    "mscx_dev_download": (c_int, [P, P, c_size_t]),
     "mscx_tile_create": (c_int, [P, P, ctypes.POINTER(P)]),
     "mscx_tile_destroy": (c_int, [P]),
     "mscx_tile_upload": (c_int, [P, P, c_size_t]),
@@ -34,6 +40,8 @@ _SIGS = {
     "mscx_tile_download_codes": (c_int, [P, P]),
     "mscx_tile_download_critical": (c_int, [P, P, P, P]),
     "mscx_complex_from_tile": (c_int, [P, ctypes.POINTER(P)]),
+    "mscx_complex_from_critical": (c_int, [P, P, c_uint32, P, P, P, P, ctypes.POINTER(P)]),
+    "mscx_complex_fill_discs_host": (c_int, [P, P, P, P, P]),
     "mscx_complex_merge_up": (c_int, [P, P, ctypes.POINTER(P)]),
     "mscx_complex_merge_down": (c_int, [P, P, P]),
     "mscx_complex_simplify_un_simplify": (c_int, [P, c_double]),

@@ -1,0 +1,212 @@
+// Host-resident Morse-Smale complex (the product's GridMSComplex).
+//
+// Flat arrays per critical point instead of the reference's heap-allocated
+// critical_point objects; arcs are sorted small multisets (iteration order
+// and multiplicities identical to std::multiset<uint>); cell id -> index is
+// an open-addressing hash map.  The algorithms follow
+// dmsc/src/grid_mscomplex.cpp and dmsc/include/discreteMorseAlgorithm.h
+// step for step, because tie orders, arc multiplicities and the
+// strangulation flags they set are observable in the output.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct mscx_tile;
+
+namespace mscx {
+
+// Sorted multiset of uint32 with inline storage for small sizes.
+class ConnSet {
+ public:
+  ConnSet() {}
+  ConnSet(const ConnSet &o) { assign(o); }
+  ConnSet &operator=(const ConnSet &o) {
+    if (this != &o) assign(o);
+    return *this;
+  }
+  ConnSet(ConnSet &&o) noexcept { steal(o); }
+  ConnSet &operator=(ConnSet &&o) noexcept {
+    if (this != &o) {
+      release();
+      steal(o);
+    }
+    return *this;
+  }
+  ~ConnSet() { release(); }
+
+  uint32_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  const uint32_t *begin() const { return data(); }
+  const uint32_t *end() const { return data() + n_; }
+  uint32_t operator[](uint32_t i) const { return data()[i]; }
+
+  void clear() { n_ = 0; }
+  // multiset::insert: after any equal elements
+  void insert(uint32_t v) {
+    reserve(n_ + 1);
+    uint32_t *d = data();
+    uint32_t pos = upper(v);
+    memmove(d + pos + 1, d + pos, (n_ - pos) * 4);
+    d[pos] = v;
+    ++n_;
+  }
+  // multiset::erase(key): removes every element equal to v
+  void erase_all(uint32_t v) {
+    uint32_t *d = data();
+    uint32_t lo = lower(v), hi = upper(v);
+    if (lo == hi) return;
+    memmove(d + lo, d + hi, (n_ - hi) * 4);
+    n_ -= hi - lo;
+  }
+  uint32_t count(uint32_t v) const { return upper(v) - lower(v); }
+  bool contains(uint32_t v) const {
+    uint32_t l = lower(v);
+    return l < n_ && data()[l] == v;
+  }
+  // append a value known to be >= every element (bulk build)
+  void push_sorted(uint32_t v) {
+    reserve(n_ + 1);
+    data()[n_++] = v;
+  }
+
+ private:
+  static constexpr uint32_t INL = 6;
+  uint32_t n_ = 0, cap_ = INL;
+  union {
+    uint32_t inl_[INL];
+    uint32_t *heap_;
+  };
+  uint32_t *data() { return cap_ > INL ? heap_ : inl_; }
+  const uint32_t *data() const { return cap_ > INL ? heap_ : inl_; }
+  uint32_t lower(uint32_t v) const {
+    const uint32_t *d = data();
+    uint32_t lo = 0, hi = n_;
+    while (lo < hi) {
+      uint32_t m = (lo + hi) >> 1;
+      if (d[m] < v) lo = m + 1; else hi = m;
+    }
+    return lo;
+  }
+  uint32_t upper(uint32_t v) const {
+    const uint32_t *d = data();
+    uint32_t lo = 0, hi = n_;
+    while (lo < hi) {
+      uint32_t m = (lo + hi) >> 1;
+      if (d[m] <= v) lo = m + 1; else hi = m;
+    }
+    return lo;
+  }
+  void reserve(uint32_t want) {
+    if (want <= cap_) return;
+    uint32_t nc = cap_ * 2;
+    if (nc < want) nc = want;
+    uint32_t *nd = new uint32_t[nc];
+    memcpy(nd, data(), n_ * 4);
+    if (cap_ > INL) delete[] heap_;
+    heap_ = nd;
+    cap_ = nc;
+  }
+  void release() {
+    if (cap_ > INL) delete[] heap_;
+    cap_ = INL;
+    n_ = 0;
+  }
+  void assign(const ConnSet &o) {
+    n_ = 0;
+    reserve(o.n_);
+    memcpy(data(), o.data(), o.n_ * 4);
+    n_ = o.n_;
+  }
+  void steal(ConnSet &o) {
+    n_ = o.n_;
+    cap_ = o.cap_;
+    if (o.cap_ > INL)
+      heap_ = o.heap_;
+    else
+      memcpy(inl_, o.inl_, sizeof(inl_));
+    o.cap_ = INL;
+    o.n_ = 0;
+  }
+};
+
+// packed cell id (x | y << 16) -> critical-point index
+class CellMap {
+ public:
+  void reserve(size_t n);
+  // returns the index, or UINT32_MAX when absent
+  uint32_t find(uint32_t key) const;
+  bool contains(uint32_t key) const { return find(key) != UINT32_MAX; }
+  // map::insert semantics: keeps an existing value
+  void insert(uint32_t key, uint32_t val);
+  // map::operator[] = val semantics: overwrites
+  void set(uint32_t key, uint32_t val);
+  // map::operator[] read semantics: inserts 0 when absent
+  uint32_t get_or_zero(uint32_t key);
+  size_t size() const { return n_; }
+
+ private:
+  std::vector<uint32_t> keys_, vals_;
+  size_t n_ = 0, mask_ = 0;
+  size_t slot(uint32_t key) const;
+  void grow();
+};
+
+enum : uint8_t { CPF_CANCELLED = 1, CPF_STRANGULATION = 2, CPF_BCANC = 4 };
+
+struct Rect {
+  int x0, y0, x1, y1;
+  bool contains(int x, int y) const { return x >= x0 && y >= y0 && x <= x1 && y <= y1; }
+  bool interior(int x, int y) const { return x > x0 && y > y0 && x < x1 && y < y1; }
+  bool on_boundary(int x, int y) const { return contains(x, y) && !interior(x, y); }
+};
+
+inline int cx_of(uint32_t c) { return (int16_t)(c & 0xFFFF); }
+inline int cy_of(uint32_t c) { return (int16_t)(c >> 16); }
+inline uint32_t cell_dim(uint32_t c) { return (c & 1) + ((c >> 16) & 1); }
+
+struct Complex {
+  Rect rect{}, ext{};
+  std::vector<uint32_t> cell;
+  std::vector<float> fn;
+  std::vector<uint32_t> pair;
+  std::vector<uint8_t> flags;
+  std::vector<ConnSet> asc, des;
+  std::vector<std::vector<uint32_t>> asc_disc, des_disc;
+  CellMap idx;
+
+  uint32_t size() const { return (uint32_t)cell.size(); }
+  // shallow_replicate_cp (grid_mscomplex.cpp:7-22)
+  uint32_t replicate(const Complex &src, uint32_t j);
+
+  void dump(std::vector<uint32_t> &out) const;
+};
+
+// GridDataset::writeout_connectivity from a worked tile.
+void complex_from_tile(const mscx_tile *t, Complex &out);
+// Same, from host copies of the tile's critical data.
+void complex_build(Complex &c, const Rect &rect, const Rect &ext, const std::vector<uint32_t> &crit,
+                   const std::vector<float> &fn, const std::vector<uint32_t> &conn4,
+                   const std::vector<uint32_t> &pair);
+
+void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> *new_edges);
+void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2);
+void merge_up(const Complex &a, const Complex &b, Complex &out);
+void merge_down(Complex &parent, Complex &a, Complex &b);
+void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh);
+void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc);
+
+// GridDataset::postMergeFillDiscs from a tile's codes and owner image (host
+// copies over ext, x fastest).
+void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *codes, std::vector<uint32_t> &own);
+
+void write_discs(const Complex &m, const std::string &prefix);
+
+}  // namespace mscx
+
+// Opaque C handle of a complex.
+struct mscx_complex {
+  mscx::Complex c;
+};

@@ -49,6 +49,38 @@ void *Scratch::reserve_host(size_t n) {
   return host;
 }
 
+StageTimer &stage_timer() {
+  static StageTimer t;
+  return t;
+}
+
+void StageTimer::begin(int k, hipStream_t s) {
+  if (!on) return;
+  if (!a[k]) {
+    MSCX_HIP(hipEventCreate(&a[k]));
+    MSCX_HIP(hipEventCreate(&b[k]));
+  }
+  MSCX_HIP(hipEventRecord(a[k], s));
+}
+
+void StageTimer::end(int k, hipStream_t s) {
+  if (!on) return;
+  MSCX_HIP(hipEventRecord(b[k], s));
+  pending[k] = true;
+}
+
+void StageTimer::collect() {
+  if (!on) return;
+  for (int k = 0; k < ST_COUNT; ++k) {
+    if (!pending[k]) continue;
+    float m = 0;
+    MSCX_HIP(hipEventElapsedTime(&m, a[k], b[k]));
+    ms[k] += m;
+    count[k]++;
+    pending[k] = false;
+  }
+}
+
 Scratch::~Scratch() {
   // Buffers live until process exit; the runtime may already be torn down
   // when static destructors run, so nothing is released here.
@@ -74,6 +106,61 @@ int mscx_init(int device) {
     MSCX_HIP(hipSetDevice(device));
     MSCX_HIP(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
     rt.device = device;
+  });
+}
+
+int mscx_profile(int enable) {
+  return mscx::guarded([&] {
+    mscx::StageTimer &t = mscx::stage_timer();
+    t.on = enable != 0;
+    for (int k = 0; k < mscx::ST_COUNT; ++k) {
+      t.ms[k] = 0;
+      t.count[k] = 0;
+      t.pending[k] = false;
+    }
+  });
+}
+
+int mscx_profile_read(double *ms, uint32_t *counts, int n) {
+  return mscx::guarded([&] {
+    mscx::StageTimer &t = mscx::stage_timer();
+    for (int k = 0; k < n && k < mscx::ST_COUNT; ++k) {
+      if (ms) ms[k] = t.ms[k];
+      if (counts) counts[k] = t.count[k];
+    }
+  });
+}
+
+int mscx_dev_alloc(size_t bytes, void **out) {
+  return mscx::guarded([&] {
+    MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
+    mscx::ensure_init();
+    *out = nullptr;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    MSCX_REQUIRE(e == hipSuccess, MSCX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  });
+}
+
+int mscx_dev_free(void *p) {
+  return mscx::guarded([&] {
+    if (!p) return;
+    MSCX_HIP(hipStreamSynchronize(mscx::runtime().stream));
+    MSCX_HIP(hipFree(p));
+  });
+}
+
+int mscx_dev_upload(void *dst, const void *src, size_t bytes) {
+  return mscx::guarded([&] {
+    mscx::ensure_init();
+    MSCX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int mscx_dev_download(void *dst, const void *src, size_t bytes) {
+  return mscx::guarded([&] {
+    mscx::ensure_init();
+    MSCX_HIP(hipStreamSynchronize(mscx::runtime().stream));
+    MSCX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   });
 }
 

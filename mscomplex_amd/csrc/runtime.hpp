@@ -76,4 +76,19 @@ struct Scratch {
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Optional per-stage device timing with HIP events recorded on the work
+// stream around each stage's launches (enabled by mscx_profile).
+enum Stage { ST_GRADIENT = 0, ST_SCAN, ST_COMPACT, ST_TRACE, ST_OWNER, ST_INCIDENCE, ST_HOST_COMPLEX, ST_COUNT };
+struct StageTimer {
+  bool on = false;
+  hipEvent_t a[ST_COUNT] = {}, b[ST_COUNT] = {};
+  bool pending[ST_COUNT] = {};
+  double ms[ST_COUNT] = {};
+  uint32_t count[ST_COUNT] = {};
+  void begin(int k, hipStream_t s);
+  void end(int k, hipStream_t s);
+  void collect();  // call after a stream synchronisation
+};
+StageTimer &stage_timer();
+
 }  // namespace mscx

@@ -116,11 +116,17 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   for (int k = 0; k < KB_CRIT; ++k) nb[k] = plan_blocks(k, hd, n, hp + k * (n + 1));
   MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
 
+  StageTimer &tm = stage_timer();
+  tm.begin(ST_GRADIENT, s);
   launch_gradient(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
+  tm.end(ST_GRADIENT, s);
+  tm.begin(ST_SCAN, s);
   launch_scan(dd, dp + KB_SCAN * (n + 1), n, nb[KB_SCAN], s);
+  tm.end(ST_SCAN, s);
   MSCX_HIP(hipGetLastError());
   MSCX_HIP(hipMemcpyAsync(hk, dk, 4 * n, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipStreamSynchronize(s));
+  tm.collect();
 
   for (int i = 0; i < n; ++i) {
     ensure_kcap(tiles[i], hk[i]);
@@ -134,22 +140,32 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   nb[KB_CRIT] = plan_blocks(KB_CRIT, hd, n, hp + KB_CRIT * (n + 1));
   MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
 
+  tm.begin(ST_COMPACT, s);
   launch_compact(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
+  tm.end(ST_COMPACT, s);
   // Pointer jumping: rounds in groups of 4 between convergence checks.
   uint32_t rounds = 0;
   for (;;) {
     MSCX_HIP(hipMemsetAsync(dchg, 0, 4, s));
+    tm.begin(ST_TRACE, s);
     for (int r = 0; r < 4; ++r) launch_jump(dd, dp + KB_JUMP * (n + 1), n, nb[KB_JUMP], dchg, s);
+    tm.end(ST_TRACE, s);
     rounds += 4;
     MSCX_HIP(hipMemcpyAsync(hchg, dchg, 4, hipMemcpyDeviceToHost, s));
     MSCX_HIP(hipStreamSynchronize(s));
+    tm.collect();
     if (*hchg == 0) break;
     MSCX_REQUIRE(rounds < 256, MSCX_ELOGIC, "trace did not converge");
   }
+  tm.begin(ST_OWNER, s);
   launch_owner(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
+  tm.end(ST_OWNER, s);
+  tm.begin(ST_INCIDENCE, s);
   launch_incidence(dd, dp + KB_CRIT * (n + 1), n, nb[KB_CRIT], s);
+  tm.end(ST_INCIDENCE, s);
   MSCX_HIP(hipGetLastError());
   MSCX_HIP(hipStreamSynchronize(s));
+  tm.collect();
   for (int i = 0; i < n; ++i) {
     tiles[i]->worked = true;
     tiles[i]->jump_rounds = rounds;

@@ -1,0 +1,52 @@
+"""GPU parity of the whole grid pipeline (mscx_grid_run: HIP stages for
+every leaf tile, merge up, simplification, merge down, manifold discs)
+against the oracle's GridDataManager order, word for word for every piece."""
+import numpy as np
+import pytest
+
+import mscomplex_amd as mx
+import oracle
+from tests.helpers import rng_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(data, label, **kw):
+    print(f"[pipeline] {label} shape={data.shape} {kw}", flush=True)
+    orc = oracle.Pipeline(data, keep=True, **kw)
+    run = mx.GridRun(data, levels=kw.get("levels", 0), lx=kw.get("lx", 0), ly=kw.get("ly", 0),
+                     threshold=kw.get("threshold", 0.0))
+    assert run.num_pieces() == orc.num_pieces()
+    for i in range(run.num_pieces()):
+        a = orc.dump(4, i)
+        b = run.piece(i).dump()
+        assert a.size == b.size and np.array_equal(a, b), f"{label}: piece {i} differs"
+
+
+@pytest.mark.parametrize("tiling", [dict(levels=0), dict(levels=1), dict(levels=3), dict(lx=1, ly=1),
+                                    dict(lx=2, ly=2), dict(lx=3, ly=2)])
+@pytest.mark.parametrize("threshold", [0.0, 0.01, 0.1])
+def test_white(tiling, threshold):
+    _compare(rng_field("white", 150, 130, 5), "white", threshold=threshold, **tiling)
+
+
+@pytest.mark.parametrize("threshold", [0.0, 0.05])
+def test_ties(threshold):
+    _compare(rng_field("ties", 90, 70, 6), "ties", lx=1, ly=1, threshold=threshold)
+
+
+def test_config_gaussians_512_l0():
+    # the reference's CPU-runnable configuration: 512^2, 32 Gaussians, threshold 0
+    _compare(mx.gen_gaussians(512, 512, 32, 1e-3, seed=1), "gauss512", levels=0, threshold=0.0)
+
+
+def test_gaussians_512_slabs():
+    _compare(mx.gen_gaussians(512, 512, 32, 1e-3, seed=1), "gauss512-l2", levels=2, threshold=0.01)
+
+
+def test_value_noise_1024_4x4():
+    _compare(mx.gen_value_noise(1024, 1024, period=256, seed=9), "vnoise1024", lx=2, ly=2, threshold=0.01)
+
+
+def test_quantised_white_256():
+    _compare(mx.gen_white(256, 256, levels=256, seed=3), "q256", lx=1, ly=1, threshold=0.005)
