@@ -4,10 +4,12 @@ one MI355X, with a bounded CPU-baseline sample timed on the host.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-One step = the whole hot path over the 16 leaf tiles of the 4x4 tiling of
-the grid: gradient, critical collation, manifold tracing (owners) and saddle
-incidence, in one batched launch per stage, with the input already resident
-in HBM.  Prints one JSON line.
+One step = the end-to-end Morse-Smale complex of the grid, input already
+resident in HBM: the 16 leaf tiles of the 4x4 tiling go through gradient,
+critical collation, manifold tracing and saddle incidence on the GPU (one
+batched launch per stage), then the complexes are merged up the tile tree,
+simplified at 1% of the critical-value range, merged down, and every leaf's
+manifold discs are filled.  Prints one JSON line.
 """
 import argparse
 import json
@@ -26,6 +28,7 @@ SIDE = 4096
 LX = LY = 2          # 4x4 tiles
 OCTAVES, LACUNARITY, GAIN = 8, 2.0, 0.5
 PERIOD = SIDE / 4.0  # base lattice period in pixels
+THRESHOLD = 0.01
 SEED = 20240601
 
 
@@ -58,23 +61,23 @@ def split_pieces(sx, sy, lx, ly):
 
 
 def cpu_baseline(data, budget_s=20.0):
-    """The oracle's GridDataset::work() (one thread) over square crops of the
-    same field, repeated until ~budget_s of CPU time; Mcells/s."""
+    """The oracle's whole GridDataManager order (one thread, as with -st) on
+    square crops of the same field with the same 4x4 tiling and threshold,
+    repeated until ~budget_s of CPU time; Mcells/s."""
     import oracle
     side = 1024
     crop = np.ascontiguousarray(data[:side, :side])
-    rect = (0, 0, 2 * (side - 1), 2 * (side - 1))
     t0 = time.perf_counter()
     n = 0
     while True:
-        oracle.tile_work(crop, rect, rect)
+        oracle.Pipeline(crop, lx=LX, ly=LY, threshold=THRESHOLD, keep=False)
         n += 1
         el = time.perf_counter() - t0
         if el > budget_s or n >= 50:
             break
-    return {"value": n * side * side / 1e6 / el, "unit": "Mcells/s", "cores": 1, "kind": "port",
-            "sample": f"{n} x oracle GridDataset::work() on a {side}x{side} crop of the same field "
-                      f"(gradient, collation, owner BFS), {el:.1f}s single-threaded"}
+    return {"value": round(n * side * side / 1e6 / el, 3), "unit": "Mcells/s", "cores": 1, "kind": "port",
+            "sample": f"{n} x oracle end-to-end pipeline (4x4 tiles, simplification at {THRESHOLD}, "
+                      f"manifold discs) on a {side}x{side} crop of the same field, {el:.1f}s single-threaded"}
 
 
 def main():
@@ -94,30 +97,33 @@ def main():
     data = mx.gen_value_noise(SIDE, SIDE, OCTAVES, LACUNARITY, GAIN, PERIOD, SEED)
     dev = mx.DeviceBuffer.from_array(data)
     pieces = split_pieces(SIDE, SIDE, LX, LY)
-    tiles = []
-    for rect, ext in pieces:
-        t = mx.GridDataset(rect, ext)
-        off = (ext[1] // 2) * SIDE + ext[0] // 2
-        t.bind_device(dev.ptr + 4 * off, SIDE)
-        tiles.append(t)
+
+    def step():
+        return mx.GridRun(None, lx=LX, ly=LY, threshold=THRESHOLD, device_ptr=dev.ptr, size=(SIDE, SIDE))
 
     for _ in range(args.warmup):
-        mx.work_tiles(tiles)
+        step()
     mx.profile(True)
+    host_t = np.zeros(5)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        mx.work_tiles(tiles)
+        run = step()
+        host_t += run.times()
+        del run
     el = time.perf_counter() - t0
     prof = mx.profile_read()
     mx.profile(False)
+    last = step()
+    K = sum(last.piece(i).num_cps() for i in range(len(pieces)))
+    root_cps = last.piece(last.num_pieces() - 1).num_cps()
 
     ms_step = 1000.0 * el / args.steps
     mcells = SIDE * SIDE / 1e6 / (el / args.steps)
-    K = sum(t.num_critical() for t in tiles)
 
     # algorithmic bytes per launch (all 16 tiles, halos included)
-    nm = sum(t.vx * t.vy for t in tiles)
-    cells = sum(t.cw * t.ch for t in tiles)
+    geo = [((e[2] - e[0]) // 2 + 1, (e[3] - e[1]) // 2 + 1) for _, e in pieces]
+    nm = sum(vx * vy for vx, vy in geo)
+    cells = sum((2 * vx - 1) * (2 * vy - 1) for vx, vy in geo)
     grad_bytes = 4 * nm + cells
     trace_bytes = 5 * cells
     g_ms, g_n = prof["gradient"]
@@ -145,12 +151,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "4096x4096 f32 value noise (8 octaves, lacunarity 2, gain 0.5, period 1024), "
-                               "4x4 tiles; stages: gradient+collation+trace+incidence",
-                   "grid": [SIDE, SIDE], "tiles": [4, 4], "critical_cells": K},
+                               "4x4 tile merge, persistence simplification at 1% of range, manifold discs",
+                   "grid": [SIDE, SIDE], "tiles": [4, 4], "threshold": THRESHOLD,
+                   "leaf_critical_points": K, "root_critical_points": root_cps},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": traffic},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "phase_ms": dict(zip(("compute", "merge_up", "simplify", "merge_down", "discs"),
+                             [round(1000 * v / args.steps, 3) for v in host_t])),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)

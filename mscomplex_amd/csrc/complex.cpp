@@ -190,7 +190,7 @@ void complex_from_tile(const mscx_tile *t, Complex &out) {
   complex_build(out, r, e, crit, fn, conn, pr);
 }
 
-static constexpr uint64_t kMaxArcProduct = 1ull << 24;
+static constexpr uint64_t kMaxArcProduct = 1ull << 20;
 
 // cancelPairs (discreteMorseAlgorithm.h:348-464 / :466-585): cp0 is made
 // the lower-dimensional cell; every asc(cp0) x des(cp1) pair is connected
@@ -203,11 +203,14 @@ void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> 
   // element equal to cp1 / cp0 is skipped), so iterating them directly is safe.
   const ConnSet &A = m.asc[cp0];
   const ConnSet &D = m.des[cp1];
-  // On tie-heavy fields repeated strangulations double arc multiplicities at
-  // every cancellation; the reference then grows without bound.  Fail the
-  // call instead of exhausting host memory.
-  if ((uint64_t)A.size() * D.size() > kMaxArcProduct)
-    throw Error(MSCX_ENOMEM, "arc multiplicity explosion in cancelPairs (tie-heavy field at this threshold)");
+  // Under dense cancellation (noisy or tie-heavy fields at large thresholds)
+  // repeated strangulations multiply arc multiplicities at every step and
+  // the reference grows without bound.  Fail the call instead of exhausting
+  // host memory.
+  const int64_t prod = (int64_t)A.size() * (int64_t)D.size();
+  m.arc_budget -= prod;
+  if (prod > (int64_t)kMaxArcProduct || m.arc_budget < 0)
+    throw Error(MSCX_ENOMEM, "arc multiplicity explosion in cancelPairs (dense cancellation at this threshold)");
   for (uint32_t a : A) {
     for (uint32_t d : D) {
       if (a == cp1 || d == cp0) continue;
@@ -239,12 +242,14 @@ void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> 
   m.asc[cp1].clear();
 }
 
-// uncancel_pairs (discreteMorseAlgorithm.h:1006-1089)
+// uncancel_pairs (discreteMorseAlgorithm.h:1006-1089).  The rebuilt
+// multiset is gathered unsorted and sorted once (same contents as the
+// reference's element-wise multiset inserts).
 void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2) {
   if (!m.asc[cp1].contains(cp2)) std::swap(cp1, cp2);
   ConnSet *acdc[2] = {&m.asc[cp1], &m.des[cp2]};
   uint32_t idxs[2] = {cp1, cp2};
-  ConnSet fresh;
+  std::vector<uint32_t> fresh;
   for (int i = 0; i < 2; ++i) {
     uint32_t other = idxs[(i + 1) % 2];
     if (!acdc[i]->contains(other)) throw std::logic_error("cancellable pair is not connected");
@@ -252,7 +257,7 @@ void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2) {
     for (uint32_t v : *acdc[i]) {
       if (v == other) continue;
       if (!(m.flags[v] & CPF_BCANC)) {
-        fresh.insert(v);
+        fresh.push_back(v);
         continue;
       }
       if (m.flags[v] & CPF_CANCELLED) throw std::logic_error("*acdc_conn_it should not have been cancelled yet");
@@ -262,10 +267,15 @@ void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2) {
       for (uint32_t w : pc) {
         if (m.flags[w] & CPF_BCANC)
           throw std::logic_error("the connected cancelable cp shold not  contain a cancelable cp in its connections");
-        fresh.insert(w);
+        fresh.push_back(w);
       }
     }
-    *acdc[i] = fresh;
+    m.arc_budget -= (int64_t)fresh.size();
+    if (m.arc_budget < 0)
+      throw Error(MSCX_ENOMEM, "arc multiplicity explosion in uncancel_pairs (dense cancellation at this threshold)");
+    std::sort(fresh.begin(), fresh.end());
+    acdc[i]->clear();
+    for (uint32_t w : fresh) acdc[i]->push_sorted(w);
   }
   m.flags[cp1] &= (uint8_t)~CPF_CANCELLED;
   m.flags[cp2] &= (uint8_t)~CPF_CANCELLED;
@@ -450,6 +460,7 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   std::vector<P2> heap_store;
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
+  m.arc_budget = (int64_t)(16 * narcs + ((size_t)1 << 20));
   heap_store.reserve(narcs * 2 + 16);
   std::priority_queue<P2, std::vector<P2>, PersistenceLess> pq(PersistenceLess{m.fn.data()}, std::move(heap_store));
 
@@ -505,8 +516,16 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   }
 }
 
+static int64_t arc_count(const Complex &m) {
+  int64_t n = 0;
+  for (uint32_t i = 0; i < m.size(); ++i) n += m.asc[i].size() + m.des[i].size();
+  return n;
+}
+
 void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc) {
+  m.arc_budget = 16 * arc_count(m) + ((int64_t)1 << 20);
   for (auto it = canc.rbegin(); it != canc.rend(); ++it) uncancel_pairs(m, it->first, it->second);
+  m.arc_budget = INT64_MAX;
 }
 
 // postMergeFillDiscs (grid_dataset.cpp:1044-1231).  codes / own are the

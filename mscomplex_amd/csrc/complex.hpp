@@ -176,6 +176,9 @@ struct Complex {
   std::vector<ConnSet> asc, des;
   std::vector<std::vector<uint32_t>> asc_disc, des_disc;
   CellMap idx;
+  // Arc insertions cancel_pairs may still make before failing (set by
+  // simplify; see cancel_pairs).
+  int64_t arc_budget = INT64_MAX;
 
   uint32_t size() const { return (uint32_t)cell.size(); }
   // shallow_replicate_cp (grid_mscomplex.cpp:7-22)

@@ -119,3 +119,13 @@ def test_value_noise_4x4():
 def test_quantised_white_ties_heavy():
     data = mx.gen_white(64, 64, levels=8, seed=2)
     compare(data, lx=1, ly=2, threshold=0.1)
+
+
+def test_dense_cancellation_fails_cleanly():
+    data = rng_field("white", 150, 130, 5)
+    rect = (0, 0, 2 * 149, 2 * 129)
+    crit, fn, conn, pr, codes, own = tile_inputs(data, rect, rect)
+    c = mx.GridMSComplex.from_critical(rect, rect, crit, fn, conn, pr)
+    with pytest.raises(mx.MscxError) as e:
+        c.simplify_un_simplify(0.1)
+    assert e.value.code == -4
