@@ -443,13 +443,18 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
 
 // persistence_comparator_t (grid_mscomplex.cpp:331-379).  Its geometric
 // tie-break compares the first pair with itself, so two pairs of equal
-// persistence are equivalent; only |f2 - f1| orders the heap.
+// persistence are equivalent; only |f2 - f1| orders the heap.  The heap
+// element carries that persistence (the function values do not change while
+// simplifying), so a comparison reads no other memory; the sequence of heap
+// operations, and hence the order among equal persistences, is the one
+// std::priority_queue produces for the reference's (uint, uint) elements.
+struct HeapPair {
+  float pers;
+  uint32_t first, second;
+};
 struct PersistenceLess {
-  const float *fn;
-  bool operator()(const std::pair<uint32_t, uint32_t> &p1, const std::pair<uint32_t, uint32_t> &p2) const {
-    float d1 = std::abs(fn[p1.second] - fn[p1.first]);
-    float d2 = std::abs(fn[p2.second] - fn[p2.first]);
-    if (d1 != d2) return d1 > d2;
+  bool operator()(const HeapPair &p1, const HeapPair &p2) const {
+    if (p1.pers != p2.pers) return p1.pers > p2.pers;
     return false;
   }
 };
@@ -457,12 +462,14 @@ struct PersistenceLess {
 // simplify (grid_mscomplex.cpp:384-512)
 void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh) {
   typedef std::pair<uint32_t, uint32_t> P2;
-  std::vector<P2> heap_store;
+  const float *fn = m.fn.data();
+  auto mk = [fn](uint32_t a, uint32_t b) { return HeapPair{std::abs(fn[b] - fn[a]), a, b}; };
+  std::vector<HeapPair> heap_store;
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
   m.arc_budget = (int64_t)(16 * narcs + ((size_t)1 << 20));
   heap_store.reserve(narcs * 2 + 16);
-  std::priority_queue<P2, std::vector<P2>, PersistenceLess> pq(PersistenceLess{m.fn.data()}, std::move(heap_store));
+  std::priority_queue<HeapPair, std::vector<HeapPair>, PersistenceLess> pq(PersistenceLess{}, std::move(heap_store));
 
   float max_val = std::numeric_limits<float>::min();
   float min_val = std::numeric_limits<float>::max();
@@ -476,18 +483,18 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     }
     max_val = std::max(max_val, m.fn[i]);
     min_val = std::min(min_val, m.fn[i]);
-    for (uint32_t d : m.des[i]) pq.push(P2(i, d));
+    for (uint32_t d : m.des[i]) pq.push(mk(i, d));
   }
-  std::vector<P2> resubmit;
+  std::vector<HeapPair> resubmit;
   float max_persistence = max_val - min_val;
   std::vector<uint32_t> new_edges;
   const Rect &R = m.rect;
   auto on_b = [&](uint32_t c) { return R.on_boundary(cx_of(c), cy_of(c)); };
 
   while (!pq.empty()) {
-    P2 cp = pq.top();
+    HeapPair hp = pq.top();
     pq.pop();
-    uint32_t v1 = cp.first, v2 = cp.second;
+    uint32_t v1 = hp.first, v2 = hp.second;
     float pers = std::abs(m.fn[v1] - m.fn[v2]);
     if ((double)pers / (double)max_persistence > thresh) break;
     if (cell_dim(m.cell[v2]) == 1) std::swap(v1, v2);
@@ -499,7 +506,7 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     const ConnSet &c2 = (d2 / 2 == 0) ? m.asc[v2] : m.des[v2];
     bool strang = m.flags[v1] & CPF_STRANGULATION;
     if (strang && c2.size() != 1) {
-      resubmit.push_back(cp);
+      resubmit.push_back(hp);
       continue;
     }
     if (strang && b1) continue;
@@ -509,9 +516,9 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     m.flags[v2] |= CPF_BCANC;
     m.pair[v1] = v2;
     m.pair[v2] = v1;
-    canc.push_back(cp);
-    for (size_t i = 0; i < new_edges.size(); i += 2) pq.push(P2(new_edges[i], new_edges[i + 1]));
-    for (const P2 &r : resubmit) pq.push(r);
+    canc.push_back(P2(hp.first, hp.second));
+    for (size_t i = 0; i < new_edges.size(); i += 2) pq.push(mk(new_edges[i], new_edges[i + 1]));
+    for (const HeapPair &r : resubmit) pq.push(r);
     resubmit.clear();
   }
 }
