@@ -84,6 +84,14 @@ uint32_t CellMap::get_or_zero(uint32_t key) {
 
 // ------------------------------------------------------------------ Complex
 
+DiscSpan Complex::disc(uint32_t i, int j) const {
+  const std::vector<uint32_t> &v = j == 0 ? asc_disc[i] : des_disc[i];
+  uint32_t d = cell_dim(cell[i]);
+  if (!xd_off.empty() && i + 1 < xd_off.size() && v.empty() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
+    return DiscSpan{xd_cells.data() + xd_off[i], xd_off[i + 1] - xd_off[i]};
+  return DiscSpan{v.data(), v.size()};
+}
+
 uint32_t Complex::replicate(const Complex &src, uint32_t j) {
   uint32_t c = src.cell[j];
   if (idx.contains(c)) throw std::logic_error("this cp is present in msc");
@@ -120,12 +128,13 @@ void Complex::dump(std::vector<uint32_t> &out) const {
     out.push_back(flags[i]);
     out.push_back(asc[i].size());
     out.push_back(des[i].size());
-    out.push_back((uint32_t)asc_disc[i].size());
-    out.push_back((uint32_t)des_disc[i].size());
+    DiscSpan ad = disc(i, 0), dd = disc(i, 1);
+    out.push_back((uint32_t)ad.n);
+    out.push_back((uint32_t)dd.n);
     out.insert(out.end(), asc[i].begin(), asc[i].end());
     out.insert(out.end(), des[i].begin(), des[i].end());
-    out.insert(out.end(), asc_disc[i].begin(), asc_disc[i].end());
-    out.insert(out.end(), des_disc[i].begin(), des_disc[i].end());
+    out.insert(out.end(), ad.p, ad.p + ad.n);
+    out.insert(out.end(), dd.p, dd.p + dd.n);
   }
 }
 
@@ -448,13 +457,16 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
 // simplifying), so a comparison reads no other memory; the sequence of heap
 // operations, and hence the order among equal persistences, is the one
 // std::priority_queue produces for the reference's (uint, uint) elements.
+// Heap element: the persistence as the bit pattern of a non-negative float
+// (ordered like the float) and the slot of the (first, second) pair in a
+// side array, so sift operations move 8 bytes and read nothing else.
 struct HeapPair {
-  float pers;
-  uint32_t first, second;
+  uint32_t key;
+  uint32_t slot;
 };
 struct PersistenceLess {
   bool operator()(const HeapPair &p1, const HeapPair &p2) const {
-    if (p1.pers != p2.pers) return p1.pers > p2.pers;
+    if (p1.key != p2.key) return p1.key > p2.key;
     return false;
   }
 };
@@ -463,12 +475,20 @@ struct PersistenceLess {
 void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh) {
   typedef std::pair<uint32_t, uint32_t> P2;
   const float *fn = m.fn.data();
-  auto mk = [fn](uint32_t a, uint32_t b) { return HeapPair{std::abs(fn[b] - fn[a]), a, b}; };
+  std::vector<P2> slots;  // (first, second) of every pushed element
+  auto mk = [&](uint32_t a, uint32_t b) {
+    float p = std::abs(fn[b] - fn[a]);
+    uint32_t key;
+    memcpy(&key, &p, 4);
+    slots.emplace_back(a, b);
+    return HeapPair{key, (uint32_t)(slots.size() - 1)};
+  };
   std::vector<HeapPair> heap_store;
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
   m.arc_budget = (int64_t)(16 * narcs + ((size_t)1 << 20));
   heap_store.reserve(narcs * 2 + 16);
+  slots.reserve(narcs * 4 + 16);
   std::priority_queue<HeapPair, std::vector<HeapPair>, PersistenceLess> pq(PersistenceLess{}, std::move(heap_store));
 
   float max_val = std::numeric_limits<float>::min();
@@ -494,7 +514,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   while (!pq.empty()) {
     HeapPair hp = pq.top();
     pq.pop();
-    uint32_t v1 = hp.first, v2 = hp.second;
+    const P2 cp = slots[hp.slot];
+    uint32_t v1 = cp.first, v2 = cp.second;
     float pers = std::abs(m.fn[v1] - m.fn[v2]);
     if ((double)pers / (double)max_persistence > thresh) break;
     if (cell_dim(m.cell[v2]) == 1) std::swap(v1, v2);
@@ -516,7 +537,7 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     m.flags[v2] |= CPF_BCANC;
     m.pair[v1] = v2;
     m.pair[v2] = v1;
-    canc.push_back(P2(hp.first, hp.second));
+    canc.push_back(cp);
     for (size_t i = 0; i < new_edges.size(); i += 2) pq.push(mk(new_edges[i], new_edges[i + 1]));
     for (const HeapPair &r : resubmit) pq.push(r);
     resubmit.clear();
@@ -535,23 +556,36 @@ void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &c
   m.arc_budget = INT64_MAX;
 }
 
-// postMergeFillDiscs (grid_dataset.cpp:1044-1231).  codes / own are the
-// tile's arrays over ext, x fastest; own is modified (owner redirects).
-void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *codes, std::vector<uint32_t> &own) {
-  const int cw = ext.x1 - ext.x0 + 1;
-  auto at = [&](int x, int y) { return (size_t)(x - ext.x0) + (size_t)(y - ext.y0) * cw; };
-  auto code = [&](int x, int y) { return codes[at(x, y)]; };
-  auto paired = [&](int x, int y) { return (code(x, y) & CF_PAIRED) != 0; };
-  auto critical = [&](int x, int y) { return (code(x, y) & CF_CRITICAL) != 0; };
-  auto pair_of = [&](int x, int y) -> uint32_t {
-    int d = code_dir(code(x, y));
-    if (!d) return UINT32_MAX;
-    return (uint32_t)(uint16_t)(x + dir_dx(d)) | ((uint32_t)(uint16_t)(y + dir_dy(d)) << 16);
-  };
-  auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
+// ---------------------------------------------------------------- discs
+// postMergeFillDiscs (grid_dataset.cpp:1044-1231) in four parts:
+//   disc_prepare  -- the pass over critical points (:1052-1140): owner
+//                    redirects of boundary-cancellable saddles, seeds of
+//                    saddle discs, the surviving-extremum maps
+//   labels        -- per interior vertex / quad, the surviving extremum
+//                    that owns it after one redirect (:1142-1187); computed
+//                    on the GPU by k_disc_labels, or on the host here
+//   disc_group    -- extremum discs as raster-ordered lists (CSR)
+//   disc_paths    -- saddle 1-manifolds traced from the seeds (:1189-1228)
 
-  CellMap asc_map, des_map;  // surviving extremum cell -> cp index
-  std::vector<uint32_t> surv_saddles;
+static inline uint32_t pack_xy(int x, int y) { return (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16); }
+
+void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out) {
+  auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
+  CellMap redir;  // cell -> slot in out.redirect (later assignments overwrite)
+  out.redirect.clear();
+  out.minmap.clear();
+  out.maxmap.clear();
+  out.surv_saddles.clear();
+  auto set_redirect = [&](uint32_t cell, uint32_t val) {
+    uint32_t s = redir.find(cell);
+    if (s == UINT32_MAX) {
+      redir.set(cell, (uint32_t)out.redirect.size());
+      out.redirect.emplace_back(cell, val);
+    } else {
+      out.redirect[s].second = val;
+    }
+  };
+  CellMap seen_min, seen_max;
   for (uint32_t i = 0; i < m.size(); ++i) {
     uint32_t c = m.cell[i];
     uint32_t dim = cell_dim(c);
@@ -563,8 +597,8 @@ void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *co
         if (conn.size() != 1) throw std::logic_error("I should be connected to exactly one surv extrema");
         uint32_t ec = m.cell[conn[0]];
         if (m.pair[pi] != i) throw std::logic_error("My pair does not know Im paired with him");
-        if (in_r(c)) own[at(cx_of(c), cy_of(c))] = ec;
-        if (in_r(pc)) own[at(cx_of(pc), cy_of(pc))] = ec;
+        if (in_r(c)) set_redirect(c, ec);
+        if (in_r(pc)) set_redirect(pc, ec);
       } else if (dim == 2) {
         for (uint32_t s : m.des[i])
           if (in_r(pc)) m.asc_disc[s].push_back(pc);
@@ -574,48 +608,90 @@ void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *co
       }
     } else {
       if (dim == 0) {
-        asc_map.insert(c, i);
+        if (!seen_min.contains(c)) {  // map::insert keeps the first
+          seen_min.set(c, i);
+          out.minmap.emplace_back(c, i);
+        }
       } else if (dim == 1) {
         int x = cx_of(c), y = cy_of(c);
         const int inc[4][2] = {{x, y + 1}, {x, y - 1}, {x - 1, y}, {x + 1, y}};
         for (int j = 0; j < 4; ++j) {
           int ix = inc[j][0], iy = inc[j][1];
-          if (rect.contains(ix, iy) && paired(ix, iy) && !critical(ix, iy)) {
-            uint32_t ic = (uint32_t)(uint16_t)ix | ((uint32_t)(uint16_t)iy << 16);
-            (cell_dim(ic) == 0 ? m.des_disc[i] : m.asc_disc[i]).push_back(pair_of(ix, iy));
+          if (!rect.contains(ix, iy)) continue;
+          uint8_t code = cv.at(ix, iy);
+          if ((code & CF_PAIRED) && !(code & CF_CRITICAL)) {
+            int d = code_dir(code);
+            uint32_t pr = pack_xy(ix + dir_dx(d), iy + dir_dy(d));
+            (((ix & 1) + (iy & 1)) == 0 ? m.des_disc[i] : m.asc_disc[i]).push_back(pr);
           }
         }
-        surv_saddles.push_back(i);
+        out.surv_saddles.push_back(i);
       } else {
-        des_map.insert(c, i);
+        if (!seen_max.contains(c)) {
+          seen_max.set(c, i);
+          out.maxmap.emplace_back(c, i);
+        }
       }
     }
   }
+  std::sort(out.redirect.begin(), out.redirect.end());
+  std::sort(out.minmap.begin(), out.minmap.end());
+  std::sort(out.maxmap.begin(), out.maxmap.end());
+}
 
-  for (int y = rect.y0; y <= rect.y1; y += 2) {
-    for (int x = rect.x0; x <= rect.x1; x += 2) {
-      uint32_t o = own[at(x, y)];
-      if (in_r(o)) o = own[at(cx_of(o), cy_of(o))];
-      if (cx_of(o) != -1 && cy_of(o) != -1) {
-        uint32_t k = asc_map.find(o);
-        if (k == UINT32_MAX) throw std::logic_error("owner extremum not in surviving map");
-        m.asc_disc[k].push_back((uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16));
-      }
-    }
-  }
-  for (int y = rect.y0 + 1; y < rect.y1; y += 2) {
-    for (int x = rect.x0 + 1; x < rect.x1; x += 2) {
-      uint32_t o = own[at(x, y)];
-      if (in_r(o)) o = own[at(cx_of(o), cy_of(o))];
-      if (cx_of(o) != -1 && cy_of(o) != -1) {
-        uint32_t k = des_map.find(o);
-        if (k == UINT32_MAX) throw std::logic_error("owner extremum not in surviving map");
-        m.des_disc[k].push_back((uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16));
-      }
-    }
-  }
+static inline uint32_t sorted_find(const std::vector<std::pair<uint32_t, uint32_t>> &v, uint32_t key) {
+  auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(key, (uint32_t)0));
+  return (it != v.end() && it->first == key) ? it->second : UINT32_MAX;
+}
 
-  for (uint32_t si : surv_saddles) {
+void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, const uint32_t *own, int64_t own_pitch,
+                      std::vector<uint32_t> &lv, std::vector<uint32_t> &lq) {
+  auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
+  auto own2 = [&](uint32_t c) {
+    uint32_t r = sorted_find(dp.redirect, c);
+    return r != UINT32_MAX ? r : own[(int64_t)(cy_of(c) - ext.y0) * own_pitch + (cx_of(c) - ext.x0)];
+  };
+  auto label = [&](uint32_t c, const std::vector<std::pair<uint32_t, uint32_t>> &mp) -> uint32_t {
+    uint32_t o = own2(c);
+    if (in_r(o)) o = own2(o);
+    if (cx_of(o) == -1 || cy_of(o) == -1) return UINT32_MAX;
+    uint32_t k = sorted_find(mp, o);
+    if (k == UINT32_MAX) throw std::logic_error("owner extremum not in surviving map");
+    return k;
+  };
+  lv.clear();
+  lq.clear();
+  for (int y = rect.y0; y <= rect.y1; y += 2)
+    for (int x = rect.x0; x <= rect.x1; x += 2) lv.push_back(label(pack_xy(x, y), dp.minmap));
+  for (int y = rect.y0 + 1; y < rect.y1; y += 2)
+    for (int x = rect.x0 + 1; x < rect.x1; x += 2) lq.push_back(label(pack_xy(x, y), dp.maxmap));
+}
+
+void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq) {
+  const int nvx = (rect.x1 - rect.x0) / 2 + 1, nvy = (rect.y1 - rect.y0) / 2 + 1;
+  const size_t nv = (size_t)nvx * nvy, nq = (size_t)(nvx - 1) * (nvy - 1);
+  const uint32_t n = m.size();
+  std::vector<uint32_t> &off = m.xd_off;
+  off.assign((size_t)n + 1, 0);
+  for (size_t r = 0; r < nv; ++r)
+    if (lv[r] != UINT32_MAX) off[lv[r] + 1]++;
+  for (size_t r = 0; r < nq; ++r)
+    if (lq[r] != UINT32_MAX) off[lq[r] + 1]++;
+  for (uint32_t i = 0; i < n; ++i) off[i + 1] += off[i];
+  m.xd_cells.resize(off[n]);
+  std::vector<uint32_t> pos(off.begin(), off.end() - 1);
+  size_t r = 0;
+  for (int y = rect.y0; y <= rect.y1; y += 2)
+    for (int x = rect.x0; x <= rect.x1; x += 2, ++r)
+      if (lv[r] != UINT32_MAX) m.xd_cells[pos[lv[r]]++] = pack_xy(x, y);
+  r = 0;
+  for (int y = rect.y0 + 1; y < rect.y1; y += 2)
+    for (int x = rect.x0 + 1; x < rect.x1; x += 2, ++r)
+      if (lq[r] != UINT32_MAX) m.xd_cells[pos[lq[r]]++] = pack_xy(x, y);
+}
+
+void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv) {
+  for (uint32_t si : dp.surv_saddles) {
     for (int j = 0; j < 2; ++j) {
       std::vector<uint32_t> &disc = j == 0 ? m.asc_disc[si] : m.des_disc[si];
       for (size_t k = 0; k != disc.size(); ++k) {
@@ -625,36 +701,30 @@ void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *co
         int ct = 0;
         int dx = px & 1, dy = py & 1;
         if (j == 0) {  // cofacets, filtered to ext
-          int cand[2][2];
-          int nc = 0;
           if (dx + dy == 0) {
             const int c4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
             for (int q = 0; q < 4; ++q)
-              if (ext.contains(c4[q][0], c4[q][1]))
-                cets[ct++] = (uint32_t)(uint16_t)c4[q][0] | ((uint32_t)(uint16_t)c4[q][1] << 16);
+              if (ext.contains(c4[q][0], c4[q][1])) cets[ct++] = pack_xy(c4[q][0], c4[q][1]);
           } else if (dx + dy == 1) {
-            cand[0][0] = px + dy; cand[0][1] = py + dx;
-            cand[1][0] = px - dy; cand[1][1] = py - dx;
-            nc = 2;
-            for (int q = 0; q < nc; ++q)
-              if (ext.contains(cand[q][0], cand[q][1]))
-                cets[ct++] = (uint32_t)(uint16_t)cand[q][0] | ((uint32_t)(uint16_t)cand[q][1] << 16);
+            if (ext.contains(px + dy, py + dx)) cets[ct++] = pack_xy(px + dy, py + dx);
+            if (ext.contains(px - dy, py - dx)) cets[ct++] = pack_xy(px - dy, py - dx);
           }
         } else {  // facets
           if (dx + dy == 1) {
-            cets[ct++] = (uint32_t)(uint16_t)(px + dx) | ((uint32_t)(uint16_t)(py + dy) << 16);
-            cets[ct++] = (uint32_t)(uint16_t)(px - dx) | ((uint32_t)(uint16_t)(py - dy) << 16);
+            cets[ct++] = pack_xy(px + dx, py + dy);
+            cets[ct++] = pack_xy(px - dx, py - dy);
           } else if (dx + dy == 2) {
             const int f4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
-            for (int q = 0; q < 4; ++q)
-              cets[ct++] = (uint32_t)(uint16_t)f4[q][0] | ((uint32_t)(uint16_t)f4[q][1] << 16);
+            for (int q = 0; q < 4; ++q) cets[ct++] = pack_xy(f4[q][0], f4[q][1]);
           }
         }
         for (int q = 0; q < ct; ++q) {
           int qx = cx_of(cets[q]), qy = cy_of(cets[q]);
-          if (critical(qx, qy)) continue;
+          uint8_t code = cv.at(qx, qy);
+          if (code & CF_CRITICAL) continue;
           if (!rect.contains(qx, qy)) continue;
-          uint32_t p = pair_of(qx, qy);
+          int d = code_dir(code);
+          uint32_t p = d ? pack_xy(qx + dir_dx(d), qy + dir_dy(d)) : UINT32_MAX;
           if (p == pcell) continue;
           disc.push_back(p);
         }
@@ -663,19 +733,31 @@ void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *co
   }
 }
 
+void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *codes, std::vector<uint32_t> &own) {
+  const int cw = ext.x1 - ext.x0 + 1;
+  CodesView cv{codes, cw, ext.x0, ext.y0};
+  DiscPrep dp;
+  disc_prepare(m, rect, cv, dp);
+  std::vector<uint32_t> lv, lq;
+  disc_labels_host(dp, rect, ext, own.data(), cw, lv, lq);
+  disc_group(m, rect, lv.data(), lq.data());
+  disc_paths(m, dp, rect, ext, cv);
+  for (const auto &r : dp.redirect) own[(size_t)(cy_of(r.first) - ext.y0) * cw + (cx_of(r.first) - ext.x0)] = r.second;
+}
+
 // write_discs (grid_mscomplex.cpp:542-590): raw packed ids appended to
 // <prefix>asc_(x,y) and <prefix>des_(x,y).
 void write_discs(const Complex &m, const std::string &prefix) {
   for (uint32_t i = 0; i < m.size(); ++i) {
     if (m.flags[i] & CPF_BCANC) continue;
     for (int j = 0; j < 2; ++j) {
-      const std::vector<uint32_t> &d = j == 0 ? m.asc_disc[i] : m.des_disc[i];
-      if (d.empty()) continue;
+      DiscSpan d = m.disc(i, j);
+      if (d.n == 0) continue;
       std::ostringstream ss;
       ss << prefix << (j == 0 ? "asc_" : "des_") << "(" << cx_of(m.cell[i]) << "," << cy_of(m.cell[i]) << ")";
       std::ofstream os(ss.str(), std::ios::out | std::ios::ate | std::ios::app | std::ios::binary);
       if (!os.is_open()) throw Error(MSCX_EINVAL, "asc/des disc file not writeable");
-      os.write((const char *)d.data(), (std::streamsize)(d.size() * 4));
+      os.write((const char *)d.p, (std::streamsize)(d.n * 4));
     }
   }
 }
@@ -755,12 +837,8 @@ int mscx_complex_simplify_un_simplify(mscx_complex *c, double threshold) {
 int mscx_tile_fill_discs(mscx_tile *t, mscx_complex *c) {
   return guarded([&] {
     MSCX_REQUIRE(t && c, MSCX_EINVAL, "null argument");
-    std::vector<uint8_t> codes;
-    std::vector<uint32_t> own;
-    tile_codes(t, codes);
-    tile_owners(t, own);
-    Rect r{t->rect[0], t->rect[1], t->rect[2], t->rect[3]}, e{t->ext[0], t->ext[1], t->ext[2], t->ext[3]};
-    fill_discs(c->c, r, e, codes.data(), own);
+    Complex *cc = &c->c;
+    fill_discs_device(&t, &cc, 1);
   });
 }
 

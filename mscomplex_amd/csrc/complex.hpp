@@ -167,6 +167,11 @@ inline int cx_of(uint32_t c) { return (int16_t)(c & 0xFFFF); }
 inline int cy_of(uint32_t c) { return (int16_t)(c >> 16); }
 inline uint32_t cell_dim(uint32_t c) { return (c & 1) + ((c >> 16) & 1); }
 
+struct DiscSpan {
+  const uint32_t *p;
+  size_t n;
+};
+
 struct Complex {
   Rect rect{}, ext{};
   std::vector<uint32_t> cell;
@@ -175,6 +180,9 @@ struct Complex {
   std::vector<uint8_t> flags;
   std::vector<ConnSet> asc, des;
   std::vector<std::vector<uint32_t>> asc_disc, des_disc;
+  // discs of surviving extrema (vertices of minima, quads of maxima) as
+  // raster-ordered lists in CSR form by critical-point index
+  std::vector<uint32_t> xd_off, xd_cells;
   CellMap idx;
   // Arc insertions cancel_pairs may still make before failing (set by
   // simplify; see cancel_pairs).
@@ -185,6 +193,8 @@ struct Complex {
   uint32_t replicate(const Complex &src, uint32_t j);
 
   void dump(std::vector<uint32_t> &out) const;
+  // the asc (j = 0) or des (j = 1) disc of critical point i
+  DiscSpan disc(uint32_t i, int j) const;
 };
 
 // GridDataset::writeout_connectivity from a worked tile.
@@ -204,6 +214,28 @@ void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &c
 // GridDataset::postMergeFillDiscs from a tile's codes and owner image (host
 // copies over ext, x fastest).
 void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *codes, std::vector<uint32_t> &own);
+
+// Parts of fill_discs (the pipeline runs the label pass on the GPU).
+struct CodesView {
+  const uint8_t *p;
+  int64_t pitch;
+  int ox, oy;
+  uint8_t at(int x, int y) const { return p[(int64_t)(y - oy) * pitch + (x - ox)]; }
+};
+struct DiscPrep {
+  std::vector<std::pair<uint32_t, uint32_t>> redirect;  // (cell, owner), sorted by cell
+  std::vector<std::pair<uint32_t, uint32_t>> minmap, maxmap;  // (cell, cp index), sorted by cell
+  std::vector<uint32_t> surv_saddles;
+};
+void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out);
+void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, const uint32_t *own, int64_t own_pitch,
+                      std::vector<uint32_t> &lv, std::vector<uint32_t> &lq);
+void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq);
+void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv);
+
+// postMergeFillDiscs for n worked tiles and their complexes, label pass on
+// the GPU (discs.cpp).
+void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n);
 
 void write_discs(const Complex &m, const std::string &prefix);
 

@@ -419,6 +419,55 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
   T.crit_pair[k] = pidx;
 }
 
+// ------------------------------------------------------------ disc labels
+// For every interior vertex (quad) of a leaf: o = owner after redirects; if
+// o lies in the rect, one more redirect step o = owner(o); the label is the
+// critical-point index of surviving extremum o.
+
+__device__ __forceinline__ uint32_t table_find(const uint32_t *__restrict__ t, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (t[2 * mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < n && t[2 * lo] == key) ? t[2 * lo + 1] : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t owner_after(const DiscDesc &D, int x, int y) {
+  uint32_t r = D.nredir ? table_find(D.redir, D.nredir, pack_cell(x, y)) : 0xFFFFFFFFu;
+  if (r != 0xFFFFFFFFu) return r;
+  return D.own[(int64_t)(y - D.oy) * D.own_pitch + (x - D.ox)];
+}
+
+__global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict__ descs, const int *__restrict__ prefix,
+                                                     int n) {
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, n, b);
+  const DiscDesc &D = descs[t];
+  const int nvx = (D.rx1 - D.rx0) / 2 + 1, nvy = (D.ry1 - D.ry0) / 2 + 1;
+  const int64_t nv = (int64_t)nvx * nvy, nq = (int64_t)(nvx - 1) * (nvy - 1);
+  const int64_t item = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
+  if (item >= nv + nq) return;
+  const bool vert = item < nv;
+  const int64_t r = vert ? item : item - nv;
+  const int w = vert ? nvx : nvx - 1;
+  const int x = D.rx0 + (vert ? 0 : 1) + 2 * (int)(r % w), y = D.ry0 + (vert ? 0 : 1) + 2 * (int)(r / w);
+  uint32_t o = owner_after(D, x, y);
+  if (o != 0xFFFFFFFFu) {
+    int ox = (int)(int16_t)(o & 0xFFFF), oy = (int)(int16_t)(o >> 16);
+    if (ox >= D.rx0 && ox <= D.rx1 && oy >= D.ry0 && oy <= D.ry1) o = owner_after(D, ox, oy);
+  }
+  uint32_t lab = 0xFFFFFFFFu;
+  if ((o & 0xFFFF) != 0xFFFF && (o >> 16) != 0xFFFF) {
+    lab = vert ? table_find(D.minmap, D.nmin, o) : table_find(D.maxmap, D.nmax, o);
+    if (lab == 0xFFFFFFFFu) atomicOr(D.err, 1u);
+  }
+  (vert ? D.lab_v : D.lab_q)[r] = lab;
+}
+
 // -------------------------------------------------------------- launchers
 
 static int blocks_of(int kind, const TileDesc &T) {
@@ -452,6 +501,22 @@ int plan_blocks(int kind, const TileDesc *tiles, int ntiles, int *prefix) {
   }
   prefix[ntiles] = acc;
   return acc;
+}
+
+int plan_disc_blocks(const DiscDesc *d, int n, int *prefix) {
+  int acc = 0;
+  for (int t = 0; t < n; ++t) {
+    prefix[t] = acc;
+    int64_t nvx = (d[t].rx1 - d[t].rx0) / 2 + 1, nvy = (d[t].ry1 - d[t].ry0) / 2 + 1;
+    int64_t items = nvx * nvy + (nvx - 1) * (nvy - 1);
+    acc += (int)((items + 255) / 256);
+  }
+  prefix[n] = acc;
+  return acc;
+}
+
+void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_disc_labels, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
 }
 
 void launch_gradient(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {

@@ -21,4 +21,8 @@ void launch_jump(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int n
 void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 
+// Manifold labels for ndisc leaves; prefix has ndisc+1 entries of 256-item blocks.
+int plan_disc_blocks(const DiscDesc *d, int n, int *prefix);
+void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s);
+
 }  // namespace mscx

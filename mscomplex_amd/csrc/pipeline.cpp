@@ -196,17 +196,9 @@ extern "C" int mscx_grid_run(const float *data, int32_t sx, int32_t sy, int32_t 
 
     // collectSubdomainManifolds / postMergeFillDiscs (:397-490)
     if (!(flags & MSCX_RUN_NO_DISCS)) {
-      std::vector<std::vector<uint8_t>> codes(nleaf);
-      std::vector<std::vector<uint32_t>> own(nleaf);
-      for (int i = 0; i < nleaf; ++i) {
-        tile_codes(G->tiles[i], codes[i]);
-        tile_owners(G->tiles[i], own[i]);
-      }
-      parallel_for(nleaf, [&](int i) {
-        fill_discs(G->pieces[i]->c, leaves[i].rect, leaves[i].ext, codes[i].data(), own[i]);
-        std::vector<uint8_t>().swap(codes[i]);
-        std::vector<uint32_t>().swap(own[i]);
-      });
+      std::vector<Complex *> cs(nleaf);
+      for (int i = 0; i < nleaf; ++i) cs[i] = &G->pieces[i]->c;
+      fill_discs_device(G->tiles.data(), cs.data(), nleaf);
     }
     G->t[4] = now_s() - t4;
     *out = G.release();

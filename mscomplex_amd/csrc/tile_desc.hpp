@@ -46,4 +46,24 @@ struct TileDesc {
   uint32_t kcap;  // capacity of the crit arrays
 };
 
+// Per-leaf inputs of the manifold label pass (postMergeFillDiscs,
+// grid_dataset.cpp:1142-1187): owner image, owner redirects of cancelled
+// pairs, and the surviving extremum maps; tables are (cell, value) pairs
+// sorted by packed cell id.
+struct DiscDesc {
+  const uint32_t *own;
+  int64_t own_pitch;
+  int ox, oy;              // ext bottom-left, global cell coords
+  int rx0, ry0, rx1, ry1;  // interior rect, global cell coords
+  const uint32_t *redir;
+  uint32_t nredir;
+  const uint32_t *minmap;
+  uint32_t nmin;
+  const uint32_t *maxmap;
+  uint32_t nmax;
+  uint32_t *lab_v;  // per interior vertex, raster order: critical-point index or ~0
+  uint32_t *lab_q;  // per interior quad
+  uint32_t *err;    // set when an owner is not a surviving extremum
+};
+
 }  // namespace mscx
