@@ -225,4 +225,58 @@ MSCX_HD uint8_t cell_code(const KF &K, const TileGeom &g, int cx, int cy) {
   return (uint8_t)(flags | (dir << 2));
 }
 
+// The same code computed from candidates that are evaluated once per cell:
+// VC(i, j) = vertex_cand of vertex (i, j), HC(i, j) = hedge_cand of edge
+// (2i+1, 2j), EC(i, j) = vedge_cand of edge (2i, 2j+1), each DIR_NONE for a
+// cell outside the interior rect (exterior cells never pick a pair).  The
+// HIP kernel stores these in LDS; cell_code above is the per-cell form used
+// to check this one.
+template <class KF, class VCF, class HCF, class ECF>
+MSCX_HD uint8_t code_from_cands(const KF &K, const VCF &VC, const HCF &HC, const ECF &EC, const TileGeom &g,
+                                int cx, int cy) {
+  const int dx = cx & 1, dy = cy & 1;
+  const int i = cx >> 1, j = cy >> 1;
+  int dir = DIR_NONE;
+  if (!dx && !dy) {
+    dir = VC(i, j);
+  } else if (dx && !dy) {
+    dir = HC(i, j);
+    if (dir == DIR_NONE) {
+      const bool a_hi = K(i, j) > K(i + 1, j);
+      if (a_hi ? VC(i, j) == DIR_PX : VC(i + 1, j) == DIR_NX) dir = a_hi ? DIR_NX : DIR_PX;
+    }
+  } else if (!dx && dy) {
+    dir = EC(i, j);
+    if (dir == DIR_NONE) {
+      const bool a_hi = K(i, j) > K(i, j + 1);
+      if (a_hi ? VC(i, j) == DIR_PY : VC(i, j + 1) == DIR_NY) dir = a_hi ? DIR_NY : DIR_PY;
+    }
+  } else {
+    const uint64_t k00 = K(i, j), k10 = K(i + 1, j), k01 = K(i, j + 1), k11 = K(i + 1, j + 1);
+    int edir;
+    if (k00 > k10 && k00 > k01 && k00 > k11) edir = (k10 > k01) ? DIR_NY : DIR_NX;
+    else if (k10 > k01 && k10 > k11) edir = (k00 > k11) ? DIR_NY : DIR_PX;
+    else if (k01 > k11) edir = (k00 > k11) ? DIR_NX : DIR_PY;
+    else edir = (k10 > k01) ? DIR_PX : DIR_PY;
+    int ec;
+    switch (edir) {
+      case DIR_NY: ec = HC(i, j); break;
+      case DIR_PY: ec = HC(i, j + 1); break;
+      case DIR_NX: ec = EC(i, j); break;
+      default: ec = EC(i + 1, j); break;
+    }
+    if (ec == dir_flip(edir)) dir = edir;
+  }
+  const bool inr = in_rect(g, cx, cy);
+  uint8_t flags;
+  if (dir != DIR_NONE) {
+    flags = CF_PAIRED;
+    const int px = cx + dir_dx(dir), py = cy + dir_dy(dir);
+    if (bndry_cancellable_site(g, cx, cy) || bndry_cancellable_site(g, px, py)) flags |= CF_CRITICAL;
+  } else {
+    flags = inr ? CF_CRITICAL : 0;
+  }
+  return (uint8_t)(flags | (dir << 2));
+}
+
 }  // namespace mscx

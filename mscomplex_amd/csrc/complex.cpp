@@ -1,5 +1,6 @@
 // The product's GridMSComplex algorithms (see complex.hpp).
 #include "complex.hpp"
+#include "heap.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -459,17 +460,8 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
 // std::priority_queue produces for the reference's (uint, uint) elements.
 // Heap element: the persistence as the bit pattern of a non-negative float
 // (ordered like the float) and the slot of the (first, second) pair in a
-// side array, so sift operations move 8 bytes and read nothing else.
-struct HeapPair {
-  uint32_t key;
-  uint32_t slot;
-};
-struct PersistenceLess {
-  bool operator()(const HeapPair &p1, const HeapPair &p2) const {
-    if (p1.key != p2.key) return p1.key > p2.key;
-    return false;
-  }
-};
+// side array.  ExactHeap reproduces std::priority_queue's element moves, so
+// equal persistences leave in the reference's order.
 
 // simplify (grid_mscomplex.cpp:384-512)
 void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh) {
@@ -481,15 +473,14 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     uint32_t key;
     memcpy(&key, &p, 4);
     slots.emplace_back(a, b);
-    return HeapPair{key, (uint32_t)(slots.size() - 1)};
+    return ExactHeap::make(key, (uint32_t)(slots.size() - 1));
   };
-  std::vector<HeapPair> heap_store;
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
   m.arc_budget = (int64_t)(16 * narcs + ((size_t)1 << 20));
-  heap_store.reserve(narcs * 2 + 16);
+  ExactHeap pq;
+  pq.reserve(narcs * 2 + 16);
   slots.reserve(narcs * 4 + 16);
-  std::priority_queue<HeapPair, std::vector<HeapPair>, PersistenceLess> pq(PersistenceLess{}, std::move(heap_store));
 
   float max_val = std::numeric_limits<float>::min();
   float min_val = std::numeric_limits<float>::max();
@@ -505,16 +496,16 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     min_val = std::min(min_val, m.fn[i]);
     for (uint32_t d : m.des[i]) pq.push(mk(i, d));
   }
-  std::vector<HeapPair> resubmit;
+  std::vector<uint64_t> resubmit;
   float max_persistence = max_val - min_val;
   std::vector<uint32_t> new_edges;
   const Rect &R = m.rect;
   auto on_b = [&](uint32_t c) { return R.on_boundary(cx_of(c), cy_of(c)); };
 
   while (!pq.empty()) {
-    HeapPair hp = pq.top();
+    uint64_t hp = pq.top();
     pq.pop();
-    const P2 cp = slots[hp.slot];
+    const P2 cp = slots[ExactHeap::payload(hp)];
     uint32_t v1 = cp.first, v2 = cp.second;
     float pers = std::abs(m.fn[v1] - m.fn[v2]);
     if ((double)pers / (double)max_persistence > thresh) break;
@@ -539,7 +530,7 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     m.pair[v2] = v1;
     canc.push_back(cp);
     for (size_t i = 0; i < new_edges.size(); i += 2) pq.push(mk(new_edges[i], new_edges[i + 1]));
-    for (const HeapPair &r : resubmit) pq.push(r);
+    for (uint64_t r : resubmit) pq.push(r);
     resubmit.clear();
   }
 }

@@ -29,89 +29,203 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 __device__ __forceinline__ uint32_t pack_cell(int x, int y) { return (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16); }
 
-// ------------------------------------------------------------------ gradient
-// One block = GRAD_TX x GRAD_TY vertices (4 waves, one vertex row each, two
-// vertices per lane).  Vertex keys of the tile plus a 1/2 vertex halo are
-// staged in LDS; each lane emits the codes of its 2x2 cells per vertex, so a
-// wave writes one 256-cell collation segment of two cell rows and the
-// critical counts of both.  The successor of every vertex and quad (the
-// trace's starting pointers) is written here too.
-__global__ __launch_bounds__(256) void k_gradient(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                                  int ntiles) {
-  constexpr int LW = GRAD_TX + 3, LH = GRAD_TY + 3;
-  __shared__ uint32_t sk[LW * LH];
+// ------------------------------------------------------- gradient + trace
+// One block = a GRAD_TX x GRAD_TY vertex tile (cells 2*GRAD_TX x 2*GRAD_TY).
+//  1. vertex keys of the tile and a 1/2 vertex halo are staged in LDS
+//  2. the candidate pair of every vertex, horizontal and vertical edge the
+//     tile's cells refer to is computed once (vertex_cand / hedge_cand /
+//     vedge_cand) into LDS
+//  3. every cell's gradient code (code_from_cands) goes to LDS
+//  4. the gradient forests are traced inside the tile: each vertex points
+//     across its paired edge to the next vertex and each quad across its
+//     paired edge to the next quad (grid_dataset.cpp:1376-1427); pointer
+//     jumping in LDS resolves every path to its critical root, to "none", or
+//     to the first cell outside the tile (a marker)
+//  5. codes, the owner image (edges take the owner of their paired vertex or
+//     quad), per-segment critical counts and the list of marker entries are
+//     written with coalesced row stores.
+// A lane owns one vertex column (two cells per cell row).
+
+__device__ __forceinline__ bool lds_is_link(uint32_t w) { return (w & 0xFFFF8000u) == 0x00008000u; }
+
+__global__ __launch_bounds__(256) void k_grad_trace(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                                    int ntiles) {
+  constexpr int TX = GRAD_TX, TY = GRAD_TY;
+  constexpr int KW = TX + 3, KH = TY + 3;  // keys of vertices [i0-1, i0+TX+1] x [j0-1, j0+TY+1]
+  constexpr int VW = TX + 1, VH = TY + 1;  // vertex candidates [i0, i0+TX] x [j0, j0+TY]
+  constexpr int HW = TX, HH = TY + 1;      // horizontal-edge candidates [i0, i0+TX) x [j0, j0+TY]
+  constexpr int EW = TX + 1, EH = TY;      // vertical-edge candidates [i0, i0+TX] x [j0, j0+TY)
+  constexpr int CW = 2 * TX, CH = 2 * TY;  // cells of the tile
+  constexpr int NV = TX * TY;              // trace entries: vertices, then quads
+  __shared__ uint32_t sk[KW * KH];
+  __shared__ uint8_t svc[VW * VH], shc[HW * HH], sec[EW * EH];
+  __shared__ uint8_t scode[CW * CH];
+  __shared__ uint32_t sl[2 * NV];
+
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc &T = tiles[t];
   const TileGeom g = T.g;
   const int lb = b - prefix[t];
-  const int nbx = (g.vx + GRAD_TX - 1) / GRAD_TX;
+  const int nbx = (g.vx + TX - 1) / TX;
   const int bx = lb % nbx, by = lb / nbx;
-  const int i0 = bx * GRAD_TX, j0 = by * GRAD_TY;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int tid = threadIdx.x;
 
-  for (int idx = threadIdx.x; idx < LW * LH; idx += 256) {
-    int li = idx % LW, lj = idx / LW;
-    int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+  // 1. keys
+  for (int idx = tid; idx < KW * KH; idx += 256) {
+    const int gi = i0 - 1 + idx % KW, gj = j0 - 1 + idx / KW;
     uint32_t k = 0;
     if (gi >= 0 && gj >= 0 && gi < g.vx && gj < g.vy) k = fkey(T.verts[(int64_t)gj * T.vpitch + gi]);
     sk[idx] = k;
   }
   __syncthreads();
+  auto K = [&](int i, int j) -> uint64_t { return vkey(sk[(j - j0 + 1) * KW + (i - i0 + 1)], i, j); };
 
-  auto K = [&](int i, int j) -> uint64_t { return vkey(sk[(j - j0 + 1) * LW + (i - i0 + 1)], i, j); };
+  // 2. candidates (DIR_NONE outside the grid or the interior rect)
+  for (int idx = tid; idx < VW * VH; idx += 256) {
+    const int i = i0 + idx % VW, j = j0 + idx / VW;
+    int c = DIR_NONE;
+    if (i < g.vx && j < g.vy && in_rect(g, 2 * i, 2 * j)) c = vertex_cand(K, g, i, j);
+    svc[idx] = (uint8_t)c;
+  }
+  for (int idx = tid; idx < HW * HH; idx += 256) {
+    const int i = i0 + idx % HW, j = j0 + idx / HW;
+    int c = DIR_NONE;
+    if (i + 1 < g.vx && j < g.vy && in_rect(g, 2 * i + 1, 2 * j)) c = hedge_cand(K, g, i, j);
+    shc[idx] = (uint8_t)c;
+  }
+  for (int idx = tid; idx < EW * EH; idx += 256) {
+    const int i = i0 + idx % EW, j = j0 + idx / EW;
+    int c = DIR_NONE;
+    if (i < g.vx && j + 1 < g.vy && in_rect(g, 2 * i, 2 * j + 1)) c = vedge_cand(K, g, i, j);
+    sec[idx] = (uint8_t)c;
+  }
+  __syncthreads();
+  auto VC = [&](int i, int j) -> int { return svc[(j - j0) * VW + (i - i0)]; };
+  auto HC = [&](int i, int j) -> int { return shc[(j - j0) * HW + (i - i0)]; };
+  auto EC = [&](int i, int j) -> int { return sec[(j - j0) * EW + (i - i0)]; };
 
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int j = j0 + w;
-  if (j >= g.vy) return;
-  const int ia = i0 + 2 * lane;
-  uint32_t c0 = 0, c1 = 0, n0 = 0, n1 = 0;
-  const bool row1 = (2 * j + 1) < g.ch;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int cx = 2 * ia + k;
-    if (cx < g.cw) {
-      uint8_t c = cell_code(K, g, cx, 2 * j);
-      c0 |= (uint32_t)c << (8 * k);
-      n0 += (c >> 1) & 1;
-      if (row1) {
-        c = cell_code(K, g, cx, 2 * j + 1);
-        c1 |= (uint32_t)c << (8 * k);
-        n1 += (c >> 1) & 1;
+  // 3. codes of the tile's cells
+  for (int idx = tid; idx < CW * CH; idx += 256) {
+    const int cx = 2 * i0 + idx % CW, cy = 2 * j0 + idx / CW;
+    uint8_t c = 0;
+    if (cx < g.cw && cy < g.ch) c = code_from_cands(K, VC, HC, EC, g, cx, cy);
+    scode[idx] = c;
+  }
+  __syncthreads();
+
+  // 4. block-local trace.  LDS word: 0x8000 | entry = link to another entry
+  //    of this tile; otherwise the final owner value (packed root id,
+  //    OWN_NONE, or OWN_MARK | id of the first cell outside the tile).
+  for (int e = tid; e < 2 * NV; e += 256) {
+    const bool quad = e >= NV;
+    const int li = (e % NV) % TX, lj = (e % NV) / TX;
+    const int lx = 2 * li + quad, ly = 2 * lj + quad;
+    const int cx = 2 * i0 + lx, cy = 2 * j0 + ly;
+    uint32_t w = OWN_NONE;
+    if (cx < g.cw && cy < g.ch) {
+      const uint8_t c = scode[ly * CW + lx];
+      if (c & CF_CRITICAL) {
+        w = pack_cell(T.ox + cx, T.oy + cy);
+      } else if (c & CF_PAIRED) {
+        const int d = code_dir(c);
+        const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
+        if (ni >= 0 && nj >= 0 && ni < TX && nj < TY)
+          w = 0x8000u | (uint32_t)((quad ? NV : 0) + nj * TX + ni);
+        else
+          w = OWN_MARK | pack_cell(T.ox + cx + 2 * dir_dx(d), T.oy + cy + 2 * dir_dy(d));
       }
     }
+    sl[e] = w;
   }
-  uint8_t *row = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i0 + 4 * lane;
-  *(uint32_t *)row = c0;
-  if (row1) *(uint32_t *)(row + T.pitch) = c1;
-
-  // successors for the trace
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    int i = ia + k;
-    if (i < g.vx) {
-      uint8_t cv = (uint8_t)(c0 >> (16 * k));
-      int32_t p = -1;
-      if ((cv & 3) == CF_PAIRED) {
-        int d = code_dir(cv);
-        p = (j + dir_dy(d)) * g.vx + (i + dir_dx(d));
+  __syncthreads();
+  for (;;) {
+    int more = 0;
+    for (int e = tid; e < 2 * NV; e += 256) {
+      const uint32_t w = sl[e];
+      if (lds_is_link(w)) {
+        const uint32_t w2 = sl[w & 0x7FFFu];
+        sl[e] = w2;
+        more |= lds_is_link(w2);
       }
-      T.vptr[(int64_t)j * g.vx + i] = p;
-      if (row1 && i + 1 < g.vx) {
-        uint8_t cq = (uint8_t)(c1 >> (16 * k + 8));
-        int32_t q = -1;
-        if ((cq & 3) == CF_PAIRED) {
-          int d = code_dir(cq);
-          q = (j + dir_dy(d)) * (g.vx - 1) + (i + dir_dx(d));
+    }
+    if (!__syncthreads_or(more)) break;
+  }
+
+  // 5. row stores: codes, owners, critical counts, marker list
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int r = wv; r < CH; r += 4) {
+    const int cy = 2 * j0 + r;
+    if (cy >= g.ch) break;
+    const int lx = 2 * lane, cx = 2 * i0 + lx;
+    const uint8_t c0 = scode[r * CW + lx], c1 = scode[r * CW + lx + 1];
+    *(uint16_t *)(T.codes + (int64_t)cy * T.pitch + cx) = (uint16_t)(c0 | (c1 << 8));
+    uint32_t o[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint8_t c = k ? c1 : c0;
+      const int x = lx + k, gx = cx + k;
+      uint32_t v = OWN_NONE;
+      if (gx < g.cw) {
+        const int dx = x & 1, dy = r & 1;
+        if (dx == dy) {  // vertex (dx = 0) or quad (dx = 1)
+          v = sl[(dx ? NV : 0) + (r >> 1) * TX + (x >> 1)];
+        } else if (c & CF_CRITICAL) {
+          v = pack_cell(T.ox + gx, T.oy + cy);
+        } else if (c & CF_PAIRED) {
+          const int d = code_dir(c);
+          const int px = x + dir_dx(d), py = r + dir_dy(d);
+          if (px >= 0 && py >= 0 && px < CW && py < CH)
+            v = sl[((px & 1) ? NV : 0) + (py >> 1) * TX + (px >> 1)];
+          else
+            v = OWN_MARK | pack_cell(T.ox + gx + dir_dx(d), T.oy + cy + dir_dy(d));
         }
-        T.qptr[(int64_t)j * (g.vx - 1) + i] = q;
       }
+      o[k] = v;
+    }
+    *(uint2 *)(T.own + (int64_t)cy * T.own_pitch + cx) = make_uint2(o[0], o[1]);
+    const uint64_t crit_mask = __ballot((c0 >> 1) & 1) , crit_mask1 = __ballot((c1 >> 1) & 1);
+    if (lane == 0) T.seg_counts[(int64_t)cy * T.nseg + bx] = (uint32_t)(__popcll(crit_mask) + __popcll(crit_mask1));
+    const bool m0 = own_is_mark(o[0]), m1 = own_is_mark(o[1]);
+    const uint64_t mm0 = __ballot(m0), mm1 = __ballot(m1);
+    const uint32_t nm = (uint32_t)(__popcll(mm0) + __popcll(mm1));
+    if (nm) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(T.nexit, nm);
+      base = __shfl(base, 0, 64);
+      const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      uint32_t pos = base + (uint32_t)(__popcll(mm0 & lt) + __popcll(mm1 & lt));
+      const uint32_t rowbase = (uint32_t)cy * (uint32_t)T.own_pitch + (uint32_t)cx;
+      if (m0) T.exits[pos++] = rowbase;
+      if (m1) T.exits[pos] = rowbase + 1;
     }
   }
+}
 
-  uint32_t s0 = wave_sum(n0), s1 = wave_sum(n1);
-  if (lane == 0) {
-    T.seg_counts[(int64_t)(2 * j) * T.nseg + bx] = s0;
-    if (row1) T.seg_counts[(int64_t)(2 * j + 1) * T.nseg + bx] = s1;
+// Resolves the marker entries left by k_grad_trace: each follows the chain of
+// markers (first cell outside a tile -> that cell's owner entry -> ...) to
+// the owner at its end.  Chains follow gradient paths, so they end; the
+// bound only guards against a corrupted image.
+__global__ __launch_bounds__(256) void k_resolve_exits(const TileDesc *__restrict__ tiles, int ntiles,
+                                                       uint32_t *err) {
+  const uint32_t gid = blockIdx.x * 256 + threadIdx.x, gsz = gridDim.x * 256;
+  for (int t = 0; t < ntiles; ++t) {
+    const TileDesc &T = tiles[t];
+    const uint32_t n = *T.nexit;
+    for (uint32_t e = gid; e < n; e += gsz) {
+      const uint32_t idx = T.exits[e];
+      uint32_t w = T.own[idx];
+      int steps = 0;
+      while (own_is_mark(w) && steps < (1 << 24)) {
+        const uint32_t c = w & ~OWN_MARK;
+        const int x = (int)(c & 0xFFFF) - T.ox, y = (int)(c >> 16) - T.oy;
+        w = T.own[(int64_t)y * T.own_pitch + x];
+        ++steps;
+      }
+      if (own_is_mark(w)) atomicOr(err, 1u);
+      T.own[idx] = w;
+    }
   }
 }
 
@@ -248,121 +362,28 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ ti
   const int64_t wid = (int64_t)lb * 4 + (threadIdx.x >> 6);
   if (wid >= (int64_t)g.ch * T.nseg) return;
   const int row = (int)(wid / T.nseg), seg = (int)(wid % T.nseg);
-  const uint32_t word = *(const uint32_t *)(T.codes + (int64_t)row * T.pitch + seg * SEG + 4 * lane);
-  uint32_t base = T.seg_offs[wid];
+  const uint16_t word = *(const uint16_t *)(T.codes + (int64_t)row * T.pitch + seg * SEG + 2 * lane);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t before = 0;
+  const uint64_t m0 = __ballot((word >> 1) & 1), m1 = __ballot((word >> 9) & 1);
+  uint32_t idx = T.seg_offs[wid] + (uint32_t)(__popcll(m0 & lt) + __popcll(m1 & lt));
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    uint64_t m = __ballot((word >> (8 * k + 1)) & 1);
-    before += __popcll(m & lt);
-  }
-  uint32_t idx = base + before;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    uint8_t c = (uint8_t)(word >> (8 * k));
+  for (int k = 0; k < 2; ++k) {
+    const uint8_t c = (uint8_t)(word >> (8 * k));
     if (c & CF_CRITICAL) {
-      int cx = seg * SEG + 4 * lane + k, cy = row;
+      const int cx = seg * SEG + 2 * lane + k, cy = row;
       if (idx < T.kcap) {
         T.crit[idx] = pack_cell(T.ox + cx, T.oy + cy);
         T.crit_fn[idx] = cell_fn(T, cx, cy);
-        if (!(cx & 1) && !(cy & 1)) T.vptr[(int64_t)(cy >> 1) * g.vx + (cx >> 1)] = -(int32_t)idx - 2;
-        if ((cx & 1) && (cy & 1)) T.qptr[(int64_t)(cy >> 1) * (g.vx - 1) + (cx >> 1)] = -(int32_t)idx - 2;
       }
       ++idx;
     }
   }
 }
 
-// --------------------------------------------------------------------- trace
-// One pointer-jumping round over every vertex and quad pointer of every
-// tile: p <- ptr[p] while p names another vertex/quad.  Rounds repeat until
-// no pointer is unresolved (the host checks *changed every few rounds).
-
-__global__ __launch_bounds__(256) void k_jump(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                              int ntiles, uint32_t *changed) {
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
-  const int lb = b - prefix[t];
-  const int64_t nv = (int64_t)T.g.vx * T.g.vy, nq = (int64_t)(T.g.vx - 1) * (T.g.vy - 1);
-  bool any = false;
-  for (int k = 0; k < 4; ++k) {
-    int64_t e = ((int64_t)lb * 4 + k) * 256 + threadIdx.x;
-    int32_t *arr;
-    int64_t i;
-    if (e < nv) {
-      arr = T.vptr;
-      i = e;
-    } else if (e < nv + nq) {
-      arr = T.qptr;
-      i = e - nv;
-    } else {
-      break;
-    }
-    int32_t p = arr[i];
-    if (p >= 0) {
-      int32_t q = arr[p];
-      arr[i] = q;
-      any |= (q >= 0);
-    }
-  }
-  if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(changed, 1u);
-}
-
-// ---------------------------------------------------------------- owners
-// Owner image (the reference's m_cell_own): critical cells own themselves;
-// vertices and quads take their resolved root; an edge paired with a vertex
-// (quad) takes that vertex's (quad's) owner; cells never reached stay ~0.
-
-__device__ __forceinline__ uint32_t own_of_ptr(const TileDesc &T, int32_t p) {
-  return (p <= -2) ? T.crit[-p - 2] : 0xFFFFFFFFu;
-}
-
-__global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                               int ntiles) {
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
-  const TileGeom g = T.g;
-  const int lb = b - prefix[t];
-  const int lane = threadIdx.x & 63;
-  const int64_t wid = (int64_t)lb * 4 + (threadIdx.x >> 6);
-  if (wid >= (int64_t)g.ch * T.nseg) return;
-  const int row = (int)(wid / T.nseg), seg = (int)(wid % T.nseg);
-  const uint32_t word = *(const uint32_t *)(T.codes + (int64_t)row * T.pitch + seg * SEG + 4 * lane);
-  uint32_t o[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    uint8_t c = (uint8_t)(word >> (8 * k));
-    int cx = seg * SEG + 4 * lane + k, cy = row;
-    uint32_t v = 0xFFFFFFFFu;
-    if (cx < g.cw) {
-      int dx = cx & 1, dy = cy & 1;
-      if (dx == dy) {  // vertex or quad
-        v = dx ? own_of_ptr(T, T.qptr[(int64_t)(cy >> 1) * (g.vx - 1) + (cx >> 1)])
-               : own_of_ptr(T, T.vptr[(int64_t)(cy >> 1) * g.vx + (cx >> 1)]);
-      } else if (c & CF_CRITICAL) {
-        v = pack_cell(T.ox + cx, T.oy + cy);
-      } else if (c & CF_PAIRED) {
-        int d = code_dir(c);
-        int px = cx + dir_dx(d), py = cy + dir_dy(d);
-        v = (px & 1) ? own_of_ptr(T, T.qptr[(int64_t)(py >> 1) * (g.vx - 1) + (px >> 1)])
-                     : own_of_ptr(T, T.vptr[(int64_t)(py >> 1) * g.vx + (px >> 1)]);
-      }
-    }
-    o[k] = v;
-  }
-  uint4 *dst = (uint4 *)(T.own + (int64_t)row * T.own_pitch + seg * SEG + 4 * lane);
-  *dst = make_uint4(o[0], o[1], o[2], o[3]);
-}
-
 // --------------------------------------------------------------- incidence
 // Per critical cell: for saddles, the critical index owning each facet and
 // each in-ext cofacet (writeout_connectivity, grid_dataset.cpp:1876-1898);
 // for boundary-cancellable cells, the critical index of the pair.
-
-__device__ __forceinline__ uint32_t idx_of_ptr(int32_t p) { return (p <= -2) ? (uint32_t)(-p - 2) : 0xFFFFFFFFu; }
 
 __device__ uint32_t find_crit(const TileDesc &T, uint32_t K, int gx, int gy) {
   // packed cellids (x | y << 16) sort numerically in raster (y, x) order
@@ -376,6 +397,14 @@ __device__ uint32_t find_crit(const TileDesc &T, uint32_t K, int gx, int gy) {
       hi = mid;
   }
   return (lo < K && T.crit[lo] == key) ? lo : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t own_at(const TileDesc &T, int cx, int cy) {
+  return T.own[(int64_t)cy * T.own_pitch + cx];
+}
+
+__device__ __forceinline__ uint32_t crit_of_owner(const TileDesc &T, uint32_t K, uint32_t o) {
+  return o == OWN_NONE ? 0xFFFFFFFFu : find_crit(T, K, (int)(o & 0xFFFF), (int)(o >> 16));
 }
 
 __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
@@ -394,27 +423,18 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
   const int dx = cx & 1, dy = cy & 1;
   uint4 conn = make_uint4(~0u, ~0u, ~0u, ~0u);
   if (dx != dy) {
-    // facets: the two vertices; cofacets: the two quads across the edge
-    int fx0 = cx + dx, fy0 = cy + dy, fx1 = cx - dx, fy1 = cy - dy;
-    conn.x = idx_of_ptr(T.vptr[(int64_t)(fy0 >> 1) * g.vx + (fx0 >> 1)]);
-    conn.y = idx_of_ptr(T.vptr[(int64_t)(fy1 >> 1) * g.vx + (fx1 >> 1)]);
-    int qx0 = cx + dy, qy0 = cy + dx, qx1 = cx - dy, qy1 = cy - dx;
-    if (qx0 >= 0 && qy0 >= 0 && qx0 < g.cw && qy0 < g.ch)
-      conn.z = idx_of_ptr(T.qptr[(int64_t)(qy0 >> 1) * (g.vx - 1) + (qx0 >> 1)]);
-    if (qx1 >= 0 && qy1 >= 0 && qx1 < g.cw && qy1 < g.ch)
-      conn.w = idx_of_ptr(T.qptr[(int64_t)(qy1 >> 1) * (g.vx - 1) + (qx1 >> 1)]);
+    // facets: the two vertices; cofacets: the quads across the edge in ext
+    conn.x = crit_of_owner(T, K, own_at(T, cx + dx, cy + dy));
+    conn.y = crit_of_owner(T, K, own_at(T, cx - dx, cy - dy));
+    const int qx0 = cx + dy, qy0 = cy + dx, qx1 = cx - dy, qy1 = cy - dx;
+    if (qx0 >= 0 && qy0 >= 0 && qx0 < g.cw && qy0 < g.ch) conn.z = crit_of_owner(T, K, own_at(T, qx0, qy0));
+    if (qx1 >= 0 && qy1 >= 0 && qx1 < g.cw && qy1 < g.ch) conn.w = crit_of_owner(T, K, own_at(T, qx1, qy1));
   }
   ((uint4 *)T.crit_conn)[k] = conn;
   uint32_t pidx = ~0u;
   if (code & CF_PAIRED) {
-    int d = code_dir(code);
-    int px = cx + dir_dx(d), py = cy + dir_dy(d);
-    int pdx = px & 1, pdy = py & 1;
-    if (pdx == pdy)
-      pidx = pdx ? idx_of_ptr(T.qptr[(int64_t)(py >> 1) * (g.vx - 1) + (px >> 1)])
-                 : idx_of_ptr(T.vptr[(int64_t)(py >> 1) * g.vx + (px >> 1)]);
-    else
-      pidx = find_crit(T, K, T.ox + px, T.oy + py);
+    const int d = code_dir(code);
+    pidx = find_crit(T, K, T.ox + cx + dir_dx(d), T.oy + cy + dir_dy(d));
   }
   T.crit_pair[k] = pidx;
 }
@@ -483,10 +503,6 @@ static int blocks_of(int kind, const TileDesc &T) {
       int64_t n = (int64_t)g.ch * T.nseg;
       return (int)((n + 3) / 4);
     }
-    case KB_JUMP: {
-      int64_t n = (int64_t)g.vx * g.vy + (int64_t)(g.vx - 1) * (g.vy - 1);
-      return (int)((n + 1023) / 1024);
-    }
     case KB_CRIT:
       return (int)((T.kcap + 255) / 256);
   }
@@ -519,8 +535,12 @@ void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int 
   if (nblocks) hipLaunchKernelGGL(k_disc_labels, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
 }
 
-void launch_gradient(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
-  if (nblocks) hipLaunchKernelGGL(k_gradient, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
+void launch_grad_trace(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_grad_trace, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
+}
+void launch_resolve_exits(const TileDesc *d_tiles, int ntiles, int nblocks, uint32_t *d_err, hipStream_t s) {
+  if (nblocks && ntiles)
+    hipLaunchKernelGGL(k_resolve_exits, dim3(nblocks), dim3(256), 0, s, d_tiles, ntiles, d_err);
 }
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (!nblocks) return;
@@ -530,13 +550,6 @@ void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int n
 }
 void launch_compact(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_compact, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
-}
-void launch_jump(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_changed,
-                 hipStream_t s) {
-  if (nblocks) hipLaunchKernelGGL(k_jump, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles, d_changed);
-}
-void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
-  if (nblocks) hipLaunchKernelGGL(k_owner, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
 }
 void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_incidence, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);

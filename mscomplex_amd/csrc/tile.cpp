@@ -42,7 +42,6 @@ static void alloc_fixed(mscx_tile *t) {
   d.own_pitch = d.pitch;
   const int64_t nrs = (int64_t)g.ch * d.nseg;
   const int64_t nparts = (nrs + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  const int64_t nv = (int64_t)g.vx * g.vy, nq = (int64_t)(g.vx - 1) * (g.vy - 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -51,8 +50,8 @@ static void alloc_fixed(mscx_tile *t) {
   };
   size_t o_codes = take((size_t)g.ch * d.pitch);
   size_t o_cnt = take(nrs * 4), o_offs = take(nrs * 4), o_part = take(nparts * 4 + 4);
-  size_t o_vptr = take(nv * 4), o_qptr = take(nq * 4 + 4);
   size_t o_own = take((size_t)g.ch * d.own_pitch * 4);
+  size_t o_exit = take((size_t)g.ch * d.own_pitch * 4);
   t->fixed_bytes = off;
   MSCX_HIP(hipMalloc(&t->fixed, off));
   char *base = (char *)t->fixed;
@@ -60,9 +59,8 @@ static void alloc_fixed(mscx_tile *t) {
   d.seg_counts = (uint32_t *)(base + o_cnt);
   d.seg_offs = (uint32_t *)(base + o_offs);
   d.scan_part = (uint32_t *)(base + o_part);
-  d.vptr = (int32_t *)(base + o_vptr);
-  d.qptr = (int32_t *)(base + o_qptr);
   d.own = (uint32_t *)(base + o_own);
+  d.exits = (uint32_t *)(base + o_exit);
 }
 
 static void ensure_kcap(mscx_tile *t, size_t K) {
@@ -90,47 +88,61 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   hipStream_t s = rt.stream;
   for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i] && tiles[i]->has_input, MSCX_ESTATE, "tile has no input bound");
 
-  // Layout of the batch scratch: descriptors | prefixes (KB_COUNT x (n+1)) | K[n] | changed
+  // Layout of the batch scratch: descriptors | prefixes (KB_COUNT x (n+1)) |
+  // K[n] | marker counts[n] | error flag.  The counts and the flag are zeroed
+  // by the descriptor upload itself.
   const size_t o_desc = 0;
   const size_t o_pref = align_up(sizeof(TileDesc) * n, 256);
   const size_t o_k = align_up(o_pref + sizeof(int) * KB_COUNT * (n + 1), 256);
-  const size_t o_chg = align_up(o_k + 4 * n, 256);
-  const size_t total = o_chg + 256;
+  const size_t o_nx = align_up(o_k + 4 * n, 256);
+  const size_t o_err = align_up(o_nx + 4 * n, 256);
+  const size_t total = o_err + 256;
   char *dev = (char *)g_batch.reserve_dev(total);
   char *host = (char *)g_batch.reserve_host(total);
   TileDesc *hd = (TileDesc *)(host + o_desc);
   int *hp = (int *)(host + o_pref);
   uint32_t *hk = (uint32_t *)(host + o_k);
+  uint32_t *hnx = (uint32_t *)(host + o_nx);
+  uint32_t *herr = (uint32_t *)(host + o_err);
   TileDesc *dd = (TileDesc *)(dev + o_desc);
   int *dp = (int *)(dev + o_pref);
   uint32_t *dk = (uint32_t *)(dev + o_k);
-  uint32_t *dchg = (uint32_t *)(dev + o_chg);
-  uint32_t *hchg = (uint32_t *)(host + o_chg);
+  uint32_t *dnx = (uint32_t *)(dev + o_nx);
+  uint32_t *derr = (uint32_t *)(dev + o_err);
 
   for (int i = 0; i < n; ++i) {
     hd[i] = tiles[i]->d;
     hd[i].nkrit = dk + i;
+    hd[i].nexit = dnx + i;
     hd[i].kcap = 0;
+    hk[i] = 0;
+    hnx[i] = 0;
   }
+  *herr = 0;
   int nb[KB_COUNT];
   for (int k = 0; k < KB_CRIT; ++k) nb[k] = plan_blocks(k, hd, n, hp + k * (n + 1));
-  MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
 
   StageTimer &tm = stage_timer();
   tm.begin(ST_GRADIENT, s);
-  launch_gradient(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
+  launch_grad_trace(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
   tm.end(ST_GRADIENT, s);
   tm.begin(ST_SCAN, s);
   launch_scan(dd, dp + KB_SCAN * (n + 1), n, nb[KB_SCAN], s);
   tm.end(ST_SCAN, s);
+  tm.begin(ST_TRACE, s);
+  launch_resolve_exits(dd, n, 2048, derr, s);
+  tm.end(ST_TRACE, s);
   MSCX_HIP(hipGetLastError());
-  MSCX_HIP(hipMemcpyAsync(hk, dk, 4 * n, hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipMemcpyAsync(hk, dk, o_err + 4 - o_k, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipStreamSynchronize(s));
   tm.collect();
+  MSCX_REQUIRE(*herr == 0, MSCX_ELOGIC, "owner trace did not resolve");
 
   for (int i = 0; i < n; ++i) {
     ensure_kcap(tiles[i], hk[i]);
     tiles[i]->K = hk[i];
+    tiles[i]->nexit = hnx[i];
     hd[i].crit = tiles[i]->d.crit;
     hd[i].crit_fn = tiles[i]->d.crit_fn;
     hd[i].crit_conn = tiles[i]->d.crit_conn;
@@ -143,33 +155,13 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   tm.begin(ST_COMPACT, s);
   launch_compact(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
   tm.end(ST_COMPACT, s);
-  // Pointer jumping: rounds in groups of 4 between convergence checks.
-  uint32_t rounds = 0;
-  for (;;) {
-    MSCX_HIP(hipMemsetAsync(dchg, 0, 4, s));
-    tm.begin(ST_TRACE, s);
-    for (int r = 0; r < 4; ++r) launch_jump(dd, dp + KB_JUMP * (n + 1), n, nb[KB_JUMP], dchg, s);
-    tm.end(ST_TRACE, s);
-    rounds += 4;
-    MSCX_HIP(hipMemcpyAsync(hchg, dchg, 4, hipMemcpyDeviceToHost, s));
-    MSCX_HIP(hipStreamSynchronize(s));
-    tm.collect();
-    if (*hchg == 0) break;
-    MSCX_REQUIRE(rounds < 256, MSCX_ELOGIC, "trace did not converge");
-  }
-  tm.begin(ST_OWNER, s);
-  launch_owner(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
-  tm.end(ST_OWNER, s);
   tm.begin(ST_INCIDENCE, s);
   launch_incidence(dd, dp + KB_CRIT * (n + 1), n, nb[KB_CRIT], s);
   tm.end(ST_INCIDENCE, s);
   MSCX_HIP(hipGetLastError());
   MSCX_HIP(hipStreamSynchronize(s));
   tm.collect();
-  for (int i = 0; i < n; ++i) {
-    tiles[i]->worked = true;
-    tiles[i]->jump_rounds = rounds;
-  }
+  for (int i = 0; i < n; ++i) tiles[i]->worked = true;
 }
 
 void tile_critical(const mscx_tile *t, std::vector<uint32_t> &crit, std::vector<float> &fn,

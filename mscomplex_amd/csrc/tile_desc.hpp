@@ -9,9 +9,9 @@
 
 namespace mscx {
 
-constexpr int SEG = 256;          // cells per collation segment (one wavefront row of the gradient kernel)
-constexpr int GRAD_TX = 128;      // vertices per gradient tile in x (64 lanes x 2)
-constexpr int GRAD_TY = 4;        // vertex rows per gradient tile (one per wave)
+constexpr int GRAD_TX = 64;       // vertices per gradient/trace block tile in x (one per lane)
+constexpr int GRAD_TY = 32;       // vertex rows per block tile
+constexpr int SEG = 2 * GRAD_TX;  // cells per collation segment: one block tile row
 constexpr int SCAN_CHUNK = 2048;  // counts per scan block
 
 struct TileDesc {
@@ -19,7 +19,7 @@ struct TileDesc {
   TileGeom g;
   int ox, oy;  // global cell coordinates of the ext rect's bottom-left
   int nseg;    // collation segments per cell row (pitch / SEG)
-  int pitch;   // bytes per code row (nseg * SEG)
+  int pitch;   // cells per code / owner row (nseg * SEG)
   // input: ext window of vertex values, x fastest, row pitch in floats
   const float *verts;
   int64_t vpitch;
@@ -36,15 +36,20 @@ struct TileDesc {
   float *crit_fn;
   uint32_t *crit_conn;  // 4 per critical cell
   uint32_t *crit_pair;  // critical index of the pair (boundary-cancellable) or ~0
-  // trace: per vertex / per quad successor; >=0 another vertex/quad,
-  // -1 unowned, <= -2 resolved to critical index (-v - 2)
-  int32_t *vptr;  // vy * vx
-  int32_t *qptr;  // (vy - 1) * (vx - 1)
-  // owners: packed cellid per cell (x | y << 16), ch rows of own_pitch words
+  // owners: packed cellid per cell (x | y << 16), ch rows of own_pitch
+  // words; between the trace kernels an entry may hold OWN_MARK | packed id
+  // of a cell whose owner it takes (a path that left its block tile)
   uint32_t *own;
   int own_pitch;
+  // owner entries that hold a marker after the block-local trace
+  uint32_t *exits;
+  uint32_t *nexit;
   uint32_t kcap;  // capacity of the crit arrays
 };
+
+constexpr uint32_t OWN_NONE = 0xFFFFFFFFu;  // (-1,-1)
+constexpr uint32_t OWN_MARK = 0x80000000u;  // cell ids never have bit 31 set (y <= 32766)
+MSCX_HD bool own_is_mark(uint32_t w) { return (w & OWN_MARK) && w != OWN_NONE; }
 
 // Per-leaf inputs of the manifold label pass (postMergeFillDiscs,
 // grid_dataset.cpp:1142-1187): owner image, owner redirects of cancelled

@@ -25,5 +25,22 @@ extern "C" int cr_codes(const float *win, const int *rect, const int *ext, uint8
   } catch (int) {
     return -1;
   }
+  // the decomposed form must agree with the per-cell form everywhere
+  auto VC = [&](int i, int j) {
+    return mscx::in_rect(g, 2 * i, 2 * j) ? mscx::vertex_cand(K, g, i, j) : mscx::DIR_NONE;
+  };
+  auto HC = [&](int i, int j) {
+    return mscx::in_rect(g, 2 * i + 1, 2 * j) ? mscx::hedge_cand(K, g, i, j) : mscx::DIR_NONE;
+  };
+  auto EC = [&](int i, int j) {
+    return mscx::in_rect(g, 2 * i, 2 * j + 1) ? mscx::vedge_cand(K, g, i, j) : mscx::DIR_NONE;
+  };
+  try {
+    for (int y = 0; y < g.ch; ++y)
+      for (int x = 0; x < g.cw; ++x)
+        if (mscx::code_from_cands(K, VC, HC, EC, g, x, y) != codes[(size_t)y * g.cw + x]) return -2;
+  } catch (int) {
+    return -3;
+  }
   return 0;
 }
