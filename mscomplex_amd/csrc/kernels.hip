@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void k_grad_trace(const TileDesc *__restrict__
   __shared__ uint8_t svc[VW * VH], shc[HW * HH], sec[EW * EH];
   __shared__ uint8_t scode[CW * CH];
   __shared__ uint32_t sl[2 * NV];
+  __shared__ uint32_t s_nexit;
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
@@ -153,7 +154,11 @@ __global__ __launch_bounds__(256) void k_grad_trace(const TileDesc *__restrict__
     if (!__syncthreads_or(more)) break;
   }
 
-  // 5. row stores: codes, owners, critical counts, marker list
+  // 5. row stores: codes, owners, critical counts, marker list (the block's
+  //    own region of T.exits, CW * CH entries, count in T.nexit[lb])
+  if (tid == 0) s_nexit = 0;
+  __syncthreads();
+  uint32_t *const xlist = T.exits + (int64_t)lb * (CW * CH);
   const int lane = tid & 63, wv = tid >> 6;
   for (int r = wv; r < CH; r += 4) {
     const int cy = 2 * j0 + r;
@@ -192,40 +197,43 @@ __global__ __launch_bounds__(256) void k_grad_trace(const TileDesc *__restrict__
     const uint32_t nm = (uint32_t)(__popcll(mm0) + __popcll(mm1));
     if (nm) {
       uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(T.nexit, nm);
+      if (lane == 0) base = atomicAdd(&s_nexit, nm);
       base = __shfl(base, 0, 64);
       const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
       uint32_t pos = base + (uint32_t)(__popcll(mm0 & lt) + __popcll(mm1 & lt));
       const uint32_t rowbase = (uint32_t)cy * (uint32_t)T.own_pitch + (uint32_t)cx;
-      if (m0) T.exits[pos++] = rowbase;
-      if (m1) T.exits[pos] = rowbase + 1;
+      if (m0) xlist[pos++] = rowbase;
+      if (m1) xlist[pos] = rowbase + 1;
     }
   }
+  __syncthreads();
+  if (tid == 0) T.nexit[lb] = s_nexit;
 }
 
 // Resolves the marker entries left by k_grad_trace: each follows the chain of
 // markers (first cell outside a tile -> that cell's owner entry -> ...) to
 // the owner at its end.  Chains follow gradient paths, so they end; the
 // bound only guards against a corrupted image.
-__global__ __launch_bounds__(256) void k_resolve_exits(const TileDesc *__restrict__ tiles, int ntiles,
-                                                       uint32_t *err) {
-  const uint32_t gid = blockIdx.x * 256 + threadIdx.x, gsz = gridDim.x * 256;
-  for (int t = 0; t < ntiles; ++t) {
-    const TileDesc &T = tiles[t];
-    const uint32_t n = *T.nexit;
-    for (uint32_t e = gid; e < n; e += gsz) {
-      const uint32_t idx = T.exits[e];
-      uint32_t w = T.own[idx];
-      int steps = 0;
-      while (own_is_mark(w) && steps < (1 << 24)) {
-        const uint32_t c = w & ~OWN_MARK;
-        const int x = (int)(c & 0xFFFF) - T.ox, y = (int)(c >> 16) - T.oy;
-        w = T.own[(int64_t)y * T.own_pitch + x];
-        ++steps;
-      }
-      if (own_is_mark(w)) atomicOr(err, 1u);
-      T.own[idx] = w;
+__global__ __launch_bounds__(256) void k_resolve_exits(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                                       int ntiles, uint32_t *err) {
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, ntiles, b);
+  const TileDesc &T = tiles[t];
+  const int lb = b - prefix[t];
+  const uint32_t n = T.nexit[lb];
+  const uint32_t *xlist = T.exits + (int64_t)lb * (4 * GRAD_TX * GRAD_TY);
+  for (uint32_t e = threadIdx.x; e < n; e += 256) {
+    const uint32_t idx = xlist[e];
+    uint32_t w = T.own[idx];
+    int steps = 0;
+    while (own_is_mark(w) && steps < (1 << 24)) {
+      const uint32_t c = w & ~OWN_MARK;
+      const int x = (int)(c & 0xFFFF) - T.ox, y = (int)(c >> 16) - T.oy;
+      w = T.own[(int64_t)y * T.own_pitch + x];
+      ++steps;
     }
+    if (own_is_mark(w)) atomicOr(err, 1u);
+    T.own[idx] = w;
   }
 }
 
@@ -538,9 +546,9 @@ void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int 
 void launch_grad_trace(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_grad_trace, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
 }
-void launch_resolve_exits(const TileDesc *d_tiles, int ntiles, int nblocks, uint32_t *d_err, hipStream_t s) {
-  if (nblocks && ntiles)
-    hipLaunchKernelGGL(k_resolve_exits, dim3(nblocks), dim3(256), 0, s, d_tiles, ntiles, d_err);
+void launch_resolve_exits(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                          hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_resolve_exits, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles, d_err);
 }
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (!nblocks) return;

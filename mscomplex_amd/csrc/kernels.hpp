@@ -14,7 +14,8 @@ enum BlockKind { KB_GRADIENT = 0, KB_SCAN = 1, KB_ROWSEG = 2, KB_CRIT = 3, KB_CO
 int plan_blocks(int kind, const TileDesc *tiles, int ntiles, int *prefix);
 
 void launch_grad_trace(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
-void launch_resolve_exits(const TileDesc *d_tiles, int ntiles, int nblocks, uint32_t *d_err, hipStream_t s);
+void launch_resolve_exits(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                          hipStream_t s);
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_compact(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);

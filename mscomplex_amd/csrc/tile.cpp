@@ -50,8 +50,10 @@ static void alloc_fixed(mscx_tile *t) {
   };
   size_t o_codes = take((size_t)g.ch * d.pitch);
   size_t o_cnt = take(nrs * 4), o_offs = take(nrs * 4), o_part = take(nparts * 4 + 4);
+  const int64_t nblk = (int64_t)((g.vx + GRAD_TX - 1) / GRAD_TX) * ((g.vy + GRAD_TY - 1) / GRAD_TY);
   size_t o_own = take((size_t)g.ch * d.own_pitch * 4);
-  size_t o_exit = take((size_t)g.ch * d.own_pitch * 4);
+  size_t o_exit = take((size_t)nblk * 4 * GRAD_TX * GRAD_TY * 4);
+  size_t o_nexit = take((size_t)nblk * 4);
   t->fixed_bytes = off;
   MSCX_HIP(hipMalloc(&t->fixed, off));
   char *base = (char *)t->fixed;
@@ -61,6 +63,7 @@ static void alloc_fixed(mscx_tile *t) {
   d.scan_part = (uint32_t *)(base + o_part);
   d.own = (uint32_t *)(base + o_own);
   d.exits = (uint32_t *)(base + o_exit);
+  d.nexit = (uint32_t *)(base + o_nexit);
 }
 
 static void ensure_kcap(mscx_tile *t, size_t K) {
@@ -89,34 +92,28 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i] && tiles[i]->has_input, MSCX_ESTATE, "tile has no input bound");
 
   // Layout of the batch scratch: descriptors | prefixes (KB_COUNT x (n+1)) |
-  // K[n] | marker counts[n] | error flag.  The counts and the flag are zeroed
-  // by the descriptor upload itself.
+  // K[n] | error flag (zeroed by the descriptor upload itself).
   const size_t o_desc = 0;
   const size_t o_pref = align_up(sizeof(TileDesc) * n, 256);
   const size_t o_k = align_up(o_pref + sizeof(int) * KB_COUNT * (n + 1), 256);
-  const size_t o_nx = align_up(o_k + 4 * n, 256);
-  const size_t o_err = align_up(o_nx + 4 * n, 256);
+  const size_t o_err = align_up(o_k + 4 * n, 256);
   const size_t total = o_err + 256;
   char *dev = (char *)g_batch.reserve_dev(total);
   char *host = (char *)g_batch.reserve_host(total);
   TileDesc *hd = (TileDesc *)(host + o_desc);
   int *hp = (int *)(host + o_pref);
   uint32_t *hk = (uint32_t *)(host + o_k);
-  uint32_t *hnx = (uint32_t *)(host + o_nx);
   uint32_t *herr = (uint32_t *)(host + o_err);
   TileDesc *dd = (TileDesc *)(dev + o_desc);
   int *dp = (int *)(dev + o_pref);
   uint32_t *dk = (uint32_t *)(dev + o_k);
-  uint32_t *dnx = (uint32_t *)(dev + o_nx);
   uint32_t *derr = (uint32_t *)(dev + o_err);
 
   for (int i = 0; i < n; ++i) {
     hd[i] = tiles[i]->d;
     hd[i].nkrit = dk + i;
-    hd[i].nexit = dnx + i;
     hd[i].kcap = 0;
     hk[i] = 0;
-    hnx[i] = 0;
   }
   *herr = 0;
   int nb[KB_COUNT];
@@ -131,7 +128,7 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   launch_scan(dd, dp + KB_SCAN * (n + 1), n, nb[KB_SCAN], s);
   tm.end(ST_SCAN, s);
   tm.begin(ST_TRACE, s);
-  launch_resolve_exits(dd, n, 2048, derr, s);
+  launch_resolve_exits(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], derr, s);
   tm.end(ST_TRACE, s);
   MSCX_HIP(hipGetLastError());
   MSCX_HIP(hipMemcpyAsync(hk, dk, o_err + 4 - o_k, hipMemcpyDeviceToHost, s));
@@ -142,7 +139,6 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   for (int i = 0; i < n; ++i) {
     ensure_kcap(tiles[i], hk[i]);
     tiles[i]->K = hk[i];
-    tiles[i]->nexit = hnx[i];
     hd[i].crit = tiles[i]->d.crit;
     hd[i].crit_fn = tiles[i]->d.crit_fn;
     hd[i].crit_conn = tiles[i]->d.crit_conn;

@@ -19,7 +19,6 @@ struct mscx_tile {
   uint32_t K = 0;
   bool has_input = false;
   bool worked = false;
-  uint32_t nexit = 0;  // owner entries resolved across block tiles (last work)
 };
 
 namespace mscx {

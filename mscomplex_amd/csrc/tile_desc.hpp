@@ -41,7 +41,9 @@ struct TileDesc {
   // of a cell whose owner it takes (a path that left its block tile)
   uint32_t *own;
   int own_pitch;
-  // owner entries that hold a marker after the block-local trace
+  // owner entries that hold a marker after the block-local trace: block b of
+  // the gradient launch lists its entries in exits[b * 4*GRAD_TX*GRAD_TY ...]
+  // and their count in nexit[b]
   uint32_t *exits;
   uint32_t *nexit;
   uint32_t kcap;  // capacity of the crit arrays
