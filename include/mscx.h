@@ -169,6 +169,22 @@ int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out);
 int mscx_grid_times(const mscx_grid *g, double t5[5]);
 int mscx_grid_destroy(mscx_grid *g);
 
+/* Many independent grids at once (no reference counterpart: the reference
+ * handles one grid per GridDataManager).  Each stage's GPU work for every
+ * leaf tile of every grid is one launch; the host phases of different grids
+ * run concurrently.  Per-grid arrays of length n: data (host, or device with
+ * MSCX_RUN_DEVICE_INPUT), sizes, 2-D tiling levels and thresholds.  A grid
+ * that fails (bad input, arc explosion) gets its own status; the others
+ * complete.  mscx_batch_grid returns a borrowed mscx_grid for the grid
+ * accessors above. */
+typedef struct mscx_batch mscx_batch;
+int mscx_batch_run(int32_t n, const float *const *data, const int32_t *size_x, const int32_t *size_y,
+                   const int32_t *lx, const int32_t *ly, const double *threshold, uint32_t flags, mscx_batch **out);
+int mscx_batch_size(const mscx_batch *b, int32_t *n);
+int mscx_batch_grid(const mscx_batch *b, int32_t i, const mscx_grid **out, int32_t *status);
+const char *mscx_batch_error(const mscx_batch *b, int32_t i);
+int mscx_batch_destroy(mscx_batch *b);
+
 /* ---- synthetic inputs (seeded, host) -------------------------------------- */
 
 /* Sum of n Gaussians, centres U[0,sx)xU[0,sy), sigma U[8,64], amplitude

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib
 from ._lib import MscxError, check, lib, ptr
 
-__all__ = ["DeviceBuffer", "profile", "profile_read", "GridDataset", "GridMSComplex", "GridRun", "work_tiles", "gen_gaussians", "gen_value_noise",
+__all__ = ["Batch", "DeviceBuffer", "profile", "profile_read", "GridDataset", "GridMSComplex", "GridRun", "work_tiles", "gen_gaussians", "gen_value_noise",
            "gen_white", "MscxError", "unpack", "pack", "grid_rect"]
 
 
@@ -269,6 +269,80 @@ class GridMSComplex:
         buf = np.empty(n.value, np.uint32)
         check(lib().mscx_complex_dump(self._h, ptr(buf), buf.size, ctypes.byref(n)))
         return buf
+
+
+class _GridView:
+    """Borrowed view of one grid of a batch (same accessors as GridRun)."""
+
+    def __init__(self, handle, owner):
+        self._h = handle
+        self._owner = owner
+
+    def num_pieces(self):
+        n = ctypes.c_int32()
+        check(lib().mscx_grid_num_pieces(self._h, ctypes.byref(n)))
+        return n.value
+
+    def piece(self, i):
+        h = ctypes.c_void_p()
+        check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
+        return GridMSComplex(h, owned=False)
+
+    def times(self):
+        t = np.zeros(5, np.float64)
+        check(lib().mscx_grid_times(self._h, ptr(t)))
+        return t
+
+
+class Batch:
+    """Many grids through the pipeline together (mscx_batch_run).  grids:
+    list of 2-D float32 arrays (or, with device=True, (ptr, sx, sy) tuples)."""
+
+    def __init__(self, grids, lx, ly, thresholds, discs=True, device=False):
+        n = len(grids)
+        lx = np.broadcast_to(np.asarray(lx, np.int32), (n,)).copy()
+        ly = np.broadcast_to(np.asarray(ly, np.int32), (n,)).copy()
+        th = np.broadcast_to(np.asarray(thresholds, np.float64), (n,)).copy()
+        sx = np.empty(n, np.int32)
+        sy = np.empty(n, np.int32)
+        ptrs = (ctypes.c_void_p * max(n, 1))()
+        keep = []
+        for i, g in enumerate(grids):
+            if device:
+                ptrs[i], sx[i], sy[i] = g[0], g[1], g[2]
+            else:
+                a = np.ascontiguousarray(g, np.float32)
+                keep.append(a)
+                sy[i], sx[i] = a.shape
+                ptrs[i] = a.ctypes.data
+        flags = (0 if discs else 2) | (1 if device else 0)
+        h = ctypes.c_void_p()
+        check(lib().mscx_batch_run(n, ptrs, ptr(sx), ptr(sy), ptr(lx), ptr(ly), ptr(th), flags, ctypes.byref(h)))
+        self._h = h
+        self.n = n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                lib().mscx_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def status(self, i):
+        g = ctypes.c_void_p()
+        st = ctypes.c_int32()
+        check(lib().mscx_batch_grid(self._h, i, ctypes.byref(g), ctypes.byref(st)))
+        return st.value, lib().mscx_batch_error(self._h, i).decode()
+
+    def grid(self, i):
+        g = ctypes.c_void_p()
+        st = ctypes.c_int32()
+        check(lib().mscx_batch_grid(self._h, i, ctypes.byref(g), ctypes.byref(st)))
+        if st.value != 0:
+            raise MscxError(st.value, lib().mscx_batch_error(self._h, i).decode())
+        return _GridView(g, self)
 
 
 class GridRun:

@@ -56,6 +56,11 @@ _SIGS = {
     "mscx_grid_piece": (c_int, [P, c_int32, ctypes.POINTER(P)]),
     "mscx_grid_times": (c_int, [P, P]),
     "mscx_grid_destroy": (c_int, [P]),
+    "mscx_batch_run": (c_int, [c_int32, P, P, P, P, P, P, c_uint32, ctypes.POINTER(P)]),
+    "mscx_batch_size": (c_int, [P, ctypes.POINTER(c_int32)]),
+    "mscx_batch_grid": (c_int, [P, c_int32, ctypes.POINTER(P), ctypes.POINTER(c_int32)]),
+    "mscx_batch_error": (ctypes.c_char_p, [P, c_int32]),
+    "mscx_batch_destroy": (c_int, [P]),
     "mscx_gen_gaussians": (c_int, [P, c_int32, c_int32, c_int32, c_double, c_uint64]),
     "mscx_gen_value_noise": (c_int, [P, c_int32, c_int32, c_int32, c_double, c_double, c_double, c_uint64]),
     "mscx_gen_white": (c_int, [P, c_int32, c_int32, c_int32, c_uint64]),

@@ -1,7 +1,10 @@
-// mscx_grid_run: the GridDataManager compute order (grid_datamanager.cpp:
-// 596-662) on one GPU.  All leaf tiles go through the HIP stages in one
-// batched launch per stage; the complex algebra runs on host threads, one
-// task per independent merge of a tree level.
+// mscx_grid_run / mscx_batch_run: the GridDataManager compute order
+// (grid_datamanager.cpp:596-662) for one grid or for many independent grids
+// at once.  Every leaf tile of every grid goes through each HIP stage in one
+// launch; the complex algebra runs on host threads (merges of one tree level
+// in parallel for a single grid, whole grids in parallel for a batch); the
+// manifold label pass of every leaf is again one launch.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -19,19 +22,27 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static int host_threads() {
+  return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+}
+
+// Dynamic parallel loop over [0, n) on up to `nt` threads; the first
+// exception is rethrown after all threads have joined.
 template <class F>
-static void parallel_for(int n, F f) {
-  int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+static void parallel_for(int n, F f, int nt = 0) {
+  if (nt <= 0) nt = host_threads();
+  nt = std::min(nt, n);
   if (n <= 1 || nt <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
+  std::atomic<int> next(0);
   std::vector<std::thread> th;
-  std::vector<std::exception_ptr> errs(std::min(n, nt));
-  for (int t = 0; t < (int)errs.size(); ++t)
+  std::vector<std::exception_ptr> errs(nt);
+  for (int t = 0; t < nt; ++t)
     th.emplace_back([&, t] {
       try {
-        for (int i = t; i < n; i += (int)errs.size()) f(i);
+        for (int i = next++; i < n; i = next++) f(i);
       } catch (...) {
         errs[t] = std::current_exception();
       }
@@ -80,6 +91,10 @@ static void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d,
 using namespace mscx;
 
 struct mscx_grid {
+  int32_t sx = 0, sy = 0;
+  double threshold = 0;
+  uint32_t flags = 0;
+  std::vector<LeafGeom> leaves;
   std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
   std::vector<mscx_tile *> tiles;
   void *dev_input = nullptr;
@@ -91,118 +106,256 @@ struct mscx_grid {
   }
 };
 
+struct mscx_batch {
+  std::vector<mscx_grid *> grids;
+  std::vector<int> status;
+  std::vector<std::string> error;
+  ~mscx_batch() {
+    for (mscx_grid *g : grids) delete g;
+  }
+};
+
+namespace mscx {
+
+// Validates arguments, uploads host input, creates and binds the leaf tiles.
+static void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx,
+                         int32_t ly, double threshold, uint32_t flags) {
+  MSCX_REQUIRE(data, MSCX_EINVAL, "null input");
+  MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
+  MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
+  MSCX_REQUIRE(threshold >= 0.0 && !std::isnan(threshold), MSCX_EINVAL, "threshold must be >= 0");
+  G->sx = sx;
+  G->sy = sy;
+  G->threshold = threshold;
+  G->flags = flags;
+  const float *dev = data;
+  if (!(flags & MSCX_RUN_DEVICE_INPUT)) {
+    size_t n = (size_t)sx * sy;
+    for (size_t i = 0; i < n; ++i) MSCX_REQUIRE(!std::isnan(data[i]), MSCX_EINVAL, "input contains NaN");
+    G->dev_input = device_pool().alloc(n * 4);
+    MSCX_HIP(hipMemcpy(G->dev_input, data, n * 4, hipMemcpyHostToDevice));
+    dev = (const float *)G->dev_input;
+  }
+  bool two_d = lx > 0 || ly > 0;
+  int L = two_d ? lx + ly : levels;
+  Rect whole{0, 0, 2 * (sx - 1), 2 * (sy - 1)};
+  create_pieces(whole, whole, L, lx, ly, two_d, G->leaves);
+  for (const LeafGeom &lg : G->leaves) {
+    int32_t r[4] = {lg.rect.x0, lg.rect.y0, lg.rect.x1, lg.rect.y1};
+    int32_t e[4] = {lg.ext.x0, lg.ext.y0, lg.ext.x1, lg.ext.y1};
+    mscx_tile *t = nullptr;
+    int rc = mscx_tile_create(r, e, &t);
+    if (rc) throw Error(rc, mscx_last_error());
+    G->tiles.push_back(t);
+    rc = mscx_tile_bind_device(t, dev + (size_t)(e[1] / 2) * sx + e[0] / 2, sx);
+    if (rc) throw Error(rc, mscx_last_error());
+  }
+}
+
+// Merge up, simplify, merge down for one grid whose leaf complexes are built.
+static void grid_tree(mscx_grid *G, bool inner_parallel) {
+  const int nleaf = (int)G->leaves.size();
+  const int nt = inner_parallel ? 0 : 1;
+  double t1 = now_s();
+  G->pieces.resize(2 * nleaf - 1, nullptr);
+  for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
+  // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
+  for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
+    parallel_for(
+        width / 2,
+        [&](int k) {
+          int j = lo + 2 * k;
+          merge_up(G->pieces[j]->c, G->pieces[j + 1]->c, G->pieces[nleaf + j / 2]->c);
+        },
+        nt);
+  double t2 = now_s();
+  G->t[1] += t2 - t1;
+  if (G->threshold > 0.0) {
+    std::vector<std::pair<uint32_t, uint32_t>> canc;
+    simplify(G->pieces.back()->c, canc, G->threshold);
+    un_simplify(G->pieces.back()->c, canc);
+  }
+  double t3 = now_s();
+  G->t[2] += t3 - t2;
+  // mergePiecesDown + finalMergeDownPiecesInRange (:308-395): parents before
+  // children, one tree level at a time
+  if (nleaf > 1) {
+    std::vector<int> level{2 * nleaf - 2};
+    while (!level.empty()) {
+      parallel_for(
+          (int)level.size(),
+          [&](int k) {
+            int p = level[k];
+            int c = 2 * (p - nleaf);
+            merge_down(G->pieces[p]->c, G->pieces[c]->c, G->pieces[c + 1]->c);
+          },
+          nt);
+      std::vector<int> next;
+      for (int p : level) {
+        int c = 2 * (p - nleaf);
+        if (c >= nleaf) next.push_back(c);
+        if (c + 1 >= nleaf) next.push_back(c + 1);
+      }
+      level.swap(next);
+    }
+  }
+  G->t[3] += now_s() - t3;
+}
+
+static Scratch g_crit_host;
+
+// The whole compute order for a set of prepared grids.  A failure of one
+// grid in the host phases is recorded in status/error (when given) so the
+// other grids complete; device failures fail the call.
+static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status, std::vector<std::string> *error) {
+  const int ng = (int)grids.size();
+  if (!ng) return;
+  const double t0 = now_s();
+  std::vector<mscx_tile *> tiles;
+  for (mscx_grid *G : grids) tiles.insert(tiles.end(), G->tiles.begin(), G->tiles.end());
+  run_tiles(tiles.data(), (int)tiles.size());
+
+  // every tile's critical data lives in one block: one copy to host
+  const size_t bytes = tiles[0]->crit_block_bytes;
+  char *host = (char *)g_crit_host.reserve_host(bytes);
+  MSCX_HIP(hipMemcpyAsync(host, tiles[0]->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
+  MSCX_HIP(hipStreamSynchronize(runtime().stream));
+
+  std::vector<std::pair<int, int>> leafs;  // (grid, leaf)
+  for (int gi = 0; gi < ng; ++gi) {
+    mscx_grid *G = grids[gi];
+    G->pieces.assign(G->leaves.size(), nullptr);
+    for (size_t i = 0; i < G->leaves.size(); ++i) {
+      G->pieces[i] = new mscx_complex;
+      leafs.emplace_back(gi, (int)i);
+    }
+  }
+  parallel_for((int)leafs.size(), [&](int k) {
+    mscx_grid *G = grids[leafs[k].first];
+    const int i = leafs[k].second;
+    const mscx_tile *t = G->tiles[i];
+    const char *b = host + t->crit_off;
+    const CritLayout L(t->K);
+    const uint32_t K = t->K;
+    std::vector<uint32_t> crit((const uint32_t *)b, (const uint32_t *)b + K);
+    std::vector<float> fn((const float *)(b + L.fn), (const float *)(b + L.fn) + K);
+    std::vector<uint32_t> conn((const uint32_t *)(b + L.conn), (const uint32_t *)(b + L.conn) + 4 * (size_t)K);
+    std::vector<uint32_t> pr((const uint32_t *)(b + L.pair), (const uint32_t *)(b + L.pair) + K);
+    complex_build(G->pieces[i]->c, G->leaves[i].rect, G->leaves[i].ext, crit, fn, conn, pr);
+  });
+  const double t1 = now_s();
+  for (mscx_grid *G : grids) G->t[0] += t1 - t0;
+
+  std::vector<char> ok(ng, 1);
+  auto tree = [&](int gi, bool inner) {
+    try {
+      grid_tree(grids[gi], inner);
+    } catch (const std::exception &e) {
+      if (!status) throw;
+      ok[gi] = 0;
+      int code = MSCX_ELOGIC;
+      if (auto *er = dynamic_cast<const Error *>(&e)) code = er->code;
+      (*status)[gi] = code;
+      (*error)[gi] = e.what();
+    }
+  };
+  if (ng == 1)
+    tree(0, true);
+  else
+    parallel_for(ng, [&](int gi) { tree(gi, false); });
+
+  // collectSubdomainManifolds / postMergeFillDiscs (:397-490) for every
+  // leaf of every grid that asked for discs and has not failed
+  const double t4 = now_s();
+  std::vector<mscx_tile *> dt;
+  std::vector<Complex *> dc;
+  for (int gi = 0; gi < ng; ++gi) {
+    mscx_grid *G = grids[gi];
+    if (!ok[gi] || (G->flags & MSCX_RUN_NO_DISCS)) continue;
+    for (size_t i = 0; i < G->leaves.size(); ++i) {
+      dt.push_back(G->tiles[i]);
+      dc.push_back(&G->pieces[i]->c);
+    }
+  }
+  fill_discs_device(dt.data(), dc.data(), (int)dt.size());
+  const double t5 = now_s();
+  for (mscx_grid *G : grids) G->t[4] += t5 - t4;
+}
+
+}  // namespace mscx
+
 extern "C" int mscx_grid_run(const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
                              double threshold, uint32_t flags, mscx_grid **out) {
   return guarded([&] {
-    MSCX_REQUIRE(data && out, MSCX_EINVAL, "null argument");
-    MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
-    MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
-    MSCX_REQUIRE(threshold >= 0.0 && !std::isnan(threshold), MSCX_EINVAL, "threshold must be >= 0");
+    MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
     ensure_init();
     std::unique_ptr<mscx_grid> G(new mscx_grid);
     const double t0 = now_s();
-
-    const float *dev = data;
-    if (!(flags & MSCX_RUN_DEVICE_INPUT)) {
-      size_t n = (size_t)sx * sy;
-      for (size_t i = 0; i < n; ++i) MSCX_REQUIRE(!std::isnan(data[i]), MSCX_EINVAL, "input contains NaN");
-      int rc = mscx_dev_alloc(n * 4, &G->dev_input);
-      if (rc) throw Error(rc, mscx_last_error());
-      rc = mscx_dev_upload(G->dev_input, data, n * 4);
-      if (rc) throw Error(rc, mscx_last_error());
-      dev = (const float *)G->dev_input;
-    }
-
-    bool two_d = lx > 0 || ly > 0;
-    int L = two_d ? lx + ly : levels;
-    std::vector<LeafGeom> leaves;
-    Rect whole{0, 0, 2 * (sx - 1), 2 * (sy - 1)};
-    create_pieces(whole, whole, L, lx, ly, two_d, leaves);
-    const int nleaf = (int)leaves.size();
-    for (const LeafGeom &lg : leaves) {
-      int32_t r[4] = {lg.rect.x0, lg.rect.y0, lg.rect.x1, lg.rect.y1};
-      int32_t e[4] = {lg.ext.x0, lg.ext.y0, lg.ext.x1, lg.ext.y1};
-      mscx_tile *t = nullptr;
-      int rc = mscx_tile_create(r, e, &t);
-      if (rc) throw Error(rc, mscx_last_error());
-      G->tiles.push_back(t);
-      const float *win = dev + (size_t)(e[1] / 2) * sx + e[0] / 2;
-      rc = mscx_tile_bind_device(t, win, sx);
-      if (rc) throw Error(rc, mscx_last_error());
-    }
-
-    // computeSubdomainMsgraphs: GPU stages for every leaf, then the complexes
-    run_tiles(G->tiles.data(), nleaf);
-    std::vector<std::vector<uint32_t>> crit(nleaf), conn(nleaf), pr(nleaf);
-    std::vector<std::vector<float>> fn(nleaf);
-    for (int i = 0; i < nleaf; ++i) tile_critical(G->tiles[i], crit[i], fn[i], conn[i], pr[i]);
-    G->pieces.assign(nleaf, nullptr);
-    for (int i = 0; i < nleaf; ++i) G->pieces[i] = new mscx_complex;
-    parallel_for(nleaf, [&](int i) {
-      complex_build(G->pieces[i]->c, leaves[i].rect, leaves[i].ext, crit[i], fn[i], conn[i], pr[i]);
-      std::vector<uint32_t>().swap(crit[i]);
-      std::vector<uint32_t>().swap(conn[i]);
-      std::vector<uint32_t>().swap(pr[i]);
-      std::vector<float>().swap(fn[i]);
-    });
-    const double t1 = now_s();
-    G->t[0] = t1 - t0;
-
-    // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is
-    // nleaf + j/2; one level at a time, merges of a level in parallel.
-    G->pieces.resize(2 * nleaf - 1, nullptr);
-    for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
-    for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2) {
-      parallel_for(width / 2, [&](int k) {
-        int j = lo + 2 * k;
-        merge_up(G->pieces[j]->c, G->pieces[j + 1]->c, G->pieces[nleaf + j / 2]->c);
-      });
-    }
-    const double t2 = now_s();
-    G->t[1] = t2 - t1;
-
-    if (threshold > 0.0) {
-      std::vector<std::pair<uint32_t, uint32_t>> canc;
-      simplify(G->pieces.back()->c, canc, threshold);
-      un_simplify(G->pieces.back()->c, canc);
-    }
-    const double t3 = now_s();
-    G->t[2] = t3 - t2;
-
-    // mergePiecesDown + finalMergeDownPiecesInRange (:308-395): parents
-    // before children, one tree level at a time.
-    if (nleaf > 1) {
-      int total = 2 * nleaf - 1;
-      // internal node p has children 2(p - nleaf), 2(p - nleaf) + 1; walk
-      // levels from the root down
-      std::vector<int> level{total - 1};
-      while (!level.empty()) {
-        parallel_for((int)level.size(), [&](int k) {
-          int p = level[k];
-          int c = 2 * (p - nleaf);
-          merge_down(G->pieces[p]->c, G->pieces[c]->c, G->pieces[c + 1]->c);
-        });
-        std::vector<int> next;
-        for (int p : level) {
-          int c = 2 * (p - nleaf);
-          if (c >= nleaf) next.push_back(c);
-          if (c + 1 >= nleaf) next.push_back(c + 1);
-        }
-        level.swap(next);
-      }
-    }
-    const double t4 = now_s();
-    G->t[3] = t4 - t3;
-
-    // collectSubdomainManifolds / postMergeFillDiscs (:397-490)
-    if (!(flags & MSCX_RUN_NO_DISCS)) {
-      std::vector<Complex *> cs(nleaf);
-      for (int i = 0; i < nleaf; ++i) cs[i] = &G->pieces[i]->c;
-      fill_discs_device(G->tiles.data(), cs.data(), nleaf);
-    }
-    G->t[4] = now_s() - t4;
+    prepare_grid(G.get(), data, sx, sy, levels, lx, ly, threshold, flags);
+    G->t[0] += now_s() - t0;
+    std::vector<mscx_grid *> gs{G.get()};
+    run_grids(gs, nullptr, nullptr);
     *out = G.release();
   });
+}
+
+extern "C" int mscx_batch_run(int32_t n, const float *const *data, const int32_t *sx, const int32_t *sy,
+                              const int32_t *lx, const int32_t *ly, const double *threshold, uint32_t flags,
+                              mscx_batch **out) {
+  return guarded([&] {
+    MSCX_REQUIRE(out && n >= 0 && (n == 0 || (data && sx && sy && lx && ly && threshold)), MSCX_EINVAL,
+                 "null argument");
+    ensure_init();
+    std::unique_ptr<mscx_batch> B(new mscx_batch);
+    B->status.assign(n, MSCX_OK);
+    B->error.assign(n, std::string());
+    std::vector<mscx_grid *> live;
+    for (int i = 0; i < n; ++i) {
+      mscx_grid *G = new mscx_grid;
+      B->grids.push_back(G);
+      try {
+        prepare_grid(G, data[i], sx[i], sy[i], 0, lx[i], ly[i], threshold[i], flags);
+        live.push_back(G);
+      } catch (const Error &e) {
+        B->status[i] = e.code;
+        B->error[i] = e.what();
+      }
+    }
+    std::vector<int> st(live.size(), MSCX_OK);
+    std::vector<std::string> er(live.size());
+    run_grids(live, &st, &er);
+    for (size_t k = 0, i = 0; i < (size_t)n; ++i) {
+      if (B->status[i] != MSCX_OK) continue;
+      B->status[i] = st[k];
+      B->error[i] = er[k];
+      ++k;
+    }
+    *out = B.release();
+  });
+}
+
+extern "C" int mscx_batch_size(const mscx_batch *b, int32_t *n) {
+  return guarded([&] {
+    MSCX_REQUIRE(b && n, MSCX_EINVAL, "null argument");
+    *n = (int32_t)b->grids.size();
+  });
+}
+
+extern "C" int mscx_batch_grid(const mscx_batch *b, int32_t i, const mscx_grid **out, int32_t *status) {
+  return guarded([&] {
+    MSCX_REQUIRE(b && out && i >= 0 && i < (int32_t)b->grids.size(), MSCX_EINVAL, "bad grid index");
+    *out = b->grids[i];
+    if (status) *status = b->status[i];
+  });
+}
+
+extern "C" const char *mscx_batch_error(const mscx_batch *b, int32_t i) {
+  if (!b || i < 0 || i >= (int32_t)b->grids.size()) return "";
+  return b->error[i].c_str();
+}
+
+extern "C" int mscx_batch_destroy(mscx_batch *b) {
+  return guarded([&] { delete b; });
 }
 
 extern "C" int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n) {

@@ -49,6 +49,69 @@ void *Scratch::reserve_host(size_t n) {
   return host;
 }
 
+size_t DevicePool::size_class(size_t n) {
+  if (n <= 4096) return 4096;
+  // classes at 1/8 steps of each power of two
+  size_t p = 1;
+  while (p * 2 <= n) p *= 2;
+  size_t step = p / 8;
+  return align_up(n, step);
+}
+
+void *DevicePool::alloc(size_t bytes) {
+  size_t cls = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.find(cls);
+    if (it != free_.end()) {
+      void *p = it->second;
+      free_.erase(it);
+      cached_ -= cls;
+      live_[p] = cls;
+      in_use_ += cls;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, cls);
+  if (e != hipSuccess) {
+    trim();  // give cached blocks back and retry once
+    e = hipMalloc(&p, cls);
+  }
+  MSCX_REQUIRE(e == hipSuccess, MSCX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(mu_);
+  live_[p] = cls;
+  in_use_ += cls;
+  return p;
+}
+
+void DevicePool::free(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  size_t cls = it->second;
+  live_.erase(it);
+  in_use_ -= cls;
+  free_.emplace(cls, p);
+  cached_ += cls;
+}
+
+void DevicePool::trim() {
+  std::multimap<size_t, void *> f;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    f.swap(free_);
+    cached_ = 0;
+  }
+  for (auto &kv : f) (void)hipFree(kv.second);
+}
+
+DevicePool &device_pool() {
+  static DevicePool *p = new DevicePool;  // lives until process exit
+  return *p;
+}
+
 StageTimer &stage_timer() {
   static StageTimer t;
   return t;
@@ -136,8 +199,7 @@ int mscx_dev_alloc(size_t bytes, void **out) {
     MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
     mscx::ensure_init();
     *out = nullptr;
-    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
-    MSCX_REQUIRE(e == hipSuccess, MSCX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    *out = mscx::device_pool().alloc(bytes ? bytes : 1);
   });
 }
 
@@ -145,7 +207,7 @@ int mscx_dev_free(void *p) {
   return mscx::guarded([&] {
     if (!p) return;
     MSCX_HIP(hipStreamSynchronize(mscx::runtime().stream));
-    MSCX_HIP(hipFree(p));
+    mscx::device_pool().free(p);
   });
 }
 

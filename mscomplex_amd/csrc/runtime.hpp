@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -75,6 +76,27 @@ struct Scratch {
 };
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Caching device allocator: freed blocks are kept per size class and reused,
+// so steady-state runs make no hipMalloc/hipFree calls.  in_use() is the
+// HBM the library currently holds for live objects (the serving loop admits
+// work against it).
+class DevicePool {
+ public:
+  void *alloc(size_t bytes);
+  void free(void *p);
+  size_t in_use() const { return in_use_; }
+  size_t cached() const { return cached_; }
+  void trim();  // returns cached blocks to the runtime
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void *> free_;  // class size -> block
+  std::map<void *, size_t> live_;       // block -> class size
+  size_t in_use_ = 0, cached_ = 0;
+  static size_t size_class(size_t n);
+};
+DevicePool &device_pool();
 
 // Optional per-stage device timing with HIP events recorded on the work
 // stream around each stage's launches (enabled by mscx_profile).

@@ -55,7 +55,7 @@ static void alloc_fixed(mscx_tile *t) {
   size_t o_exit = take((size_t)nblk * 4 * GRAD_TX * GRAD_TY * 4);
   size_t o_nexit = take((size_t)nblk * 4);
   t->fixed_bytes = off;
-  MSCX_HIP(hipMalloc(&t->fixed, off));
+  t->fixed = device_pool().alloc(off);
   char *base = (char *)t->fixed;
   d.codes = (uint8_t *)(base + o_codes);
   d.seg_counts = (uint32_t *)(base + o_cnt);
@@ -66,21 +66,12 @@ static void alloc_fixed(mscx_tile *t) {
   d.nexit = (uint32_t *)(base + o_nexit);
 }
 
-static void ensure_kcap(mscx_tile *t, size_t K) {
-  if (K <= t->kcap && t->crit_mem) return;
-  if (t->crit_mem) MSCX_HIP(hipFree(t->crit_mem));
-  t->crit_mem = nullptr;
-  size_t cap = K < 64 ? 64 : K + K / 8;
-  // crit (4) + fn (4) + conn (16) + pair (4)
-  size_t o_fn = align_up(cap * 4, 256), o_conn = o_fn + align_up(cap * 4, 256),
-         o_pair = o_conn + align_up(cap * 16, 256), total = o_pair + align_up(cap * 4, 256);
-  MSCX_HIP(hipMalloc(&t->crit_mem, total));
-  char *b = (char *)t->crit_mem;
-  t->d.crit = (uint32_t *)b;
-  t->d.crit_fn = (float *)(b + o_fn);
-  t->d.crit_conn = (uint32_t *)(b + o_conn);
-  t->d.crit_pair = (uint32_t *)(b + o_pair);
-  t->kcap = cap;
+CritLayout::CritLayout(size_t K) {
+  size_t cap = K ? K : 1;
+  fn = align_up(cap * 4, 256);
+  conn = fn + align_up(cap * 4, 256);
+  pair = conn + align_up(cap * 16, 256);
+  total = pair + align_up(cap * 4, 256);
 }
 
 void run_tiles(mscx_tile *const *tiles, int n) {
@@ -136,8 +127,25 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   tm.collect();
   MSCX_REQUIRE(*herr == 0, MSCX_ELOGIC, "owner trace did not resolve");
 
+  // one block for the critical data of every tile of this launch
+  size_t block_bytes = 0;
+  std::vector<size_t> coff(n);
   for (int i = 0; i < n; ++i) {
-    ensure_kcap(tiles[i], hk[i]);
+    coff[i] = block_bytes;
+    block_bytes += CritLayout(hk[i]).total;
+  }
+  void *blk = device_pool().alloc(block_bytes);
+  std::shared_ptr<void> block(blk, [](void *p) { device_pool().free(p); });
+  for (int i = 0; i < n; ++i) {
+    CritLayout L(hk[i]);
+    char *b = (char *)blk + coff[i];
+    tiles[i]->crit_block = block;
+    tiles[i]->crit_off = coff[i];
+    tiles[i]->crit_block_bytes = block_bytes;
+    tiles[i]->d.crit = (uint32_t *)b;
+    tiles[i]->d.crit_fn = (float *)(b + L.fn);
+    tiles[i]->d.crit_conn = (uint32_t *)(b + L.conn);
+    tiles[i]->d.crit_pair = (uint32_t *)(b + L.pair);
     tiles[i]->K = hk[i];
     hd[i].crit = tiles[i]->d.crit;
     hd[i].crit_fn = tiles[i]->d.crit_fn;
@@ -224,9 +232,9 @@ int mscx_tile_destroy(mscx_tile *t) {
   return guarded([&] {
     if (!t) return;
     MSCX_HIP(hipStreamSynchronize(runtime().stream));
-    if (t->fixed) MSCX_HIP(hipFree(t->fixed));
-    if (t->verts_owned) MSCX_HIP(hipFree(t->verts_owned));
-    if (t->crit_mem) MSCX_HIP(hipFree(t->crit_mem));
+    device_pool().free(t->fixed);
+    device_pool().free(t->verts_owned);
+    t->crit_block.reset();
     delete t;
   });
 }
@@ -239,9 +247,9 @@ int mscx_tile_upload(mscx_tile *t, const float *data, size_t n) {
     MSCX_REQUIRE(n == need, MSCX_EINVAL, "input size does not match the ext window");
     for (size_t i = 0; i < n; ++i) MSCX_REQUIRE(!std::isnan(data[i]), MSCX_EINVAL, "input contains NaN");
     if (t->verts_cap < need) {
-      if (t->verts_owned) MSCX_HIP(hipFree(t->verts_owned));
+      device_pool().free(t->verts_owned);
       t->verts_owned = nullptr;
-      MSCX_HIP(hipMalloc(&t->verts_owned, need * 4));
+      t->verts_owned = (float *)device_pool().alloc(need * 4);
       t->verts_cap = need;
     }
     MSCX_HIP(hipMemcpyAsync(t->verts_owned, data, need * 4, hipMemcpyHostToDevice, runtime().stream));

@@ -1,6 +1,7 @@
 // mscx_tile: one GridDataset (rect + ext window) resident in HBM.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "kernels.hpp"
@@ -14,8 +15,11 @@ struct mscx_tile {
   size_t fixed_bytes = 0;
   float *verts_owned = nullptr;
   size_t verts_cap = 0;
-  void *crit_mem = nullptr;  // crit, fn, conn (x4), pair
-  size_t kcap = 0;
+  // crit, fn, conn (x4), pair of the last work: one block shared by all
+  // tiles of that launch (one device->host copy serves them all)
+  std::shared_ptr<void> crit_block;
+  size_t crit_off = 0;  // byte offset of this tile's arrays in the block
+  size_t crit_block_bytes = 0;
   uint32_t K = 0;
   bool has_input = false;
   bool worked = false;
@@ -26,6 +30,12 @@ namespace mscx {
 // Runs gradient, collation, trace, owners and incidence for n tiles with one
 // launch per stage, then synchronises.  Sets K / worked on every tile.
 void run_tiles(mscx_tile *const *tiles, int n);
+
+// Byte offsets of the crit / fn / conn / pair arrays for K critical cells.
+struct CritLayout {
+  size_t fn, conn, pair, total;
+  explicit CritLayout(size_t K);
+};
 
 // Host copies of a tile's critical data after work (K entries each).
 void tile_critical(const mscx_tile *t, std::vector<uint32_t> &crit, std::vector<float> &fn,
