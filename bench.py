@@ -124,19 +124,13 @@ def main():
     geo = [((e[2] - e[0]) // 2 + 1, (e[3] - e[1]) // 2 + 1) for _, e in pieces]
     nm = sum(vx * vy for vx, vy in geo)
     cells = sum((2 * vx - 1) * (2 * vy - 1) for vx, vy in geo)
-    grad_bytes = 4 * nm + cells
-    trace_bytes = 5 * cells
-    g_ms, g_n = prof["gradient"]
-    grad_ms = g_ms / max(g_n, 1)
-    tr_ms = (prof["trace"][0] + prof["owner"][0]) / args.steps
+    # the fused gradient + block-local trace kernel moves the f32 input once
+    # (4 B per vertex), writes one code byte and one 4-byte owner per cell
+    gt_bytes = 4 * nm + cells + 4 * cells
+    gt_ms, gt_n = prof["gradient"]
+    gt_ms = gt_ms / max(gt_n, 1)
     stage_ms = {k: v[0] / args.steps for k, v in prof.items()}
-    dominant = max(("gradient", "trace"), key=lambda k: stage_ms["gradient"] if k == "gradient" else tr_ms)
-    if dominant == "gradient":
-        ach = grad_bytes / (grad_ms * 1e-3) / 1e9
-        traffic = grad_bytes
-    else:
-        ach = trace_bytes / (tr_ms * 1e-3) / 1e9
-        traffic = trace_bytes
+    ach = gt_bytes / (gt_ms * 1e-3) / 1e9
     line = {
         "metric": "Mcells/s end-to-end MS complex @4096^2 f32",
         "value": round(mcells, 2),
@@ -154,9 +148,9 @@ def main():
                                "4x4 tile merge, persistence simplification at 1% of range, manifold discs",
                    "grid": [SIDE, SIDE], "tiles": [4, 4], "threshold": THRESHOLD,
                    "leaf_critical_points": K, "root_critical_points": root_cps},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_grad_trace", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": traffic},
+                     "algorithmic_bytes": gt_bytes, "avg_launch_ms": round(gt_ms, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "phase_ms": dict(zip(("compute", "merge_up", "simplify", "merge_down", "discs"),
                              [round(1000 * v / args.steps, 3) for v in host_t])),

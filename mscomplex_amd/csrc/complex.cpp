@@ -201,6 +201,11 @@ void complex_from_tile(const mscx_tile *t, Complex &out) {
 }
 
 static constexpr uint64_t kMaxArcProduct = 1ull << 20;
+// Arc-multiset entries one simplify or un_simplify call may add (4 bytes
+// each: about 1 GiB).  Legitimate heavy cases stay far below it: a noisy
+// 1024^2 Gaussian sum simplified at 1% grows to 4.4e7 entries (165K
+// distinct arcs with large multiplicities) after un_simplify.
+static constexpr int64_t kArcEntryBudget = (int64_t)1 << 28;
 
 // cancelPairs (discreteMorseAlgorithm.h:348-464 / :466-585): cp0 is made
 // the lower-dimensional cell; every asc(cp0) x des(cp1) pair is connected
@@ -477,7 +482,7 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   };
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
-  m.arc_budget = (int64_t)(16 * narcs + ((size_t)1 << 20));
+  m.arc_budget = kArcEntryBudget;
   ExactHeap pq;
   pq.reserve(narcs * 2 + 16);
   slots.reserve(narcs * 4 + 16);
@@ -535,14 +540,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   }
 }
 
-static int64_t arc_count(const Complex &m) {
-  int64_t n = 0;
-  for (uint32_t i = 0; i < m.size(); ++i) n += m.asc[i].size() + m.des[i].size();
-  return n;
-}
-
 void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc) {
-  m.arc_budget = 16 * arc_count(m) + ((int64_t)1 << 20);
+  m.arc_budget = kArcEntryBudget;
   for (auto it = canc.rbegin(); it != canc.rend(); ++it) uncancel_pairs(m, it->first, it->second);
   m.arc_budget = INT64_MAX;
 }
