@@ -71,21 +71,42 @@ MSCX_HD bool on_ext_bndry(const TileGeom &g, int cx, int cy) {
   return cx == 0 || cy == 0 || cx == g.cw - 1 || cy == g.ch - 1;
 }
 
-MSCX_HD void sort4(uint64_t &a, uint64_t &b, uint64_t &c, uint64_t &d) {
-#define MSCX_CS(x, y) \
-  if (y < x) {        \
-    uint64_t t = x;   \
-    x = y;            \
-    y = t;            \
+// Vertex key with the reference's order (value, then x, then y) kept as
+// separate fields: when the kernels compare neighbours whose relative
+// positions are fixed, the position tie-break folds to a constant and the
+// comparison becomes one 32-bit compare.  Same order as vkey().
+struct VKey {
+  uint32_t f;
+  int i, j;
+};
+MSCX_HD bool operator<(const VKey &a, const VKey &b) {
+  return a.f != b.f ? a.f < b.f : (a.i != b.i ? a.i < b.i : a.j < b.j);
+}
+MSCX_HD bool operator>(const VKey &a, const VKey &b) { return b < a; }
+MSCX_HD bool operator==(const VKey &a, const VKey &b) { return a.f == b.f && a.i == b.i && a.j == b.j; }
+MSCX_HD bool operator!=(const VKey &a, const VKey &b) { return !(a == b); }
+
+template <class T>
+MSCX_HD void cswap(T &x, T &y) {
+  if (y < x) {
+    T t = x;
+    x = y;
+    y = t;
   }
-  MSCX_CS(a, b) MSCX_CS(c, d) MSCX_CS(a, c) MSCX_CS(b, d) MSCX_CS(b, c)
-#undef MSCX_CS
+}
+template <class T>
+MSCX_HD void sort4(T &a, T &b, T &c, T &d) {
+  cswap(a, b);
+  cswap(c, d);
+  cswap(a, c);
+  cswap(b, d);
+  cswap(b, c);
 }
 
 // Lexicographic "quad p < quad q" on ascending-sorted vertex keys
 // (grid_dataset.cpp:1519-1536 for dim-2 cells).
-MSCX_HD bool quad_lt(uint64_t p0, uint64_t p1, uint64_t p2, uint64_t p3, uint64_t q0, uint64_t q1, uint64_t q2,
-                     uint64_t q3) {
+template <class T>
+MSCX_HD bool quad_lt(T p0, T p1, T p2, T p3, T q0, T q1, T q2, T q3) {
   sort4(p0, p1, p2, p3);
   sort4(q0, q1, q2, q3);
   if (p0 != q0) return p0 < q0;
@@ -99,28 +120,28 @@ MSCX_HD bool quad_lt(uint64_t p0, uint64_t p1, uint64_t p2, uint64_t p3, uint64_
 // for v on the ext boundary, the edge also lies on it.  Returns DIR_*.
 template <class KF>
 MSCX_HD int vertex_cand(const KF &K, const TileGeom &g, int i, int j) {
-  uint64_t kv = K(i, j);
+  const auto kv = K(i, j);
   bool tb = (i == 0 || j == 0 || i == g.vx - 1 || j == g.vy - 1);
   int best = DIR_NONE;
-  uint64_t bk = ~(uint64_t)0;
+  auto bk = kv;  // meaningful once best != DIR_NONE
   // +x / -x: horizontal edge on the boundary iff its row is a boundary row
   bool hrow_tb = (j == 0 || j == g.vy - 1);
   bool vcol_tb = (i == 0 || i == g.vx - 1);
   if (i + 1 < g.vx && (!tb || hrow_tb)) {
-    uint64_t k = K(i + 1, j);
-    if (k < kv && k < bk) { bk = k; best = DIR_PX; }
+    const auto k = K(i + 1, j);
+    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_PX; }
   }
   if (i - 1 >= 0 && (!tb || hrow_tb)) {
-    uint64_t k = K(i - 1, j);
-    if (k < kv && k < bk) { bk = k; best = DIR_NX; }
+    const auto k = K(i - 1, j);
+    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_NX; }
   }
   if (j + 1 < g.vy && (!tb || vcol_tb)) {
-    uint64_t k = K(i, j + 1);
-    if (k < kv && k < bk) { bk = k; best = DIR_PY; }
+    const auto k = K(i, j + 1);
+    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_PY; }
   }
   if (j - 1 >= 0 && (!tb || vcol_tb)) {
-    uint64_t k = K(i, j - 1);
-    if (k < kv && k < bk) { bk = k; best = DIR_NY; }
+    const auto k = K(i, j - 1);
+    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_NY; }
   }
   return best;
 }
@@ -133,12 +154,12 @@ MSCX_HD int vertex_cand(const KF &K, const TileGeom &g, int i, int j) {
 template <class KF>
 MSCX_HD int hedge_cand(const KF &K, const TileGeom &g, int i, int j) {
   if (j == 0 || j == g.vy - 1) return DIR_NONE;
-  uint64_t a = K(i, j), b = K(i + 1, j);
+  const auto a = K(i, j), b = K(i + 1, j);
   bool a_hi = a > b;
-  uint64_t hi = a_hi ? a : b, lo = a_hi ? b : a;
+  const auto hi = a_hi ? a : b, lo = a_hi ? b : a;
   int hi_i = a_hi ? i : i + 1, lo_i = a_hi ? i + 1 : i;
-  uint64_t u_hi = K(hi_i, j + 1), u_lo = K(lo_i, j + 1);
-  uint64_t d_hi = K(hi_i, j - 1), d_lo = K(lo_i, j - 1);
+  const auto u_hi = K(hi_i, j + 1), u_lo = K(lo_i, j + 1);
+  const auto d_hi = K(hi_i, j - 1), d_lo = K(lo_i, j - 1);
   bool up = (u_hi < lo) && (u_lo < hi);
   bool dn = (d_hi < lo) && (d_lo < hi);
   if (up && dn) return quad_lt(lo, hi, u_hi, u_lo, lo, hi, d_hi, d_lo) ? DIR_PY : DIR_NY;
@@ -150,12 +171,12 @@ MSCX_HD int hedge_cand(const KF &K, const TileGeom &g, int i, int j) {
 template <class KF>
 MSCX_HD int vedge_cand(const KF &K, const TileGeom &g, int i, int j) {
   if (i == 0 || i == g.vx - 1) return DIR_NONE;
-  uint64_t a = K(i, j), b = K(i, j + 1);
+  const auto a = K(i, j), b = K(i, j + 1);
   bool a_hi = a > b;
-  uint64_t hi = a_hi ? a : b, lo = a_hi ? b : a;
+  const auto hi = a_hi ? a : b, lo = a_hi ? b : a;
   int hi_j = a_hi ? j : j + 1, lo_j = a_hi ? j + 1 : j;
-  uint64_t r_hi = K(i + 1, hi_j), r_lo = K(i + 1, lo_j);
-  uint64_t l_hi = K(i - 1, hi_j), l_lo = K(i - 1, lo_j);
+  const auto r_hi = K(i + 1, hi_j), r_lo = K(i + 1, lo_j);
+  const auto l_hi = K(i - 1, hi_j), l_lo = K(i - 1, lo_j);
   bool rt = (r_hi < lo) && (r_lo < hi);
   bool lf = (l_hi < lo) && (l_lo < hi);
   if (rt && lf) return quad_lt(lo, hi, r_hi, r_lo, lo, hi, l_hi, l_lo) ? DIR_PX : DIR_NX;
@@ -201,7 +222,7 @@ MSCX_HD uint8_t cell_code(const KF &K, const TileGeom &g, int cx, int cy) {
     }
   } else {  // quad: paired with its largest edge when that edge picks it
     int i = cx >> 1, j = cy >> 1;
-    uint64_t k00 = K(i, j), k10 = K(i + 1, j), k01 = K(i, j + 1), k11 = K(i + 1, j + 1);
+    const auto k00 = K(i, j), k10 = K(i + 1, j), k01 = K(i, j + 1), k11 = K(i + 1, j + 1);
     // maximum corner and its larger quad neighbour give the largest edge
     int edir;  // direction from the quad to that edge
     if (k00 > k10 && k00 > k01 && k00 > k11) edir = (k10 > k01) ? DIR_NY : DIR_NX;
@@ -252,7 +273,7 @@ MSCX_HD uint8_t code_from_cands(const KF &K, const VCF &VC, const HCF &HC, const
       if (a_hi ? VC(i, j) == DIR_PY : VC(i, j + 1) == DIR_NY) dir = a_hi ? DIR_NY : DIR_PY;
     }
   } else {
-    const uint64_t k00 = K(i, j), k10 = K(i + 1, j), k01 = K(i, j + 1), k11 = K(i + 1, j + 1);
+    const auto k00 = K(i, j), k10 = K(i + 1, j), k01 = K(i, j + 1), k11 = K(i + 1, j + 1);
     int edir;
     if (k00 > k10 && k00 > k01 && k00 > k11) edir = (k10 > k01) ? DIR_NY : DIR_NX;
     else if (k10 > k01 && k10 > k11) edir = (k00 > k11) ? DIR_NY : DIR_PX;

@@ -39,6 +39,62 @@ static void par_for(int n, F f) {
     if (e) std::rethrow_exception(e);
 }
 
+// The label of disc_labels_host (grid_dataset.cpp:1142-1187) depends on a
+// cell only through its owner o and whether the cell is itself critical:
+// label = map[o2] with o2 = own2(o) when o lies in the rect, where own2(c)
+// is c's redirect if it has one and its owner otherwise.  A critical cell
+// owns itself and is looked up through own2 first.  Both cases are tabled
+// per critical index k of the leaf: owned[k] (cell not critical, owner k)
+// and self[k] (the critical cell k itself).  A cell owning another cell is
+// always critical in the leaf, except possibly a redirect target; for such a
+// target the owner is read from the device image.
+static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, const DiscPrep &dp,
+                              const mscx_tile *t, uint32_t *owned, uint32_t *self) {
+  auto find = [](const std::vector<std::pair<uint32_t, uint32_t>> &v, uint32_t key) {
+    auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(key, (uint32_t)0));
+    return (it != v.end() && it->first == key) ? it->second : UINT32_MAX;
+  };
+  auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
+  auto owner_of = [&](uint32_t c) -> uint32_t {  // own[c] for a cell c inside the ext
+    uint32_t i = m.idx.find(c);
+    if (i != UINT32_MAX && i < K) return c;
+    uint32_t oi = 0;
+    const int64_t off = (int64_t)(cy_of(c) - t->ext[1]) * t->d.own_pitch + (cx_of(c) - t->ext[0]);
+    MSCX_HIP(hipMemcpy(&oi, t->d.own + off, 4, hipMemcpyDeviceToHost));
+    return oi == OWN_NONE ? UINT32_MAX : m.cell[oi];
+  };
+  auto own2 = [&](uint32_t c) {
+    uint32_t r = find(dp.redirect, c);
+    return r != UINT32_MAX ? r : owner_of(c);
+  };
+  auto label = [&](uint32_t o, const std::vector<std::pair<uint32_t, uint32_t>> &mp) -> uint32_t {
+    if (in_r(o)) o = own2(o);
+    if (o == UINT32_MAX || cx_of(o) == -1 || cy_of(o) == -1) return UINT32_MAX;
+    uint32_t k = find(mp, o);
+    return k == UINT32_MAX ? LAB_ERR : k;
+  };
+  // a redirect at a vertex or quad that the leaf does not hold as critical
+  // would change the label of a non-critical cell, which the tables cannot
+  // express
+  for (const auto &r : dp.redirect) {
+    if (cell_dim(r.first) == 1 || !in_r(r.first)) continue;
+    uint32_t i = m.idx.find(r.first);
+    if (i == UINT32_MAX || i >= K) throw std::logic_error("owner redirect at a cell that is not critical in the leaf");
+  }
+  for (uint32_t k = 0; k < K; ++k) {
+    const uint32_t c = m.cell[k];
+    const uint32_t dim = cell_dim(c);
+    if (dim == 1) {  // saddles own no vertex or quad
+      owned[k] = self[k] = LAB_ERR;
+      continue;
+    }
+    const auto &mp = dim == 0 ? dp.minmap : dp.maxmap;
+    owned[k] = label(c, mp);
+    const uint32_t r = find(dp.redirect, c);
+    self[k] = label(r != UINT32_MAX ? r : c, mp);
+  }
+}
+
 void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   if (n <= 0) return;
   ensure_init();
@@ -79,42 +135,40 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   // 2. critical-point pass (seeds, redirects, surviving maps)
   par_for(n, [&](int i) { disc_prepare(*cs[i], rect[i], cv[i], dp[i]); });
 
-  // 3. device tables + label pass
-  size_t tab_words = 0;
-  for (int i = 0; i < n; ++i) tab_words += 2 * (dp[i].redirect.size() + dp[i].minmap.size() + dp[i].maxmap.size());
+  // 3. label tables per critical index (host) + label pass (device)
+  std::vector<size_t> tab_off(n + 1, 0);
+  for (int i = 0; i < n; ++i) tab_off[i + 1] = tab_off[i] + align_up(2 * (size_t)tiles[i]->K * 4, 256);
   const size_t o_desc = 0, o_pref = align_up(sizeof(DiscDesc) * n, 256);
   const size_t o_err = align_up(o_pref + 4 * (n + 1), 256);
   const size_t o_tab = o_err + 256;
-  const size_t o_lab = align_up(o_tab + 4 * tab_words, 256);
+  const size_t o_lab = o_tab + tab_off[n];
   const size_t dev_bytes = o_lab + lab_total;
   char *dev = (char *)g_disc.reserve_dev(dev_bytes);
   char *stage = (char *)g_stage.reserve_host(o_lab);
   DiscDesc *hd = (DiscDesc *)(stage + o_desc);
-  uint32_t *tab = (uint32_t *)(stage + o_tab);
-  size_t tw = 0, lo = 0;
+  par_for(n, [&](int i) {
+    const uint32_t K = tiles[i]->K;
+    uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
+    disc_label_tables(*cs[i], K, rect[i], dp[i], tiles[i], owned, owned + K);
+  });
+  size_t lo = 0;
   std::vector<size_t> lab_off(n);
   for (int i = 0; i < n; ++i) {
     const mscx_tile *t = tiles[i];
     DiscDesc &D = hd[i];
     D.own = t->d.own;
     D.own_pitch = t->d.own_pitch;
+    D.codes = t->d.codes;
+    D.pitch = t->d.pitch;
     D.ox = t->ext[0];
     D.oy = t->ext[1];
     D.rx0 = rect[i].x0;
     D.ry0 = rect[i].y0;
     D.rx1 = rect[i].x1;
     D.ry1 = rect[i].y1;
-    auto put = [&](const std::vector<std::pair<uint32_t, uint32_t>> &v, const uint32_t *&dptr, uint32_t &cnt) {
-      dptr = (const uint32_t *)(dev + o_tab) + tw;
-      cnt = (uint32_t)v.size();
-      for (const auto &e : v) {
-        tab[tw++] = e.first;
-        tab[tw++] = e.second;
-      }
-    };
-    put(dp[i].redirect, D.redir, D.nredir);
-    put(dp[i].minmap, D.minmap, D.nmin);
-    put(dp[i].maxmap, D.maxmap, D.nmax);
+    D.lab_owned = (const uint32_t *)(dev + o_tab + tab_off[i]);
+    D.lab_self = D.lab_owned + t->K;
+    D.nlab = t->K;
     lab_off[i] = lo;
     D.lab_v = (uint32_t *)(dev + o_lab + lo);
     D.lab_q = D.lab_v + nv[i];
@@ -123,7 +177,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
   int nblocks = plan_disc_blocks(hd, n, (int *)(stage + o_pref));
   *(uint32_t *)(stage + o_err) = 0;
-  MSCX_HIP(hipMemcpyAsync(dev, stage, o_tab + 4 * tw, hipMemcpyHostToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(dev, stage, o_lab, hipMemcpyHostToDevice, s));
   launch_disc_labels((const DiscDesc *)dev, (const int *)(dev + o_pref), n, nblocks, s);
   MSCX_HIP(hipGetLastError());
   char *labels = host + codes_bytes;

@@ -1,8 +1,9 @@
 // CDNA4 (gfx950) kernels for the per-tile discrete Morse-Smale stages:
-// gradient assignment, critical-cell collation (segment counts, scan,
-// compaction), manifold tracing (pointer jumping), owner images and saddle
-// incidence.  All kernels take an array of tiles so one launch covers every
-// tile of a tiling or of a batch of grids.
+// gradient assignment with a block-local manifold trace, resolution of the
+// paths that cross block tiles, owner images, critical-cell collation (scan,
+// compaction), saddle incidence and the manifold label pass.  All kernels
+// take an array of tiles so one launch covers every tile of a tiling or of a
+// batch of grids.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -21,219 +22,345 @@ __device__ __forceinline__ int find_tile(const int *__restrict__ prefix, int nti
   return lo;
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 __device__ __forceinline__ uint32_t pack_cell(int x, int y) { return (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16); }
 
-// ------------------------------------------------------- gradient + trace
-// One block = a GRAD_TX x GRAD_TY vertex tile (cells 2*GRAD_TX x 2*GRAD_TY).
-//  1. vertex keys of the tile and a 1/2 vertex halo are staged in LDS
-//  2. the candidate pair of every vertex, horizontal and vertical edge the
-//     tile's cells refer to is computed once (vertex_cand / hedge_cand /
-//     vedge_cand) into LDS
-//  3. every cell's gradient code (code_from_cands) goes to LDS
-//  4. the gradient forests are traced inside the tile: each vertex points
-//     across its paired edge to the next vertex and each quad across its
-//     paired edge to the next quad (grid_dataset.cpp:1376-1427); pointer
-//     jumping in LDS resolves every path to its critical root, to "none", or
-//     to the first cell outside the tile (a marker)
-//  5. codes, the owner image (edges take the owner of their paired vertex or
-//     quad), per-segment critical counts and the list of marker entries are
-//     written with coalesced row stores.
-// A lane owns one vertex column (two cells per cell row).
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
-__device__ __forceinline__ bool lds_is_link(uint32_t w) { return (w & 0xFFFF8000u) == 0x00008000u; }
-
-__global__ __launch_bounds__(256) void k_grad_trace(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                                    int ntiles) {
+// ------------------------------------------------- gradient + local trace
+// One block = a GRAD_TX x GRAD_TY vertex tile (cells 2*GRAD_TX x 2*GRAD_TY);
+// a lane owns one vertex column and, per vertex (i, j), the four cells
+// (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1).
+//  1. keys of the tile's vertices and a 1/2-vertex halo go to LDS (all
+//     global loads of a lane are issued before the first LDS store)
+//  2. the pairing candidate of every vertex and of both edges starting at
+//     it (vertex_cand / hedge_cand / vedge_cand) is computed once into LDS
+//  3. the four cell codes of every vertex position (code_from_cands), the
+//     critical flags of each 128-cell row segment (two ballots) and the
+//     trace links: a vertex points across its paired edge to the next
+//     vertex, a quad across its paired edge to the next quad
+//     (grid_dataset.cpp:1376-1427)
+//  4. pointer jumping in LDS takes every entry to its terminal in the tile:
+//     its critical root, "none", or the exit through the tile's side
+//  5. terminals go to `loc` (16 bits per vertex and per quad); the
+//     terminals of the tile's perimeter entries go to its ring words, in the
+//     form the ring resolution chases.
+__global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                                    int ntiles, uint32_t *err) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY;
   constexpr int KW = TX + 3, KH = TY + 3;  // keys of vertices [i0-1, i0+TX+1] x [j0-1, j0+TY+1]
-  constexpr int VW = TX + 1, VH = TY + 1;  // vertex candidates [i0, i0+TX] x [j0, j0+TY]
-  constexpr int HW = TX, HH = TY + 1;      // horizontal-edge candidates [i0, i0+TX) x [j0, j0+TY]
-  constexpr int EW = TX + 1, EH = TY;      // vertical-edge candidates [i0, i0+TX] x [j0, j0+TY)
-  constexpr int CW = 2 * TX, CH = 2 * TY;  // cells of the tile
-  constexpr int NV = TX * TY;              // trace entries: vertices, then quads
-  __shared__ uint32_t sk[KW * KH];
-  __shared__ uint8_t svc[VW * VH], shc[HW * HH], sec[EW * EH];
-  __shared__ uint8_t scode[CW * CH];
-  __shared__ uint32_t sl[2 * NV];
-  __shared__ uint32_t s_nexit;
+  constexpr int CW1 = TX + 1, CH1 = TY + 1;  // candidates of vertices [i0, i0+TX] x [j0, j0+TY]
+  constexpr int NV = TX * TY;              // entries per forest
+  constexpr int RW = TY / 4;               // vertex rows per wave
+  __shared__ uint32_t sk[KH * KW];
+  __shared__ uint16_t sc[CH1 * CW1];
+  __shared__ uint16_t sl[2 * NV];
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc &T = tiles[t];
   const TileGeom g = T.g;
   const int lb = b - prefix[t];
-  const int nbx = (g.vx + TX - 1) / TX;
+  const int nbx = T.nseg;
   const int bx = lb % nbx, by = lb / nbx;
   const int i0 = bx * TX, j0 = by * TY;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  // 1. keys
-  for (int idx = tid; idx < KW * KH; idx += 256) {
-    const int gi = i0 - 1 + idx % KW, gj = j0 - 1 + idx / KW;
-    uint32_t k = 0;
-    if (gi >= 0 && gj >= 0 && gi < g.vx && gj < g.vy) k = fkey(T.verts[(int64_t)gj * T.vpitch + gi]);
-    sk[idx] = k;
+  // 1. keys (0 outside the ext window; the rules never read those)
+  {
+    constexpr int NR = (KH + 3) / 4;
+    float vm[NR], vh[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int r = wv + 4 * k;
+      const int gj = j0 - 1 + r;
+      const bool rowok = r < KH && gj >= 0 && gj < g.vy;
+      const int gi = i0 - 1 + lane, gi2 = i0 + TX - 1 + lane;
+      vm[k] = (rowok && gi >= 0 && gi < g.vx) ? T.verts[(int64_t)gj * T.vpitch + gi] : 0.0f;
+      vh[k] = (rowok && lane < 3 && gi2 < g.vx) ? T.verts[(int64_t)gj * T.vpitch + gi2] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int r = wv + 4 * k;
+      if (r < KH) {
+        sk[r * KW + lane] = fkey(vm[k]);
+        if (lane < 3) sk[r * KW + TX + lane] = fkey(vh[k]);
+      }
+    }
   }
   __syncthreads();
-  auto K = [&](int i, int j) -> uint64_t { return vkey(sk[(j - j0 + 1) * KW + (i - i0 + 1)], i, j); };
+  auto K = [&](int i, int j) -> VKey { return VKey{sk[(j - j0 + 1) * KW + (i - i0 + 1)], i, j}; };
 
   // 2. candidates (DIR_NONE outside the grid or the interior rect)
-  for (int idx = tid; idx < VW * VH; idx += 256) {
-    const int i = i0 + idx % VW, j = j0 + idx / VW;
-    int c = DIR_NONE;
-    if (i < g.vx && j < g.vy && in_rect(g, 2 * i, 2 * j)) c = vertex_cand(K, g, i, j);
-    svc[idx] = (uint8_t)c;
-  }
-  for (int idx = tid; idx < HW * HH; idx += 256) {
-    const int i = i0 + idx % HW, j = j0 + idx / HW;
-    int c = DIR_NONE;
-    if (i + 1 < g.vx && j < g.vy && in_rect(g, 2 * i + 1, 2 * j)) c = hedge_cand(K, g, i, j);
-    shc[idx] = (uint8_t)c;
-  }
-  for (int idx = tid; idx < EW * EH; idx += 256) {
-    const int i = i0 + idx % EW, j = j0 + idx / EW;
-    int c = DIR_NONE;
-    if (i < g.vx && j + 1 < g.vy && in_rect(g, 2 * i, 2 * j + 1)) c = vedge_cand(K, g, i, j);
-    sec[idx] = (uint8_t)c;
+  for (int idx = tid; idx < CW1 * CH1; idx += 256) {
+    const int li = idx % CW1, lj = idx / CW1;
+    const int i = i0 + li, j = j0 + lj;
+    int vc = DIR_NONE, hc = DIR_NONE, ec = DIR_NONE;
+    if (i < g.vx && j < g.vy) {
+      if (in_rect(g, 2 * i, 2 * j)) vc = vertex_cand(K, g, i, j);
+      if (i + 1 < g.vx && in_rect(g, 2 * i + 1, 2 * j)) hc = hedge_cand(K, g, i, j);
+      if (j + 1 < g.vy && in_rect(g, 2 * i, 2 * j + 1)) ec = vedge_cand(K, g, i, j);
+    }
+    sc[idx] = (uint16_t)(vc | (hc << 3) | (ec << 6));
   }
   __syncthreads();
-  auto VC = [&](int i, int j) -> int { return svc[(j - j0) * VW + (i - i0)]; };
-  auto HC = [&](int i, int j) -> int { return shc[(j - j0) * HW + (i - i0)]; };
-  auto EC = [&](int i, int j) -> int { return sec[(j - j0) * EW + (i - i0)]; };
+  auto SC = [&](int i, int j) -> uint32_t { return sc[(j - j0) * CW1 + (i - i0)]; };
+  auto VC = [&](int i, int j) -> int { return (int)(SC(i, j) & 7u); };
+  auto HC = [&](int i, int j) -> int { return (int)((SC(i, j) >> 3) & 7u); };
+  auto EC = [&](int i, int j) -> int { return (int)((SC(i, j) >> 6) & 7u); };
 
-  // 3. codes of the tile's cells
-  for (int idx = tid; idx < CW * CH; idx += 256) {
-    const int cx = 2 * i0 + idx % CW, cy = 2 * j0 + idx / CW;
-    uint8_t c = 0;
-    if (cx < g.cw && cy < g.ch) c = code_from_cands(K, VC, HC, EC, g, cx, cy);
-    scode[idx] = c;
-  }
-  __syncthreads();
+  // trace link of a vertex (q = 0) or quad (q = 1) entry from its code
+  auto link_of = [&](uint8_t c, int li, int lj, int q) -> uint32_t {
+    if (c & CF_CRITICAL) return T_ROOT | (uint32_t)(lj * TX + li);
+    if (!(c & CF_PAIRED)) return T_NONE;
+    const int d = code_dir(c);
+    const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
+    if (ni >= 0 && nj >= 0 && ni < TX && nj < TY) return (uint32_t)(q * NV + nj * TX + ni);
+    const uint32_t s = ni >= TX ? 0u : (ni < 0 ? 1u : (nj >= TY ? 2u : 3u));
+    const uint32_t p = (s < 2) ? (uint32_t)lj : (uint32_t)li;
+    return T_EXIT | (s << 6) | p;
+  };
 
-  // 4. block-local trace.  LDS word: 0x8000 | entry = link to another entry
-  //    of this tile; otherwise the final owner value (packed root id,
-  //    OWN_NONE, or OWN_MARK | id of the first cell outside the tile).
-  for (int e = tid; e < 2 * NV; e += 256) {
-    const bool quad = e >= NV;
-    const int li = (e % NV) % TX, lj = (e % NV) / TX;
-    const int lx = 2 * li + quad, ly = 2 * lj + quad;
-    const int cx = 2 * i0 + lx, cy = 2 * j0 + ly;
-    uint32_t w = OWN_NONE;
-    if (cx < g.cw && cy < g.ch) {
-      const uint8_t c = scode[ly * CW + lx];
-      if (c & CF_CRITICAL) {
-        w = pack_cell(T.ox + cx, T.oy + cy);
-      } else if (c & CF_PAIRED) {
-        const int d = code_dir(c);
-        const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
-        if (ni >= 0 && nj >= 0 && ni < TX && nj < TY)
-          w = 0x8000u | (uint32_t)((quad ? NV : 0) + nj * TX + ni);
-        else
-          w = OWN_MARK | pack_cell(T.ox + cx + 2 * dir_dx(d), T.oy + cy + 2 * dir_dy(d));
+  // 3. codes, critical flags per row segment, trace links
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r, li = lane;
+    const int i = i0 + li, j = j0 + lj;
+    if (j >= g.vy) break;  // uniform across the wave
+    const bool vin = i < g.vx, hin = i + 1 < g.vx, ein = j + 1 < g.vy;
+    uint8_t cv = 0, chh = 0, ce = 0, cq = 0;
+    if (vin) cv = code_from_cands(K, VC, HC, EC, g, 2 * i, 2 * j);
+    if (hin) chh = code_from_cands(K, VC, HC, EC, g, 2 * i + 1, 2 * j);
+    if (vin && ein) ce = code_from_cands(K, VC, HC, EC, g, 2 * i, 2 * j + 1);
+    if (hin && ein) cq = code_from_cands(K, VC, HC, EC, g, 2 * i + 1, 2 * j + 1);
+    uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
+    *(uint16_t *)crow = (uint16_t)(cv | (chh << 8));
+    const int64_t rs0 = (int64_t)(2 * j) * T.nseg + bx;
+    const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
+    if (lane == 0) {
+      T.critbits[2 * rs0] = b0;
+      T.critbits[2 * rs0 + 1] = b1;
+      T.seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
+    }
+    if (ein) {
+      *(uint16_t *)(crow + T.pitch) = (uint16_t)(ce | (cq << 8));
+      const int64_t rs1 = rs0 + T.nseg;
+      const uint64_t b2 = __ballot(ce & CF_CRITICAL), b3 = __ballot(cq & CF_CRITICAL);
+      if (lane == 0) {
+        T.critbits[2 * rs1] = b2;
+        T.critbits[2 * rs1 + 1] = b3;
+        T.seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
       }
     }
-    sl[e] = w;
+    sl[lj * TX + li] = (uint16_t)(vin ? link_of(cv, li, lj, 0) : T_NONE);
+    sl[NV + lj * TX + li] = (uint16_t)((hin && ein) ? link_of(cq, li, lj, 1) : T_NONE);
+  }
+  // rows past the ext edge hold no entries
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r;
+    if (j0 + lj < g.vy) continue;
+    sl[lj * TX + lane] = (uint16_t)T_NONE;
+    sl[NV + lj * TX + lane] = (uint16_t)T_NONE;
   }
   __syncthreads();
-  for (;;) {
-    int more = 0;
-    for (int e = tid; e < 2 * NV; e += 256) {
-      const uint32_t w = sl[e];
-      if (lds_is_link(w)) {
-        const uint32_t w2 = sl[w & 0x7FFFu];
-        sl[e] = w2;
-        more |= lds_is_link(w2);
-      }
-    }
-    if (!__syncthreads_or(more)) break;
-  }
 
-  // 5. row stores: codes, owners, critical counts, marker list (the block's
-  //    own region of T.exits, CW * CH entries, count in T.nexit[lb])
-  if (tid == 0) s_nexit = 0;
-  __syncthreads();
-  uint32_t *const xlist = T.exits + (int64_t)lb * (CW * CH);
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int r = wv; r < CH; r += 4) {
-    const int cy = 2 * j0 + r;
-    if (cy >= g.ch) break;
-    const int lx = 2 * lane, cx = 2 * i0 + lx;
-    const uint8_t c0 = scode[r * CW + lx], c1 = scode[r * CW + lx + 1];
-    *(uint16_t *)(T.codes + (int64_t)cy * T.pitch + cx) = (uint16_t)(c0 | (c1 << 8));
-    uint32_t o[2];
+  // 4. pointer jumping.  Links only ever move further along their path, so
+  //    the entries are updated without barriers; gradient paths are acyclic
+  //    and every round moves an unresolved entry at least one step, so the
+  //    loop ends within the longest path (the bound guards bad input).
+  {
+    volatile uint16_t *vl = sl;
+    constexpr int PER = 2 * NV / 256;
+    uint32_t pend = (1u << PER) - 1u;
+    for (int round = 0; pend && round < 2 * NV; ++round) {
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const uint8_t c = k ? c1 : c0;
-      const int x = lx + k, gx = cx + k;
-      uint32_t v = OWN_NONE;
-      if (gx < g.cw) {
-        const int dx = x & 1, dy = r & 1;
-        if (dx == dy) {  // vertex (dx = 0) or quad (dx = 1)
-          v = sl[(dx ? NV : 0) + (r >> 1) * TX + (x >> 1)];
-        } else if (c & CF_CRITICAL) {
-          v = pack_cell(T.ox + gx, T.oy + cy);
-        } else if (c & CF_PAIRED) {
-          const int d = code_dir(c);
-          const int px = x + dir_dx(d), py = r + dir_dy(d);
-          if (px >= 0 && py >= 0 && px < CW && py < CH)
-            v = sl[((px & 1) ? NV : 0) + (py >> 1) * TX + (px >> 1)];
-          else
-            v = OWN_MARK | pack_cell(T.ox + gx + dir_dx(d), T.oy + cy + dir_dy(d));
+      for (int k = 0; k < PER; ++k) {
+        if (!(pend & (1u << k))) continue;
+        const int e = tid + 256 * k;
+        const uint32_t w = vl[e];
+        if (w & 0x8000u) {
+          pend &= ~(1u << k);
+          continue;
         }
+        const uint32_t w2 = vl[w];
+        vl[e] = (uint16_t)w2;
+        if (w2 & 0x8000u) pend &= ~(1u << k);
       }
-      o[k] = v;
     }
-    *(uint2 *)(T.own + (int64_t)cy * T.own_pitch + cx) = make_uint2(o[0], o[1]);
-    const uint64_t crit_mask = __ballot((c0 >> 1) & 1) , crit_mask1 = __ballot((c1 >> 1) & 1);
-    if (lane == 0) T.seg_counts[(int64_t)cy * T.nseg + bx] = (uint32_t)(__popcll(crit_mask) + __popcll(crit_mask1));
-    const bool m0 = own_is_mark(o[0]), m1 = own_is_mark(o[1]);
-    const uint64_t mm0 = __ballot(m0), mm1 = __ballot(m1);
-    const uint32_t nm = (uint32_t)(__popcll(mm0) + __popcll(mm1));
-    if (nm) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&s_nexit, nm);
-      base = __shfl(base, 0, 64);
-      const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-      uint32_t pos = base + (uint32_t)(__popcll(mm0 & lt) + __popcll(mm1 & lt));
-      const uint32_t rowbase = (uint32_t)cy * (uint32_t)T.own_pitch + (uint32_t)cx;
-      if (m0) xlist[pos++] = rowbase;
-      if (m1) xlist[pos] = rowbase + 1;
+    if (pend) {
+      atomicOr(err, 2u);
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        if (pend & (1u << k)) vl[tid + 256 * k] = (uint16_t)T_NONE;
     }
   }
   __syncthreads();
-  if (tid == 0) T.nexit[lb] = s_nexit;
+
+  // 5. terminals and ring words
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r;
+    const int j = j0 + lj;
+    if (j >= g.vy) break;
+    T.loc[(int64_t)j * T.loc_pitch + i0 + lane] = (uint32_t)sl[lj * TX + lane] | ((uint32_t)sl[NV + lj * TX + lane] << 16);
+  }
+  uint32_t *ring = T.ring + (int64_t)lb * RING_W;
+  for (int p = tid; p < RING_W; p += 256) {
+    const int q = p >= RING_Q ? 1 : 0;
+    const int pp = p - q * RING_Q;
+    if (pp >= RING_N) continue;
+    int li, lj;
+    ring_at(pp, li, lj);
+    uint32_t out = OWN_NONE;
+    if (i0 + li + q < g.vx && j0 + lj + q < g.vy) {
+      const uint32_t w = sl[q * NV + lj * TX + li];
+      const uint32_t kind = term_kind(w);
+      if (kind == 0) {
+        const int e = (int)(w & 0x7FFu);
+        out = pack_cell(2 * (i0 + e % TX) + q, 2 * (j0 + e / TX) + q);
+      } else if (kind == 1) {
+        const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
+        int tbx = bx, tby = by, tli, tlj;
+        if (s == 0) { tbx = bx + 1; tli = 0; tlj = pos; }
+        else if (s == 1) { tbx = bx - 1; tli = TX - 1; tlj = pos; }
+        else if (s == 2) { tby = by + 1; tli = pos; tlj = 0; }
+        else { tby = by - 1; tli = pos; tlj = TY - 1; }
+        // a path never leaves the ext window, so the neighbour exists
+        if (tbx >= 0 && tby >= 0 && tbx < nbx && tby * TY < g.vy)
+          out = RING_MARK | (uint32_t)((tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj));
+        else
+          atomicOr(err, 8u);
+      }
+    }
+    ring[p] = out;
+  }
 }
 
-// Resolves the marker entries left by k_grad_trace: each follows the chain of
-// markers (first cell outside a tile -> that cell's owner entry -> ...) to
-// the owner at its end.  Chains follow gradient paths, so they end; the
-// bound only guards against a corrupted image.
-__global__ __launch_bounds__(256) void k_resolve_exits(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                                       int ntiles, uint32_t *err) {
+// Ring resolution: every perimeter entry follows ring words (one per block
+// tile crossed) to its root and stores the root's critical index.
+__global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                                      int ntiles, uint32_t *err) {
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc &T = tiles[t];
-  const int lb = b - prefix[t];
-  const uint32_t n = T.nexit[lb];
-  const uint32_t *xlist = T.exits + (int64_t)lb * (4 * GRAD_TX * GRAD_TY);
-  for (uint32_t e = threadIdx.x; e < n; e += 256) {
-    const uint32_t idx = xlist[e];
-    uint32_t w = T.own[idx];
-    int steps = 0;
-    while (own_is_mark(w) && steps < (1 << 24)) {
-      const uint32_t c = w & ~OWN_MARK;
-      const int x = (int)(c & 0xFFFF) - T.ox, y = (int)(c >> 16) - T.oy;
-      w = T.own[(int64_t)y * T.own_pitch + x];
-      ++steps;
+  const int nblk = T.nseg * ((T.g.vy + GRAD_TY - 1) / GRAD_TY);
+  const int64_t e = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
+  if (e >= (int64_t)nblk * RING_W) return;
+  const int p = (int)(e % RING_W);
+  if ((p % RING_Q) >= RING_N) return;
+  uint32_t w = T.ring[e];
+  for (int steps = 0; w != OWN_NONE && (w & RING_MARK); ++steps) {
+    if (steps > nblk) {
+      atomicOr(err, 4u);
+      w = OWN_NONE;
+      break;
     }
-    if (own_is_mark(w)) atomicOr(err, 1u);
-    T.own[idx] = w;
+    w = T.ring[w & ~RING_MARK];
+  }
+  uint32_t out = OWN_NONE;
+  if (w != OWN_NONE) {
+    const int x = (int)(w & 0xFFFFu), y = (int)(w >> 16);
+    const int64_t rs = (int64_t)y * T.nseg + (x / SEG);
+    out = crit_rank(T.seg_offs[rs], T.critbits[2 * rs], T.critbits[2 * rs + 1], x);
+  }
+  T.ringidx[e] = out;
+}
+
+// Owner images: the terminals of the block's entries become critical indices
+// (a root by its rank in the scanned critical flags, an exit by the resolved
+// ring word it leads to); an edge takes the owner of the vertex or quad it
+// is paired with, a critical cell owns itself.
+__global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                               int ntiles) {
+  constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY, RW = TY / 4;
+  __shared__ uint64_t sb[2 * TY * 2];
+  __shared__ uint32_t so[2 * TY];
+  __shared__ uint32_t dv[NV], dq[NV];
+
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, ntiles, b);
+  const TileDesc &T = tiles[t];
+  const TileGeom g = T.g;
+  const int lb = b - prefix[t];
+  const int nbx = T.nseg;
+  const int bx = lb % nbx, by = lb / nbx;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+  for (int r = tid; r < 2 * TY; r += 256) {
+    const int cy = 2 * j0 + r;
+    if (cy < g.ch) {
+      const int64_t rs = (int64_t)cy * T.nseg + bx;
+      sb[2 * r] = T.critbits[2 * rs];
+      sb[2 * r + 1] = T.critbits[2 * rs + 1];
+      so[r] = T.seg_offs[rs];
+    }
+  }
+  __syncthreads();
+  auto rank_local = [&](int lx, int ly) { return crit_rank(so[ly], sb[2 * ly], sb[2 * ly + 1], lx); };
+  auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
+    // the neighbour always exists for a valid gradient (paths stay in ext)
+    if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return OWN_NONE;
+    return T.ringidx[(int64_t)(tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj)];
+  };
+  auto decode = [&](uint32_t w, int q) -> uint32_t {
+    const uint32_t kind = term_kind(w);
+    if (kind == 0) {
+      const int e = (int)(w & 0x7FFu);
+      return rank_local(2 * (e % TX) + q, 2 * (e / TX) + q);
+    }
+    if (kind == 1) {
+      const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
+      if (s == 0) return ring_val(bx + 1, by, q, 0, pos);
+      if (s == 1) return ring_val(bx - 1, by, q, TX - 1, pos);
+      if (s == 2) return ring_val(bx, by + 1, q, pos, 0);
+      return ring_val(bx, by - 1, q, pos, TY - 1);
+    }
+    return OWN_NONE;
+  };
+
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r, j = j0 + lj;
+    uint32_t a = OWN_NONE, c = OWN_NONE;
+    if (j < g.vy && i0 + lane < g.vx) {
+      const uint32_t w = T.loc[(int64_t)j * T.loc_pitch + i0 + lane];
+      a = decode(w & 0xFFFFu, 0);
+      c = decode(w >> 16, 1);
+    }
+    dv[lj * TX + lane] = a;
+    dq[lj * TX + lane] = c;
+  }
+  __syncthreads();
+
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r, li = lane;
+    const int i = i0 + li, j = j0 + lj;
+    if (j >= g.vy) break;
+    const bool ein = j + 1 < g.vy;
+    const uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
+    const uint16_t w0 = *(const uint16_t *)crow;
+    const uint16_t w1 = ein ? *(const uint16_t *)(crow + T.pitch) : (uint16_t)0;
+    const uint8_t chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1;
+    const uint32_t ov = dv[lj * TX + li], oq = dq[lj * TX + li];
+    uint32_t oh = OWN_NONE, oe = OWN_NONE;
+    if (chh & CF_CRITICAL) {
+      oh = rank_local(2 * li + 1, 2 * lj);
+    } else if (chh & CF_PAIRED) {
+      const int d = code_dir(chh);
+      if (d == DIR_PX) oh = li + 1 < TX ? dv[lj * TX + li + 1] : ring_val(bx + 1, by, 0, 0, lj);
+      else if (d == DIR_NX) oh = ov;
+      else if (d == DIR_PY) oh = oq;
+      else oh = lj > 0 ? dq[(lj - 1) * TX + li] : ring_val(bx, by - 1, 1, li, TY - 1);
+    }
+    if (ce & CF_CRITICAL) {
+      oe = rank_local(2 * li, 2 * lj + 1);
+    } else if (ce & CF_PAIRED) {
+      const int d = code_dir(ce);
+      if (d == DIR_PY) oe = lj + 1 < TY ? dv[(lj + 1) * TX + li] : ring_val(bx, by + 1, 0, li, 0);
+      else if (d == DIR_NY) oe = ov;
+      else if (d == DIR_PX) oe = oq;
+      else oe = li > 0 ? dq[lj * TX + li - 1] : ring_val(bx - 1, by, 1, TX - 1, lj);
+    }
+    uint32_t *orow = T.own + (int64_t)(2 * j) * T.own_pitch + 2 * i;
+    *(uint2 *)orow = make_uint2(ov, oh);
+    if (ein) *(uint2 *)(orow + T.own_pitch) = make_uint2(oe, oq);
   }
 }
 
@@ -326,10 +453,10 @@ __global__ __launch_bounds__(256) void k_scan_down(const TileDesc *__restrict__ 
 }
 
 // ---------------------------------------------------------------- compaction
-// One wave per (cell row, segment): critical cells are written in raster
-// order at their scanned offsets with their cell function value
-// (grid_dataset.cpp:1538-1552: float sum in getCellPoints order, then the
-// division), and critical vertices/quads become resolved trace roots.
+// One wave per 8 consecutive (cell row, segment) items: critical cells are
+// written in raster order at their scanned offsets with their cell function
+// value (grid_dataset.cpp:1538-1552: float sum in getCellPoints order, then
+// the division).
 
 __device__ __forceinline__ float vval(const TileDesc &T, int i, int j) { return T.verts[(int64_t)j * T.vpitch + i]; }
 
@@ -359,31 +486,34 @@ __device__ __forceinline__ float cell_fn(const TileDesc &T, int cx, int cy) {
   return fn / 4.0f;
 }
 
+constexpr int COMPACT_ITEMS = 8;  // (row, segment) items per wave
+
 __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                  int ntiles) {
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc &T = tiles[t];
-  const TileGeom g = T.g;
-  const int lb = b - prefix[t];
   const int lane = threadIdx.x & 63;
-  const int64_t wid = (int64_t)lb * 4 + (threadIdx.x >> 6);
-  if (wid >= (int64_t)g.ch * T.nseg) return;
-  const int row = (int)(wid / T.nseg), seg = (int)(wid % T.nseg);
-  const uint16_t word = *(const uint16_t *)(T.codes + (int64_t)row * T.pitch + seg * SEG + 2 * lane);
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint64_t m0 = __ballot((word >> 1) & 1), m1 = __ballot((word >> 9) & 1);
-  uint32_t idx = T.seg_offs[wid] + (uint32_t)(__popcll(m0 & lt) + __popcll(m1 & lt));
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const uint8_t c = (uint8_t)(word >> (8 * k));
-    if (c & CF_CRITICAL) {
-      const int cx = seg * SEG + 2 * lane + k, cy = row;
-      if (idx < T.kcap) {
-        T.crit[idx] = pack_cell(T.ox + cx, T.oy + cy);
-        T.crit_fn[idx] = cell_fn(T, cx, cy);
-      }
-      ++idx;
+  const int64_t n = (int64_t)T.g.ch * T.nseg;
+  const int64_t first = ((int64_t)(b - prefix[t]) * 4 + (threadIdx.x >> 6)) * COMPACT_ITEMS;
+  const uint64_t lt = lanes_below(lane);
+  for (int it = 0; it < COMPACT_ITEMS; ++it) {
+    const int64_t item = first + it;
+    if (item >= n) break;
+    const uint64_t w0 = T.critbits[2 * item], w1 = T.critbits[2 * item + 1];
+    if (!(w0 | w1)) continue;
+    const int row = (int)(item / T.nseg), seg = (int)(item % T.nseg);
+    const uint32_t c0 = (uint32_t)(w0 >> lane) & 1u, c1 = (uint32_t)(w1 >> lane) & 1u;
+    uint32_t k = T.seg_offs[item] + (uint32_t)(__popcll(w0 & lt) + __popcll(w1 & lt));
+    const int cx = seg * SEG + 2 * lane;
+    if (c0 && k < T.kcap) {
+      T.crit[k] = pack_cell(T.ox + cx, T.oy + row);
+      T.crit_fn[k] = cell_fn(T, cx, row);
+    }
+    k += c0;
+    if (c1 && k < T.kcap) {
+      T.crit[k] = pack_cell(T.ox + cx + 1, T.oy + row);
+      T.crit_fn[k] = cell_fn(T, cx + 1, row);
     }
   }
 }
@@ -391,28 +521,16 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ ti
 // --------------------------------------------------------------- incidence
 // Per critical cell: for saddles, the critical index owning each facet and
 // each in-ext cofacet (writeout_connectivity, grid_dataset.cpp:1876-1898);
-// for boundary-cancellable cells, the critical index of the pair.
-
-__device__ uint32_t find_crit(const TileDesc &T, uint32_t K, int gx, int gy) {
-  // packed cellids (x | y << 16) sort numerically in raster (y, x) order
-  const uint32_t key = pack_cell(gx, gy);
-  uint32_t lo = 0, hi = K;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (T.crit[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return (lo < K && T.crit[lo] == key) ? lo : 0xFFFFFFFFu;
-}
+// for boundary-cancellable cells, the critical index of the pair (the pair
+// is critical too, so its index is its rank in the scanned flags).
 
 __device__ __forceinline__ uint32_t own_at(const TileDesc &T, int cx, int cy) {
   return T.own[(int64_t)cy * T.own_pitch + cx];
 }
 
-__device__ __forceinline__ uint32_t crit_of_owner(const TileDesc &T, uint32_t K, uint32_t o) {
-  return o == OWN_NONE ? 0xFFFFFFFFu : find_crit(T, K, (int)(o & 0xFFFF), (int)(o >> 16));
+__device__ __forceinline__ uint32_t rank_at(const TileDesc &T, int cx, int cy) {
+  const int64_t rs = (int64_t)cy * T.nseg + cx / SEG;
+  return crit_rank(T.seg_offs[rs], T.critbits[2 * rs], T.critbits[2 * rs + 1], cx);
 }
 
 __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
@@ -421,9 +539,8 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc &T = tiles[t];
   const TileGeom g = T.g;
-  const int lb = b - prefix[t];
   const uint32_t K = T.nkrit[0];
-  const uint32_t k = (uint32_t)lb * 256 + threadIdx.x;
+  const uint32_t k = (uint32_t)(b - prefix[t]) * 256 + threadIdx.x;
   if (k >= K || k >= T.kcap) return;
   const uint32_t c = T.crit[k];
   const int cx = (int)(c & 0xFFFF) - T.ox, cy = (int)(c >> 16) - T.oy;
@@ -432,43 +549,25 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
   uint4 conn = make_uint4(~0u, ~0u, ~0u, ~0u);
   if (dx != dy) {
     // facets: the two vertices; cofacets: the quads across the edge in ext
-    conn.x = crit_of_owner(T, K, own_at(T, cx + dx, cy + dy));
-    conn.y = crit_of_owner(T, K, own_at(T, cx - dx, cy - dy));
+    conn.x = own_at(T, cx + dx, cy + dy);
+    conn.y = own_at(T, cx - dx, cy - dy);
     const int qx0 = cx + dy, qy0 = cy + dx, qx1 = cx - dy, qy1 = cy - dx;
-    if (qx0 >= 0 && qy0 >= 0 && qx0 < g.cw && qy0 < g.ch) conn.z = crit_of_owner(T, K, own_at(T, qx0, qy0));
-    if (qx1 >= 0 && qy1 >= 0 && qx1 < g.cw && qy1 < g.ch) conn.w = crit_of_owner(T, K, own_at(T, qx1, qy1));
+    if (qx0 >= 0 && qy0 >= 0 && qx0 < g.cw && qy0 < g.ch) conn.z = own_at(T, qx0, qy0);
+    if (qx1 >= 0 && qy1 >= 0 && qx1 < g.cw && qy1 < g.ch) conn.w = own_at(T, qx1, qy1);
   }
   ((uint4 *)T.crit_conn)[k] = conn;
   uint32_t pidx = ~0u;
   if (code & CF_PAIRED) {
     const int d = code_dir(code);
-    pidx = find_crit(T, K, T.ox + cx + dir_dx(d), T.oy + cy + dir_dy(d));
+    pidx = rank_at(T, cx + dir_dx(d), cy + dir_dy(d));
   }
   T.crit_pair[k] = pidx;
 }
 
 // ------------------------------------------------------------ disc labels
-// For every interior vertex (quad) of a leaf: o = owner after redirects; if
-// o lies in the rect, one more redirect step o = owner(o); the label is the
-// critical-point index of surviving extremum o.
-
-__device__ __forceinline__ uint32_t table_find(const uint32_t *__restrict__ t, uint32_t n, uint32_t key) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (t[2 * mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return (lo < n && t[2 * lo] == key) ? t[2 * lo + 1] : 0xFFFFFFFFu;
-}
-
-__device__ __forceinline__ uint32_t owner_after(const DiscDesc &D, int x, int y) {
-  uint32_t r = D.nredir ? table_find(D.redir, D.nredir, pack_cell(x, y)) : 0xFFFFFFFFu;
-  if (r != 0xFFFFFFFFu) return r;
-  return D.own[(int64_t)(y - D.oy) * D.own_pitch + (x - D.ox)];
-}
+// For every interior vertex (quad) of a leaf: the label of its owner from
+// the host's per-critical-index tables (a critical cell looks up its own
+// entry, which applies its redirect first).
 
 __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict__ descs, const int *__restrict__ prefix,
                                                      int n) {
@@ -482,16 +581,21 @@ __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict_
   const bool vert = item < nv;
   const int64_t r = vert ? item : item - nv;
   const int w = vert ? nvx : nvx - 1;
-  const int x = D.rx0 + (vert ? 0 : 1) + 2 * (int)(r % w), y = D.ry0 + (vert ? 0 : 1) + 2 * (int)(r / w);
-  uint32_t o = owner_after(D, x, y);
-  if (o != 0xFFFFFFFFu) {
-    int ox = (int)(int16_t)(o & 0xFFFF), oy = (int)(int16_t)(o >> 16);
-    if (ox >= D.rx0 && ox <= D.rx1 && oy >= D.ry0 && oy <= D.ry1) o = owner_after(D, ox, oy);
-  }
+  const int lx = D.rx0 - D.ox + (vert ? 0 : 1) + 2 * (int)(r % w);
+  const int ly = D.ry0 - D.oy + (vert ? 0 : 1) + 2 * (int)(r / w);
+  const uint32_t o = D.own[(int64_t)ly * D.own_pitch + lx];
   uint32_t lab = 0xFFFFFFFFu;
-  if ((o & 0xFFFF) != 0xFFFF && (o >> 16) != 0xFFFF) {
-    lab = vert ? table_find(D.minmap, D.nmin, o) : table_find(D.maxmap, D.nmax, o);
-    if (lab == 0xFFFFFFFFu) atomicOr(D.err, 1u);
+  if (o != OWN_NONE) {
+    if (o >= D.nlab) {
+      atomicOr(D.err, 2u);
+    } else {
+      const uint8_t c = D.codes[(int64_t)ly * D.pitch + lx];
+      lab = (c & CF_CRITICAL) ? D.lab_self[o] : D.lab_owned[o];
+      if (lab == LAB_ERR) {
+        atomicOr(D.err, 1u);
+        lab = 0xFFFFFFFFu;
+      }
+    }
   }
   (vert ? D.lab_v : D.lab_q)[r] = lab;
 }
@@ -500,16 +604,19 @@ __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict_
 
 static int blocks_of(int kind, const TileDesc &T) {
   const TileGeom &g = T.g;
+  const int64_t nblk = (int64_t)T.nseg * ((g.vy + GRAD_TY - 1) / GRAD_TY);
   switch (kind) {
     case KB_GRADIENT:
-      return ((g.vx + GRAD_TX - 1) / GRAD_TX) * ((g.vy + GRAD_TY - 1) / GRAD_TY);
+      return (int)nblk;
     case KB_SCAN: {
       int64_t n = (int64_t)g.ch * T.nseg;
       return (int)((n + SCAN_CHUNK - 1) / SCAN_CHUNK);
     }
-    case KB_ROWSEG: {
+    case KB_RING:
+      return (int)((nblk * RING_W + 255) / 256);
+    case KB_COMPACT: {
       int64_t n = (int64_t)g.ch * T.nseg;
-      return (int)((n + 3) / 4);
+      return (int)((n + 4 * COMPACT_ITEMS - 1) / (4 * COMPACT_ITEMS));
     }
     case KB_CRIT:
       return (int)((T.kcap + 255) / 256);
@@ -542,13 +649,16 @@ int plan_disc_blocks(const DiscDesc *d, int n, int *prefix) {
 void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_disc_labels, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
 }
-
-void launch_grad_trace(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
-  if (nblocks) hipLaunchKernelGGL(k_grad_trace, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
+void launch_grad_local(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                       hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_grad_local, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles, d_err);
 }
-void launch_resolve_exits(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
-                          hipStream_t s) {
-  if (nblocks) hipLaunchKernelGGL(k_resolve_exits, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles, d_err);
+void launch_ring_resolve(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                         hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_ring_resolve, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles, d_err);
+}
+void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
+  if (nblocks) hipLaunchKernelGGL(k_owner, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
 }
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (!nblocks) return;

@@ -7,16 +7,21 @@
 
 namespace mscx {
 
-enum BlockKind { KB_GRADIENT = 0, KB_SCAN = 1, KB_ROWSEG = 2, KB_CRIT = 3, KB_COUNT = 4 };
+enum BlockKind { KB_GRADIENT = 0, KB_SCAN = 1, KB_RING = 2, KB_COMPACT = 3, KB_CRIT = 4, KB_COUNT = 5 };
 
 // prefix[0..ntiles] = first block of each tile for a launch of `kind`;
 // returns the total block count.
 int plan_blocks(int kind, const TileDesc *tiles, int ntiles, int *prefix);
 
-void launch_grad_trace(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
-void launch_resolve_exits(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
-                          hipStream_t s);
+// Error flags set by the kernels in *d_err: 2 = a block-local trace did not
+// end, 4 = a ring chain did not end, 8 = a path left the ext window (all
+// mean an invalid gradient).
+void launch_grad_local(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                       hipStream_t s);
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
+void launch_ring_resolve(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
+                         hipStream_t s);
+void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_compact(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 

@@ -37,11 +37,15 @@ static void alloc_fixed(mscx_tile *t) {
   g.ry1 = t->rect[3] - t->ext[1];
   d.ox = t->ext[0];
   d.oy = t->ext[1];
+  // one collation segment per block tile column: cw = 2 vx - 1 is odd, so
+  // ceil(cw / SEG) == ceil(vx / GRAD_TX)
   d.nseg = (g.cw + SEG - 1) / SEG;
   d.pitch = d.nseg * SEG;
   d.own_pitch = d.pitch;
+  d.loc_pitch = d.nseg * GRAD_TX;
   const int64_t nrs = (int64_t)g.ch * d.nseg;
   const int64_t nparts = (nrs + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  const int64_t nblk = (int64_t)d.nseg * ((g.vy + GRAD_TY - 1) / GRAD_TY);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -50,10 +54,10 @@ static void alloc_fixed(mscx_tile *t) {
   };
   size_t o_codes = take((size_t)g.ch * d.pitch);
   size_t o_cnt = take(nrs * 4), o_offs = take(nrs * 4), o_part = take(nparts * 4 + 4);
-  const int64_t nblk = (int64_t)((g.vx + GRAD_TX - 1) / GRAD_TX) * ((g.vy + GRAD_TY - 1) / GRAD_TY);
+  size_t o_bits = take(nrs * 16);
   size_t o_own = take((size_t)g.ch * d.own_pitch * 4);
-  size_t o_exit = take((size_t)nblk * 4 * GRAD_TX * GRAD_TY * 4);
-  size_t o_nexit = take((size_t)nblk * 4);
+  size_t o_loc = take((size_t)g.vy * d.loc_pitch * 4);
+  size_t o_ring = take((size_t)nblk * RING_W * 4), o_ringidx = take((size_t)nblk * RING_W * 4);
   t->fixed_bytes = off;
   t->fixed = device_pool().alloc(off);
   char *base = (char *)t->fixed;
@@ -61,9 +65,11 @@ static void alloc_fixed(mscx_tile *t) {
   d.seg_counts = (uint32_t *)(base + o_cnt);
   d.seg_offs = (uint32_t *)(base + o_offs);
   d.scan_part = (uint32_t *)(base + o_part);
+  d.critbits = (uint64_t *)(base + o_bits);
   d.own = (uint32_t *)(base + o_own);
-  d.exits = (uint32_t *)(base + o_exit);
-  d.nexit = (uint32_t *)(base + o_nexit);
+  d.loc = (uint32_t *)(base + o_loc);
+  d.ring = (uint32_t *)(base + o_ring);
+  d.ringidx = (uint32_t *)(base + o_ringidx);
 }
 
 CritLayout::CritLayout(size_t K) {
@@ -111,21 +117,26 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   for (int k = 0; k < KB_CRIT; ++k) nb[k] = plan_blocks(k, hd, n, hp + k * (n + 1));
   MSCX_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
 
+  // gradient + block-local trace, scan of the critical counts, ring
+  // resolution and owner images need no host decision in between
   StageTimer &tm = stage_timer();
   tm.begin(ST_GRADIENT, s);
-  launch_grad_trace(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
+  launch_grad_local(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], derr, s);
   tm.end(ST_GRADIENT, s);
   tm.begin(ST_SCAN, s);
   launch_scan(dd, dp + KB_SCAN * (n + 1), n, nb[KB_SCAN], s);
   tm.end(ST_SCAN, s);
   tm.begin(ST_TRACE, s);
-  launch_resolve_exits(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], derr, s);
+  launch_ring_resolve(dd, dp + KB_RING * (n + 1), n, nb[KB_RING], derr, s);
   tm.end(ST_TRACE, s);
+  tm.begin(ST_OWNER, s);
+  launch_owner(dd, dp + KB_GRADIENT * (n + 1), n, nb[KB_GRADIENT], s);
+  tm.end(ST_OWNER, s);
   MSCX_HIP(hipGetLastError());
   MSCX_HIP(hipMemcpyAsync(hk, dk, o_err + 4 - o_k, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipStreamSynchronize(s));
   tm.collect();
-  MSCX_REQUIRE(*herr == 0, MSCX_ELOGIC, "owner trace did not resolve");
+  MSCX_REQUIRE(*herr == 0, MSCX_ELOGIC, "manifold trace did not resolve (invalid gradient)");
 
   // one block for the critical data of every tile of this launch
   size_t block_bytes = 0;
@@ -157,7 +168,7 @@ void run_tiles(mscx_tile *const *tiles, int n) {
   MSCX_HIP(hipMemcpyAsync(dev, host, o_k, hipMemcpyHostToDevice, s));
 
   tm.begin(ST_COMPACT, s);
-  launch_compact(dd, dp + KB_ROWSEG * (n + 1), n, nb[KB_ROWSEG], s);
+  launch_compact(dd, dp + KB_COMPACT * (n + 1), n, nb[KB_COMPACT], s);
   tm.end(ST_COMPACT, s);
   tm.begin(ST_INCIDENCE, s);
   launch_incidence(dd, dp + KB_CRIT * (n + 1), n, nb[KB_CRIT], s);
@@ -195,13 +206,24 @@ void tile_codes(const mscx_tile *t, std::vector<uint8_t> &codes) {
   MSCX_HIP(hipStreamSynchronize(runtime().stream));
 }
 
-void tile_owners(const mscx_tile *t, std::vector<uint32_t> &own) {
+void tile_owner_index(const mscx_tile *t, std::vector<uint32_t> &own) {
   MSCX_REQUIRE(t->worked, MSCX_ESTATE, "tile has not been worked");
   const TileGeom &g = t->d.g;
   own.resize((size_t)g.cw * g.ch);
   MSCX_HIP(hipMemcpy2DAsync(own.data(), (size_t)g.cw * 4, t->d.own, (size_t)t->d.own_pitch * 4, (size_t)g.cw * 4,
                             g.ch, hipMemcpyDeviceToHost, runtime().stream));
   MSCX_HIP(hipStreamSynchronize(runtime().stream));
+}
+
+void tile_owners(const mscx_tile *t, std::vector<uint32_t> &own) {
+  tile_owner_index(t, own);
+  std::vector<uint32_t> crit(t->K);
+  if (t->K) MSCX_HIP(hipMemcpy(crit.data(), t->d.crit, (size_t)t->K * 4, hipMemcpyDeviceToHost));
+  for (uint32_t &o : own) {
+    if (o == OWN_NONE) continue;
+    MSCX_REQUIRE(o < t->K, MSCX_ELOGIC, "owner index out of range");
+    o = crit[o];
+  }
 }
 
 }  // namespace mscx

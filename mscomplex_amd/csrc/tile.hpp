@@ -42,6 +42,9 @@ void tile_critical(const mscx_tile *t, std::vector<uint32_t> &crit, std::vector<
                    std::vector<uint32_t> &conn4, std::vector<uint32_t> &pair);
 // Host copies of a tile's codes and owners over ext, x fastest (unpitched).
 void tile_codes(const mscx_tile *t, std::vector<uint8_t> &codes);
+// Owners as packed cell ids (the reference's m_cell_own) or as indices into
+// the tile's critical list (the device representation).
 void tile_owners(const mscx_tile *t, std::vector<uint32_t> &own);
+void tile_owner_index(const mscx_tile *t, std::vector<uint32_t> &own);
 
 }  // namespace mscx

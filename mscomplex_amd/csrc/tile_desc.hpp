@@ -14,63 +14,119 @@ constexpr int GRAD_TY = 32;       // vertex rows per block tile
 constexpr int SEG = 2 * GRAD_TX;  // cells per collation segment: one block tile row
 constexpr int SCAN_CHUNK = 2048;  // counts per scan block
 
+// Perimeter ("ring") of a block tile: the vertex positions (and, separately,
+// the quad positions) with li in {0, TX-1} or lj in {0, TY-1}.  A gradient
+// path that leaves a block enters its neighbour on that neighbour's ring.
+constexpr int RING_N = 2 * GRAD_TX + 2 * GRAD_TY - 4;  // positions per forest
+constexpr int RING_Q = 192;                            // quad ring offset in a block's ring words
+constexpr int RING_W = 2 * RING_Q;                     // ring words per block
+static_assert(RING_N <= RING_Q, "ring does not fit");
+
+MSCX_HD int ring_pos(int li, int lj) {
+  if (lj == 0) return li;
+  if (lj == GRAD_TY - 1) return GRAD_TX + li;
+  if (li == 0) return 2 * GRAD_TX + lj - 1;
+  return 2 * GRAD_TX + GRAD_TY - 2 + lj - 1;  // li == TX - 1
+}
+MSCX_HD void ring_at(int p, int &li, int &lj) {
+  if (p < GRAD_TX) {
+    li = p;
+    lj = 0;
+  } else if (p < 2 * GRAD_TX) {
+    li = p - GRAD_TX;
+    lj = GRAD_TY - 1;
+  } else if (p < 2 * GRAD_TX + GRAD_TY - 2) {
+    li = 0;
+    lj = p - 2 * GRAD_TX + 1;
+  } else {
+    li = GRAD_TX - 1;
+    lj = p - (2 * GRAD_TX + GRAD_TY - 2) + 1;
+  }
+}
+
+// Local terminal of a vertex or quad entry after the block-local trace
+// (16 bits; in LDS an entry with bit 15 clear is a link to another entry):
+//   0x8000 | e          root: the critical entry e (li + lj*TX) of the same forest
+//   0xA000 | s<<6 | p   exit: the path leaves through side s (0 +x, 1 -x,
+//                       2 +y, 3 -y) at position p along that side
+//   0xC000              none: the cell has no owner
+constexpr uint32_t T_ROOT = 0x8000u, T_EXIT = 0xA000u, T_NONE = 0xC000u;
+MSCX_HD uint32_t term_kind(uint32_t t) { return (t >> 13) & 3u; }  // 0 root, 1 exit, 2 none
+
 struct TileDesc {
   // geometry (local coordinates relative to the ext rect's bottom-left cell)
   TileGeom g;
   int ox, oy;  // global cell coordinates of the ext rect's bottom-left
-  int nseg;    // collation segments per cell row (pitch / SEG)
+  int nseg;    // collation segments (= block tiles) per cell row (pitch / SEG)
   int pitch;   // cells per code / owner row (nseg * SEG)
   // input: ext window of vertex values, x fastest, row pitch in floats
   const float *verts;
   int64_t vpitch;
   // gradient codes, ch rows of `pitch` bytes
   uint8_t *codes;
-  // collation: per (cell row, segment) critical counts and their exclusive scan
+  // collation: per (cell row, segment) critical counts, their exclusive
+  // scan, and the critical flags as two 64-bit words (cells 2l and 2l+1 of
+  // the segment are bit l of word 0 and word 1)
   uint32_t *seg_counts;  // ch * nseg
   uint32_t *seg_offs;    // ch * nseg
+  uint64_t *critbits;    // ch * nseg * 2
   uint32_t *scan_part;   // per scan block partial sums
   uint32_t *nkrit;       // [0] = number of critical cells K
+  // local terminals: per vertex position, vertex terminal | quad terminal << 16
+  uint32_t *loc;
+  int loc_pitch;  // words per vertex row (nseg * GRAD_TX)
+  // ring words per block (RING_W each): after the gradient kernel an
+  // ext-local packed root cell, OWN_NONE or RING_MARK | ring index of the
+  // entry the path continues at; ringidx holds the resolved critical index
+  uint32_t *ring;
+  uint32_t *ringidx;
   // critical list (raster order), their function values, saddle incidence
   // (facet owners then cofacet owners, critical indices or ~0) and pair index
   uint32_t *crit;
   float *crit_fn;
   uint32_t *crit_conn;  // 4 per critical cell
   uint32_t *crit_pair;  // critical index of the pair (boundary-cancellable) or ~0
-  // owners: packed cellid per cell (x | y << 16), ch rows of own_pitch
-  // words; between the trace kernels an entry may hold OWN_MARK | packed id
-  // of a cell whose owner it takes (a path that left its block tile)
+  // owners: critical index of each cell's owner (the reference's m_cell_own
+  // as an index into the critical list), OWN_NONE when unreached; ch rows of
+  // own_pitch words
   uint32_t *own;
   int own_pitch;
-  // owner entries that hold a marker after the block-local trace: block b of
-  // the gradient launch lists its entries in exits[b * 4*GRAD_TX*GRAD_TY ...]
-  // and their count in nexit[b]
-  uint32_t *exits;
-  uint32_t *nexit;
   uint32_t kcap;  // capacity of the crit arrays
 };
 
-constexpr uint32_t OWN_NONE = 0xFFFFFFFFu;  // (-1,-1)
-constexpr uint32_t OWN_MARK = 0x80000000u;  // cell ids never have bit 31 set (y <= 32766)
-MSCX_HD bool own_is_mark(uint32_t w) { return (w & OWN_MARK) && w != OWN_NONE; }
+constexpr uint32_t OWN_NONE = 0xFFFFFFFFu;   // (-1,-1) / no owner
+constexpr uint32_t RING_MARK = 0x80000000u;  // ext-local cell ids never have bit 31 set
+
+// Critical index of ext-local cell (x, y), known to be critical: the scanned
+// offset of its (row, segment) plus the critical cells before it there.
+MSCX_HD uint32_t crit_rank(uint32_t base, uint64_t w0, uint64_t w1, int x) {
+  const int l = (x & (SEG - 1)) >> 1;
+  const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+  uint32_t r = (uint32_t)(__builtin_popcountll(w0 & below) + __builtin_popcountll(w1 & below));
+  if (x & 1) r += (uint32_t)((w0 >> l) & 1u);
+  return base + r;
+}
 
 // Per-leaf inputs of the manifold label pass (postMergeFillDiscs,
-// grid_dataset.cpp:1142-1187): owner image, owner redirects of cancelled
-// pairs, and the surviving extremum maps; tables are (cell, value) pairs
-// sorted by packed cell id.
+// grid_dataset.cpp:1142-1187): the owner index image and codes, and per
+// critical index the label of a cell it owns (lab_owned: the cell is not
+// critical) or of the critical cell itself (lab_self), as computed on the
+// host from the owner redirects and the surviving extremum maps.
 struct DiscDesc {
   const uint32_t *own;
   int64_t own_pitch;
+  const uint8_t *codes;
+  int64_t pitch;
   int ox, oy;              // ext bottom-left, global cell coords
   int rx0, ry0, rx1, ry1;  // interior rect, global cell coords
-  const uint32_t *redir;
-  uint32_t nredir;
-  const uint32_t *minmap;
-  uint32_t nmin;
-  const uint32_t *maxmap;
-  uint32_t nmax;
+  const uint32_t *lab_owned;
+  const uint32_t *lab_self;
+  uint32_t nlab;
   uint32_t *lab_v;  // per interior vertex, raster order: critical-point index or ~0
   uint32_t *lab_q;  // per interior quad
   uint32_t *err;    // set when an owner is not a surviving extremum
 };
+
+constexpr uint32_t LAB_ERR = 0xFFFFFFFEu;  // label table entry: owner is no surviving extremum
 
 }  // namespace mscx

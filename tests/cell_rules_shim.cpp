@@ -25,20 +25,25 @@ extern "C" int cr_codes(const float *win, const int *rect, const int *ext, uint8
   } catch (int) {
     return -1;
   }
-  // the decomposed form must agree with the per-cell form everywhere
+  // the decomposed form the kernels run, with the split (value, x, y) keys
+  // they use, must agree with the per-cell form on packed 64-bit keys
+  auto KS = [&](int i, int j) -> mscx::VKey {
+    if (i < 0 || j < 0 || i >= g.vx || j >= g.vy) throw 1;
+    return mscx::VKey{mscx::fkey(win[(size_t)j * g.vx + i]), i, j};
+  };
   auto VC = [&](int i, int j) {
-    return mscx::in_rect(g, 2 * i, 2 * j) ? mscx::vertex_cand(K, g, i, j) : mscx::DIR_NONE;
+    return mscx::in_rect(g, 2 * i, 2 * j) ? mscx::vertex_cand(KS, g, i, j) : mscx::DIR_NONE;
   };
   auto HC = [&](int i, int j) {
-    return mscx::in_rect(g, 2 * i + 1, 2 * j) ? mscx::hedge_cand(K, g, i, j) : mscx::DIR_NONE;
+    return mscx::in_rect(g, 2 * i + 1, 2 * j) ? mscx::hedge_cand(KS, g, i, j) : mscx::DIR_NONE;
   };
   auto EC = [&](int i, int j) {
-    return mscx::in_rect(g, 2 * i, 2 * j + 1) ? mscx::vedge_cand(K, g, i, j) : mscx::DIR_NONE;
+    return mscx::in_rect(g, 2 * i, 2 * j + 1) ? mscx::vedge_cand(KS, g, i, j) : mscx::DIR_NONE;
   };
   try {
     for (int y = 0; y < g.ch; ++y)
       for (int x = 0; x < g.cw; ++x)
-        if (mscx::code_from_cands(K, VC, HC, EC, g, x, y) != codes[(size_t)y * g.cw + x]) return -2;
+        if (mscx::code_from_cands(KS, VC, HC, EC, g, x, y) != codes[(size_t)y * g.cw + x]) return -2;
   } catch (int) {
     return -3;
   }
