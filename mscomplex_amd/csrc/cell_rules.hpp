@@ -86,13 +86,23 @@ MSCX_HD bool operator>(const VKey &a, const VKey &b) { return b < a; }
 MSCX_HD bool operator==(const VKey &a, const VKey &b) { return a.f == b.f && a.i == b.i && a.j == b.j; }
 MSCX_HD bool operator!=(const VKey &a, const VKey &b) { return !(a == b); }
 
+// c ? a : b on keys.  For the split key it selects field by field: a
+// select between two structs would otherwise be lowered through memory
+// (a select of their addresses), which on the GPU means scratch.
+template <class T>
+MSCX_HD T ksel(bool c, const T &a, const T &b) {
+  return c ? a : b;
+}
+MSCX_HD VKey ksel(bool c, const VKey &a, const VKey &b) {
+  return VKey{c ? a.f : b.f, c ? a.i : b.i, c ? a.j : b.j};
+}
+
 template <class T>
 MSCX_HD void cswap(T &x, T &y) {
-  if (y < x) {
-    T t = x;
-    x = y;
-    y = t;
-  }
+  const bool s = y < x;
+  const T lo = ksel(s, y, x), hi = ksel(s, x, y);
+  x = lo;
+  y = hi;
 }
 template <class T>
 MSCX_HD void sort4(T &a, T &b, T &c, T &d) {
@@ -118,32 +128,34 @@ MSCX_HD bool quad_lt(T p0, T p1, T p2, T p3, T q0, T q1, T q2, T q3) {
 // cand(v): the lowest usable edge of vertex (i, j) (lowestPairableCoFacet
 // for a dim-0 cell).  An edge is usable when v is its larger vertex and,
 // for v on the ext boundary, the edge also lies on it.  Returns DIR_*.
-template <class KF>
+// The minimum is taken as a tournament over the neighbours in position
+// order NX < NY < PY < PX, so a kernel with split keys compares each pair
+// with the position tie-break already known.  INTERIOR: the caller knows
+// (i, j) is not on the ext boundary, which removes every boundary test.
+template <bool INTERIOR = false, class KF>
 MSCX_HD int vertex_cand(const KF &K, const TileGeom &g, int i, int j) {
   const auto kv = K(i, j);
-  bool tb = (i == 0 || j == 0 || i == g.vx - 1 || j == g.vy - 1);
-  int best = DIR_NONE;
-  auto bk = kv;  // meaningful once best != DIR_NONE
+  const bool tb = !INTERIOR && (i == 0 || j == 0 || i == g.vx - 1 || j == g.vy - 1);
   // +x / -x: horizontal edge on the boundary iff its row is a boundary row
-  bool hrow_tb = (j == 0 || j == g.vy - 1);
-  bool vcol_tb = (i == 0 || i == g.vx - 1);
-  if (i + 1 < g.vx && (!tb || hrow_tb)) {
-    const auto k = K(i + 1, j);
-    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_PX; }
-  }
-  if (i - 1 >= 0 && (!tb || hrow_tb)) {
-    const auto k = K(i - 1, j);
-    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_NX; }
-  }
-  if (j + 1 < g.vy && (!tb || vcol_tb)) {
-    const auto k = K(i, j + 1);
-    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_PY; }
-  }
-  if (j - 1 >= 0 && (!tb || vcol_tb)) {
-    const auto k = K(i, j - 1);
-    if (k < kv && (best == DIR_NONE || k < bk)) { bk = k; best = DIR_NY; }
-  }
-  return best;
+  const bool hx = !tb || (j == 0 || j == g.vy - 1);
+  const bool vy = !tb || (i == 0 || i == g.vx - 1);
+  const bool inx = INTERIOR || i - 1 >= 0, iny = INTERIOR || j - 1 >= 0;
+  const bool ipy = INTERIOR || j + 1 < g.vy, ipx = INTERIOR || i + 1 < g.vx;
+  // K is called at fixed offsets only (a neighbour outside the ext may be
+  // read but its key is discarded)
+  const auto knx = ksel(inx, K(i - 1, j), kv), kny = ksel(iny, K(i, j - 1), kv);
+  const auto kpy = ksel(ipy, K(i, j + 1), kv), kpx = ksel(ipx, K(i + 1, j), kv);
+  const bool unx = inx && hx && knx < kv, uny = iny && vy && kny < kv;
+  const bool upy = ipy && vy && kpy < kv, upx = ipx && hx && kpx < kv;
+  int m1 = unx ? DIR_NX : (uny ? DIR_NY : DIR_NONE);
+  if (unx && uny && kny < knx) m1 = DIR_NY;
+  int m2 = upy ? DIR_PY : (upx ? DIR_PX : DIR_NONE);
+  if (upy && upx && kpx < kpy) m2 = DIR_PX;
+  if (m1 == DIR_NONE) return m2;
+  if (m2 == DIR_NONE) return m1;
+  const auto k1 = ksel(m1 == DIR_NY, kny, knx);
+  const auto k2 = ksel(m2 == DIR_PX, kpx, kpy);
+  return (k2 < k1) ? m2 : m1;
 }
 
 // cand(e) for the horizontal edge (2i+1, 2j) between vertices (i,j) and
@@ -151,15 +163,15 @@ MSCX_HD int vertex_cand(const KF &K, const TileGeom &g, int i, int j) {
 // is usable when e is its largest edge, i.e. e joins the quad's maximum
 // vertex to the larger of that vertex's two quad neighbours.  Edges on the
 // ext boundary have no usable quad (quads are never on it).
-template <class KF>
+template <bool INTERIOR = false, class KF>
 MSCX_HD int hedge_cand(const KF &K, const TileGeom &g, int i, int j) {
-  if (j == 0 || j == g.vy - 1) return DIR_NONE;
+  if (!INTERIOR && (j == 0 || j == g.vy - 1)) return DIR_NONE;
   const auto a = K(i, j), b = K(i + 1, j);
   bool a_hi = a > b;
-  const auto hi = a_hi ? a : b, lo = a_hi ? b : a;
-  int hi_i = a_hi ? i : i + 1, lo_i = a_hi ? i + 1 : i;
-  const auto u_hi = K(hi_i, j + 1), u_lo = K(lo_i, j + 1);
-  const auto d_hi = K(hi_i, j - 1), d_lo = K(lo_i, j - 1);
+  const auto hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
+  const auto ua = K(i, j + 1), ub = K(i + 1, j + 1), da = K(i, j - 1), db = K(i + 1, j - 1);
+  const auto u_hi = ksel(a_hi, ua, ub), u_lo = ksel(a_hi, ub, ua);
+  const auto d_hi = ksel(a_hi, da, db), d_lo = ksel(a_hi, db, da);
   bool up = (u_hi < lo) && (u_lo < hi);
   bool dn = (d_hi < lo) && (d_lo < hi);
   if (up && dn) return quad_lt(lo, hi, u_hi, u_lo, lo, hi, d_hi, d_lo) ? DIR_PY : DIR_NY;
@@ -168,15 +180,15 @@ MSCX_HD int hedge_cand(const KF &K, const TileGeom &g, int i, int j) {
 
 // cand(e) for the vertical edge (2i, 2j+1) between (i,j) and (i,j+1); its
 // cofacets are the quads right (+x) and left (-x).
-template <class KF>
+template <bool INTERIOR = false, class KF>
 MSCX_HD int vedge_cand(const KF &K, const TileGeom &g, int i, int j) {
-  if (i == 0 || i == g.vx - 1) return DIR_NONE;
+  if (!INTERIOR && (i == 0 || i == g.vx - 1)) return DIR_NONE;
   const auto a = K(i, j), b = K(i, j + 1);
   bool a_hi = a > b;
-  const auto hi = a_hi ? a : b, lo = a_hi ? b : a;
-  int hi_j = a_hi ? j : j + 1, lo_j = a_hi ? j + 1 : j;
-  const auto r_hi = K(i + 1, hi_j), r_lo = K(i + 1, lo_j);
-  const auto l_hi = K(i - 1, hi_j), l_lo = K(i - 1, lo_j);
+  const auto hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
+  const auto ra = K(i + 1, j), rb = K(i + 1, j + 1), la = K(i - 1, j), lb = K(i - 1, j + 1);
+  const auto r_hi = ksel(a_hi, ra, rb), r_lo = ksel(a_hi, rb, ra);
+  const auto l_hi = ksel(a_hi, la, lb), l_lo = ksel(a_hi, lb, la);
   bool rt = (r_hi < lo) && (r_lo < hi);
   bool lf = (l_hi < lo) && (l_lo < hi);
   if (rt && lf) return quad_lt(lo, hi, r_hi, r_lo, lo, hi, l_hi, l_lo) ? DIR_PX : DIR_NX;
@@ -250,9 +262,11 @@ MSCX_HD uint8_t cell_code(const KF &K, const TileGeom &g, int cx, int cy) {
 // VC(i, j) = vertex_cand of vertex (i, j), HC(i, j) = hedge_cand of edge
 // (2i+1, 2j), EC(i, j) = vedge_cand of edge (2i, 2j+1), each DIR_NONE for a
 // cell outside the interior rect (exterior cells never pick a pair).  The
-// HIP kernel stores these in LDS; cell_code above is the per-cell form used
-// to check this one.
-template <class KF, class VCF, class HCF, class ECF>
+// HIP kernel keeps these in registers; cell_code above is the per-cell form
+// used to check this one.  INTERIOR: the caller knows the cell and its
+// neighbours lie strictly inside the interior rect (no exterior cells, no
+// boundary-cancellable sites).
+template <bool INTERIOR = false, class KF, class VCF, class HCF, class ECF>
 MSCX_HD uint8_t code_from_cands(const KF &K, const VCF &VC, const HCF &HC, const ECF &EC, const TileGeom &g,
                                 int cx, int cy) {
   const int dx = cx & 1, dy = cy & 1;
@@ -279,25 +293,262 @@ MSCX_HD uint8_t code_from_cands(const KF &K, const VCF &VC, const HCF &HC, const
     else if (k10 > k01 && k10 > k11) edir = (k00 > k11) ? DIR_NY : DIR_PX;
     else if (k01 > k11) edir = (k00 > k11) ? DIR_NX : DIR_PY;
     else edir = (k10 > k01) ? DIR_PX : DIR_PY;
-    int ec;
-    switch (edir) {
-      case DIR_NY: ec = HC(i, j); break;
-      case DIR_PY: ec = HC(i, j + 1); break;
-      case DIR_NX: ec = EC(i, j); break;
-      default: ec = EC(i + 1, j); break;
-    }
+    // candidates of the four edges, selected by value (a switch over the
+    // accessor calls would select between the addresses of their sources)
+    const int hc0 = HC(i, j), hc1 = HC(i, j + 1), ec0 = EC(i, j), ec1 = EC(i + 1, j);
+    const int ec = edir == DIR_NY ? hc0 : edir == DIR_PY ? hc1 : edir == DIR_NX ? ec0 : ec1;
     if (ec == dir_flip(edir)) dir = edir;
   }
-  const bool inr = in_rect(g, cx, cy);
+  const bool inr = INTERIOR || in_rect(g, cx, cy);
   uint8_t flags;
   if (dir != DIR_NONE) {
     flags = CF_PAIRED;
     const int px = cx + dir_dx(dir), py = cy + dir_dy(dir);
-    if (bndry_cancellable_site(g, cx, cy) || bndry_cancellable_site(g, px, py)) flags |= CF_CRITICAL;
+    if (!INTERIOR && (bndry_cancellable_site(g, cx, cy) || bndry_cancellable_site(g, px, py))) flags |= CF_CRITICAL;
   } else {
     flags = inr ? CF_CRITICAL : 0;
   }
   return (uint8_t)(flags | (dir << 2));
+}
+
+}  // namespace mscx
+
+namespace mscx {
+
+// ---------------------------------------------------------------------------
+// Row-step form of the rules used by the gradient kernel.  A lane walks one
+// vertex column i upward; for the current row j it holds the 4x4 window of
+// value keys wRC = fkey of vertex (i - 1 + C, j - 1 + R) and the candidates
+// of vertices (i, j), (i+1, j) and of the edge (i, j)-(i+1, j) carried from
+// the previous row.  Every comparison below is between two vertices whose
+// relative position is fixed, so the reference's position tie-break
+// (grid_dataset.h:158-167: value, then x, then y) is a choice between <= and
+// < on the 32-bit keys.  Each function equals the corresponding generic rule
+// above (checked exhaustively on the CPU against cell_code).
+
+constexpr uint32_t KEY_INF = 0xFFFFFFFFu;  // above every fkey() value (NaN-free input)
+constexpr int CAND_BOTH = 7;               // both cofacets usable: decided by quad_lt
+
+struct KeyWin {
+  uint32_t w00, w01, w02, w03, w10, w11, w12, w13, w20, w21, w22, w23, w30, w31, w32, w33;
+  int i, j;
+  // generic accessor at fixed offsets (ii - i, jj - j in -1..2)
+  MSCX_HD VKey operator()(int ii, int jj) const {
+    const int c = ii - i + 1, r = jj - j + 1;
+    const uint32_t f0 = r == 0 ? w00 : r == 1 ? w10 : r == 2 ? w20 : w30;
+    const uint32_t f1 = r == 0 ? w01 : r == 1 ? w11 : r == 2 ? w21 : w31;
+    const uint32_t f2 = r == 0 ? w02 : r == 1 ? w12 : r == 2 ? w22 : w32;
+    const uint32_t f3 = r == 0 ? w03 : r == 1 ? w13 : r == 2 ? w23 : w33;
+    return VKey{c == 0 ? f0 : c == 1 ? f1 : c == 2 ? f2 : f3, ii, jj};
+  }
+  // next row: rows move down by one, n0..n3 = keys of (i-1..i+2, j+3)
+  MSCX_HD void slide(uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3) {
+    w00 = w10, w01 = w11, w02 = w12, w03 = w13;
+    w10 = w20, w11 = w21, w12 = w22, w13 = w23;
+    w20 = w30, w21 = w31, w22 = w32, w23 = w33;
+    w30 = n0, w31 = n1, w32 = n2, w33 = n3;
+    ++j;
+  }
+};
+
+// vertex_cand from the vertex key and its four neighbours (NX, NY before the
+// vertex, PY, PX after it) and the usability of each direction.
+MSCX_HD int vcand_keys(uint32_t fv, uint32_t fnx, uint32_t fny, uint32_t fpy, uint32_t fpx, bool anx, bool any,
+                       bool apy, bool apx) {
+  const uint32_t enx = (anx && fnx <= fv) ? fnx : KEY_INF;
+  const uint32_t eny = (any && fny <= fv) ? fny : KEY_INF;
+  const uint32_t epy = (apy && fpy < fv) ? fpy : KEY_INF;
+  const uint32_t epx = (apx && fpx < fv) ? fpx : KEY_INF;
+  const bool l1 = enx <= eny, l2 = epy <= epx;
+  const uint32_t e1 = l1 ? enx : eny, e2 = l2 ? epy : epx;
+  const bool l = e1 <= e2;
+  const uint32_t e = l ? e1 : e2;
+  return e == KEY_INF ? DIR_NONE : (l ? (l1 ? DIR_NX : DIR_NY) : (l2 ? DIR_PY : DIR_PX));
+}
+// hedge_cand of the edge A=(x,y)-B=(x+1,y): Au, Bu the row above, Ad, Bd below.
+MSCX_HD int hcand_keys(uint32_t fA, uint32_t fB, uint32_t fAu, uint32_t fBu, uint32_t fAd, uint32_t fBd) {
+  const bool up = fAu <= fB && fBu < fA;
+  const bool dn = fAd <= fB && fBd < fA;
+  return up ? (dn ? CAND_BOTH : DIR_PY) : (dn ? DIR_NY : DIR_NONE);
+}
+// vedge_cand of the edge A=(x,y)-B=(x,y+1): Ra, Rb the column right, La, Lb left.
+MSCX_HD int ecand_keys(uint32_t fA, uint32_t fB, uint32_t fRa, uint32_t fRb, uint32_t fLa, uint32_t fLb) {
+  const bool rt = fRa < fB && fRb < fA;
+  const bool lf = fLa <= fB && fLb <= fA;
+  return rt ? (lf ? CAND_BOTH : DIR_PX) : (lf ? DIR_NX : DIR_NONE);
+}
+
+// Boundary classes of vertex column x / row y (INTERIOR: none applies).
+struct VClass {
+  bool in;     // x < vx
+  bool lo;     // x > 0
+  bool hi;     // x < vx - 1
+  bool rv;     // vertex coordinate 2x inside the rect
+  bool re;     // edge coordinate 2x+1 inside the rect
+  bool rm;     // edge coordinate 2x-1 inside the rect
+  bool sv;     // 2x is a boundary-cancellable side of the rect
+  bool se;     // 2x+1 likewise
+  bool sm;     // 2x-1 likewise
+};
+MSCX_HD VClass vclass(int x, int n, int r0, int r1, int cmax) {
+  VClass c;
+  c.in = x < n;
+  c.lo = x > 0;
+  c.hi = x < n - 1;
+  c.rv = 2 * x >= r0 && 2 * x <= r1;
+  c.re = 2 * x + 1 >= r0 && 2 * x + 1 <= r1;
+  c.rm = 2 * x - 1 >= r0 && 2 * x - 1 <= r1;
+  auto side = [&](int v) { return (v == r0 && r0 > 0) || (v == r1 && r1 < cmax); };
+  c.sv = side(2 * x);
+  c.se = side(2 * x + 1);
+  c.sm = side(2 * x - 1);
+  return c;
+}
+
+template <bool INTERIOR>
+MSCX_HD int vcand_at(uint32_t fv, uint32_t fnx, uint32_t fny, uint32_t fpy, uint32_t fpx, const VClass &cx,
+                     const VClass &cy) {
+  if (INTERIOR) return vcand_keys(fv, fnx, fny, fpy, fpx, true, true, true, true);
+  if (!(cx.in && cy.in && cx.rv && cy.rv)) return DIR_NONE;
+  const bool colB = !cx.lo || !cx.hi, rowB = !cy.lo || !cy.hi;
+  const bool hx = !colB || rowB, vy = !rowB || colB;
+  return vcand_keys(fv, fnx, fny, fpy, fpx, cx.lo && hx, cy.lo && vy, cy.hi && vy, cx.hi && hx);
+}
+
+// quad_lt decision of an edge with both cofacets usable, on split keys
+MSCX_HD int hcand_both(const KeyWin &K, int x, int y) {
+  const VKey a = K(x, y), b = K(x + 1, y);
+  const bool a_hi = a > b;
+  const VKey hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
+  const VKey ua = K(x, y + 1), ub = K(x + 1, y + 1), da = K(x, y - 1), db = K(x + 1, y - 1);
+  return quad_lt(lo, hi, ksel(a_hi, ua, ub), ksel(a_hi, ub, ua), lo, hi, ksel(a_hi, da, db), ksel(a_hi, db, da))
+             ? DIR_PY
+             : DIR_NY;
+}
+MSCX_HD int ecand_both(const KeyWin &K, int x, int y) {
+  const VKey a = K(x, y), b = K(x, y + 1);
+  const bool a_hi = a > b;
+  const VKey hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
+  const VKey ra = K(x + 1, y), rb = K(x + 1, y + 1), la = K(x - 1, y), lb = K(x - 1, y + 1);
+  return quad_lt(lo, hi, ksel(a_hi, ra, rb), ksel(a_hi, rb, ra), lo, hi, ksel(a_hi, la, lb), ksel(a_hi, lb, la))
+             ? DIR_PX
+             : DIR_NX;
+}
+
+// Candidates of the first row of a walk: vertices (i, j), (i+1, j) and the
+// edge (i, j)-(i+1, j).
+template <bool INTERIOR>
+MSCX_HD void row_init(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, int &hc00) {
+  const int i = K.i, j = K.j;
+  const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
+  const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1);
+  vc00 = vcand_at<INTERIOR>(K.w11, K.w10, K.w01, K.w21, K.w12, c0, r0);
+  vc10 = vcand_at<INTERIOR>(K.w12, K.w11, K.w02, K.w22, K.w13, c1, r0);
+  int h = hcand_keys(K.w11, K.w12, K.w21, K.w22, K.w01, K.w02);
+  if (!INTERIOR && !(c1.in && r0.in && c0.re && r0.rv && r0.lo && r0.hi)) h = DIR_NONE;
+  if (h == CAND_BOTH) h = hcand_both(K, i, j);
+  hc00 = h;
+}
+
+// One row: the codes of cells (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1)
+// (code_from_cands), and the carried candidates advanced to row j + 1.
+// Cells outside the ext get code 0.
+template <bool INTERIOR>
+MSCX_HD void row_codes(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, int &hc00, uint8_t &cv, uint8_t &chh,
+                       uint8_t &ce, uint8_t &cq) {
+  const int i = K.i, j = K.j;
+  const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
+  const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
+  // new candidates: vertices (i, j+1), (i+1, j+1), edges (i, j+1)-(i+1, j+1),
+  // (i, j)-(i, j+1) and (i+1, j)-(i+1, j+1)
+  const int vc01 = vcand_at<INTERIOR>(K.w21, K.w20, K.w11, K.w31, K.w22, c0, r1);
+  const int vc11 = vcand_at<INTERIOR>(K.w22, K.w21, K.w12, K.w32, K.w23, c1, r1);
+  int hc01 = hcand_keys(K.w21, K.w22, K.w31, K.w32, K.w11, K.w12);
+  int ec00 = ecand_keys(K.w11, K.w21, K.w12, K.w22, K.w10, K.w20);
+  int ec10 = ecand_keys(K.w12, K.w22, K.w13, K.w23, K.w11, K.w21);
+  if (!INTERIOR) {
+    if (!(c1.in && r1.in && c0.re && r1.rv && r1.lo && r1.hi)) hc01 = DIR_NONE;
+    if (!(c0.in && r1.in && c0.rv && r0.re && c0.lo && c0.hi)) ec00 = DIR_NONE;
+    if (!(c1.in && r1.in && c1.rv && r0.re && c1.lo && c1.hi)) ec10 = DIR_NONE;
+  }
+  if (hc01 == CAND_BOTH) hc01 = hcand_both(K, i, j + 1);
+  if (ec00 == CAND_BOTH) ec00 = ecand_both(K, i, j);
+  if (ec10 == CAND_BOTH) ec10 = ecand_both(K, i + 1, j);
+
+  // codes (code_from_cands)
+  int dv = vc00;
+  int dh = hc00;
+  if (dh == DIR_NONE) {
+    const bool a_hi = K.w11 > K.w12;
+    if (a_hi ? vc00 == DIR_PX : vc10 == DIR_NX) dh = a_hi ? DIR_NX : DIR_PX;
+  }
+  int de = ec00;
+  if (de == DIR_NONE) {
+    const bool a_hi = K.w11 > K.w21;
+    if (a_hi ? vc00 == DIR_PY : vc01 == DIR_NY) de = a_hi ? DIR_NY : DIR_PY;
+  }
+  int dq = DIR_NONE;
+  {
+    const uint32_t k00 = K.w11, k10 = K.w12, k01 = K.w21, k11 = K.w22;
+    // position order 00 < 01 < 10 < 11 (x, then y)
+    const bool g00_10 = k00 > k10, g00_01 = k00 > k01, g00_11 = k00 > k11;
+    const bool g10_01 = k10 >= k01, g10_11 = k10 > k11, g01_11 = k01 > k11;
+    int edir;
+    if (g00_10 && g00_01 && g00_11) edir = g10_01 ? DIR_NY : DIR_NX;
+    else if (g10_01 && g10_11) edir = g00_11 ? DIR_NY : DIR_PX;
+    else if (g01_11) edir = g00_11 ? DIR_NX : DIR_PY;
+    else edir = g10_01 ? DIR_PX : DIR_PY;
+    const int ec = edir == DIR_NY ? hc00 : edir == DIR_PY ? hc01 : edir == DIR_NX ? ec00 : ec10;
+    if (ec == dir_flip(edir)) dq = edir;
+  }
+  if (INTERIOR) {
+    cv = (uint8_t)((dv ? CF_PAIRED : CF_CRITICAL) | (dv << 2));
+    chh = (uint8_t)((dh ? CF_PAIRED : CF_CRITICAL) | (dh << 2));
+    ce = (uint8_t)((de ? CF_PAIRED : CF_CRITICAL) | (de << 2));
+    cq = (uint8_t)((dq ? CF_PAIRED : CF_CRITICAL) | (dq << 2));
+  } else {
+    // flags: in rect (exterior unpaired cells are not critical) and the
+    // boundary-cancellable sites of a cell and of its pair
+    auto site = [&](bool inx, bool iny, bool sx, bool sy) { return inx && iny && (sx || sy); };
+    const bool in_v = c0.rv && r0.rv, in_h = c0.re && r0.rv, in_e = c0.rv && r0.re, in_q = c0.re && r0.re;
+    const bool s_v = site(c0.rv, r0.rv, c0.sv, r0.sv), s_h = site(c0.re, r0.rv, c0.se, r0.sv);
+    const bool s_e = site(c0.rv, r0.re, c0.sv, r0.se), s_q = site(c0.re, r0.re, c0.se, r0.se);
+    // sites of the cells each of the four cells can be paired with
+    const bool s_vpx = s_h;                                  // (2i+1, 2j)
+    const bool s_vnx = site(c0.rm, r0.rv, c0.sm, r0.sv);     // (2i-1, 2j)
+    const bool s_vpy = s_e;                                  // (2i, 2j+1)
+    const bool s_vny = site(c0.rv, r0.rm, c0.sv, r0.sm);     // (2i, 2j-1)
+    const bool s_hpx = site(c1.rv, r0.rv, c1.sv, r0.sv);     // (2i+2, 2j)
+    const bool s_hnx = s_v;                                  // (2i, 2j)
+    const bool s_hpy = s_q;                                  // (2i+1, 2j+1)
+    const bool s_hny = site(c0.re, r0.rm, c0.se, r0.sm);     // (2i+1, 2j-1)
+    const bool s_epx = s_q;                                  // (2i+1, 2j+1)
+    const bool s_enx = site(c0.rm, r0.re, c0.sm, r0.se);     // (2i-1, 2j+1)
+    const bool s_epy = site(c0.rv, r1.rv, c0.sv, r1.sv);     // (2i, 2j+2)
+    const bool s_eny = s_v;                                  // (2i, 2j)
+    const bool s_qpx = site(c1.rv, r0.re, c1.sv, r0.se);     // (2i+2, 2j+1)
+    const bool s_qnx = s_e;                                  // (2i, 2j+1)
+    const bool s_qpy = site(c0.re, r1.rv, c0.se, r1.sv);     // (2i+1, 2j+2)
+    const bool s_qny = s_h;                                  // (2i+1, 2j)
+    auto pick = [](int d, bool px, bool nx, bool py, bool ny) {
+      return d == DIR_PX ? px : d == DIR_NX ? nx : d == DIR_PY ? py : d == DIR_NY ? ny : false;
+    };
+    auto mk = [](int d, bool inr, bool self_site, bool pair_site) -> uint8_t {
+      if (d == DIR_NONE) return (uint8_t)(inr ? CF_CRITICAL : 0);
+      return (uint8_t)(CF_PAIRED | ((self_site || pair_site) ? CF_CRITICAL : 0) | (d << 2));
+    };
+    cv = mk(dv, in_v, s_v, pick(dv, s_vpx, s_vnx, s_vpy, s_vny));
+    chh = mk(dh, in_h, s_h, pick(dh, s_hpx, s_hnx, s_hpy, s_hny));
+    ce = mk(de, in_e, s_e, pick(de, s_epx, s_enx, s_epy, s_eny));
+    cq = mk(dq, in_q, s_q, pick(dq, s_qpx, s_qnx, s_qpy, s_qny));
+    if (!(c0.in && r0.in)) cv = 0;
+    if (!(c1.in && r0.in)) chh = 0;
+    if (!(c0.in && r1.in)) ce = 0;
+    if (!(c1.in && r1.in)) cq = 0;
+  }
+  vc00 = vc01;
+  vc10 = vc11;
+  hc00 = hc01;
 }
 
 }  // namespace mscx

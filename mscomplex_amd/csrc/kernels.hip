@@ -32,27 +32,97 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 // (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1).
 //  1. keys of the tile's vertices and a 1/2-vertex halo go to LDS (all
 //     global loads of a lane are issued before the first LDS store)
-//  2. the pairing candidate of every vertex and of both edges starting at
-//     it (vertex_cand / hedge_cand / vedge_cand) is computed once into LDS
-//  3. the four cell codes of every vertex position (code_from_cands), the
-//     critical flags of each 128-cell row segment (two ballots) and the
-//     trace links: a vertex points across its paired edge to the next
-//     vertex, a quad across its paired edge to the next quad
-//     (grid_dataset.cpp:1376-1427)
-//  4. pointer jumping in LDS takes every entry to its terminal in the tile:
+//  2. each wave walks GRAD_TY/4 rows with a 4x4 window of keys in registers
+//     (rows j-1..j+2, columns i-1..i+2; one new LDS row per step) and
+//     evaluates the pairing candidates it needs there (vertex_cand /
+//     hedge_cand / vedge_cand, carrying the next row's over), then the four
+//     cell codes (code_from_cands), the critical flags of each 128-cell row
+//     segment (two ballots) and the trace links: a vertex points across its
+//     paired edge to the next vertex, a quad across its paired edge to the
+//     next quad (grid_dataset.cpp:1376-1427).  Blocks whose cells all lie
+//     strictly inside the interior rect take a path without boundary tests.
+//  3. pointer jumping in LDS takes every entry to its terminal in the tile:
 //     its critical root, "none", or the exit through the tile's side
-//  5. terminals go to `loc` (16 bits per vertex and per quad); the
+//  4. terminals go to `loc` (16 bits per vertex and per quad); the
 //     terminals of the tile's perimeter entries go to its ring words, in the
 //     form the ring resolution chases.
+
+// trace link of a vertex (q = 0) or quad (q = 1) entry from its code
+__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q) {
+  constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY;
+  if (c & CF_CRITICAL) return T_ROOT | (uint32_t)(lj * TX + li);
+  if (!(c & CF_PAIRED)) return T_NONE;
+  const int d = code_dir(c);
+  const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
+  if (ni >= 0 && nj >= 0 && ni < TX && nj < TY) return (uint32_t)(q * NV + nj * TX + ni);
+  const uint32_t s = ni >= TX ? 0u : (ni < 0 ? 1u : (nj >= TY ? 2u : 3u));
+  const uint32_t p = (s < 2) ? (uint32_t)lj : (uint32_t)li;
+  return T_EXIT | (s << 6) | p;
+}
+
+// Step 2 for one wave: rows lj0 .. lj0 + GRAD_TY/4 - 1 of the block tile,
+// with the row-step rules (row_init / row_codes) on a register window.
+template <bool INTERIOR>
+__device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, const uint32_t *__restrict__ sk,
+                                          uint16_t *__restrict__ sl, int i0, int j0, int bx, int lane, int lj0) {
+  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, NV = TX * TY, RW = TY / 4;
+  const int li = lane, i = i0 + li;
+  KeyWin K;
+  {
+    const uint32_t *s0 = sk + lj0 * KW + li;
+    K.w00 = s0[0], K.w01 = s0[1], K.w02 = s0[2], K.w03 = s0[3];
+    K.w10 = s0[KW], K.w11 = s0[KW + 1], K.w12 = s0[KW + 2], K.w13 = s0[KW + 3];
+    K.w20 = s0[2 * KW], K.w21 = s0[2 * KW + 1], K.w22 = s0[2 * KW + 2], K.w23 = s0[2 * KW + 3];
+    K.w30 = s0[3 * KW], K.w31 = s0[3 * KW + 1], K.w32 = s0[3 * KW + 2], K.w33 = s0[3 * KW + 3];
+  }
+  K.i = i;
+  K.j = j0 + lj0;
+  int vc00, vc10, hc00;
+  row_init<INTERIOR>(K, g, vc00, vc10, hc00);
+  for (int rr = 0; rr < RW; ++rr) {
+    const int lj = lj0 + rr, j = K.j;
+    if (j >= g.vy) break;  // uniform across the wave
+    uint8_t cv, chh, ce, cq;
+    row_codes<INTERIOR>(K, g, vc00, vc10, hc00, cv, chh, ce, cq);
+    const bool ein = INTERIOR || j + 1 < g.vy;
+    uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
+    *(uint16_t *)crow = (uint16_t)(cv | (chh << 8));
+    const int64_t rs0 = (int64_t)(2 * j) * T.nseg + bx;
+    const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
+    if (lane == 0) {
+      T.critbits[2 * rs0] = b0;
+      T.critbits[2 * rs0 + 1] = b1;
+      T.seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
+    }
+    if (ein) {
+      *(uint16_t *)(crow + T.pitch) = (uint16_t)(ce | (cq << 8));
+      const int64_t rs1 = rs0 + T.nseg;
+      const uint64_t b2 = __ballot(ce & CF_CRITICAL), b3 = __ballot(cq & CF_CRITICAL);
+      if (lane == 0) {
+        T.critbits[2 * rs1] = b2;
+        T.critbits[2 * rs1 + 1] = b3;
+        T.seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
+      }
+    }
+    // codes outside the ext are 0, so those entries get T_NONE
+    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0);
+    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1);
+    if (rr + 1 < RW) {
+      const uint32_t *sn = sk + (lj + 4) * KW + li;
+      K.slide(sn[0], sn[1], sn[2], sn[3]);
+    } else {
+      ++K.j;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                     int ntiles, uint32_t *err) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY;
   constexpr int KW = TX + 3, KH = TY + 3;  // keys of vertices [i0-1, i0+TX+1] x [j0-1, j0+TY+1]
-  constexpr int CW1 = TX + 1, CH1 = TY + 1;  // candidates of vertices [i0, i0+TX] x [j0, j0+TY]
   constexpr int NV = TX * TY;              // entries per forest
   constexpr int RW = TY / 4;               // vertex rows per wave
   __shared__ uint32_t sk[KH * KW];
-  __shared__ uint16_t sc[CH1 * CW1];
   __shared__ uint16_t sl[2 * NV];
 
   const int b = blockIdx.x;
@@ -88,71 +158,14 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     }
   }
   __syncthreads();
-  auto K = [&](int i, int j) -> VKey { return VKey{sk[(j - j0 + 1) * KW + (i - i0 + 1)], i, j}; };
 
-  // 2. candidates (DIR_NONE outside the grid or the interior rect)
-  for (int idx = tid; idx < CW1 * CH1; idx += 256) {
-    const int li = idx % CW1, lj = idx / CW1;
-    const int i = i0 + li, j = j0 + lj;
-    int vc = DIR_NONE, hc = DIR_NONE, ec = DIR_NONE;
-    if (i < g.vx && j < g.vy) {
-      if (in_rect(g, 2 * i, 2 * j)) vc = vertex_cand(K, g, i, j);
-      if (i + 1 < g.vx && in_rect(g, 2 * i + 1, 2 * j)) hc = hedge_cand(K, g, i, j);
-      if (j + 1 < g.vy && in_rect(g, 2 * i, 2 * j + 1)) ec = vedge_cand(K, g, i, j);
-    }
-    sc[idx] = (uint16_t)(vc | (hc << 3) | (ec << 6));
-  }
-  __syncthreads();
-  auto SC = [&](int i, int j) -> uint32_t { return sc[(j - j0) * CW1 + (i - i0)]; };
-  auto VC = [&](int i, int j) -> int { return (int)(SC(i, j) & 7u); };
-  auto HC = [&](int i, int j) -> int { return (int)((SC(i, j) >> 3) & 7u); };
-  auto EC = [&](int i, int j) -> int { return (int)((SC(i, j) >> 6) & 7u); };
-
-  // trace link of a vertex (q = 0) or quad (q = 1) entry from its code
-  auto link_of = [&](uint8_t c, int li, int lj, int q) -> uint32_t {
-    if (c & CF_CRITICAL) return T_ROOT | (uint32_t)(lj * TX + li);
-    if (!(c & CF_PAIRED)) return T_NONE;
-    const int d = code_dir(c);
-    const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
-    if (ni >= 0 && nj >= 0 && ni < TX && nj < TY) return (uint32_t)(q * NV + nj * TX + ni);
-    const uint32_t s = ni >= TX ? 0u : (ni < 0 ? 1u : (nj >= TY ? 2u : 3u));
-    const uint32_t p = (s < 2) ? (uint32_t)lj : (uint32_t)li;
-    return T_EXIT | (s << 6) | p;
-  };
-
-  // 3. codes, critical flags per row segment, trace links
-  for (int r = 0; r < RW; ++r) {
-    const int lj = wv * RW + r, li = lane;
-    const int i = i0 + li, j = j0 + lj;
-    if (j >= g.vy) break;  // uniform across the wave
-    const bool vin = i < g.vx, hin = i + 1 < g.vx, ein = j + 1 < g.vy;
-    uint8_t cv = 0, chh = 0, ce = 0, cq = 0;
-    if (vin) cv = code_from_cands(K, VC, HC, EC, g, 2 * i, 2 * j);
-    if (hin) chh = code_from_cands(K, VC, HC, EC, g, 2 * i + 1, 2 * j);
-    if (vin && ein) ce = code_from_cands(K, VC, HC, EC, g, 2 * i, 2 * j + 1);
-    if (hin && ein) cq = code_from_cands(K, VC, HC, EC, g, 2 * i + 1, 2 * j + 1);
-    uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
-    *(uint16_t *)crow = (uint16_t)(cv | (chh << 8));
-    const int64_t rs0 = (int64_t)(2 * j) * T.nseg + bx;
-    const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
-    if (lane == 0) {
-      T.critbits[2 * rs0] = b0;
-      T.critbits[2 * rs0 + 1] = b1;
-      T.seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
-    }
-    if (ein) {
-      *(uint16_t *)(crow + T.pitch) = (uint16_t)(ce | (cq << 8));
-      const int64_t rs1 = rs0 + T.nseg;
-      const uint64_t b2 = __ballot(ce & CF_CRITICAL), b3 = __ballot(cq & CF_CRITICAL);
-      if (lane == 0) {
-        T.critbits[2 * rs1] = b2;
-        T.critbits[2 * rs1 + 1] = b3;
-        T.seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
-      }
-    }
-    sl[lj * TX + li] = (uint16_t)(vin ? link_of(cv, li, lj, 0) : T_NONE);
-    sl[NV + lj * TX + li] = (uint16_t)((hin && ein) ? link_of(cq, li, lj, 1) : T_NONE);
-  }
+  // 2. codes, critical flags per row segment, trace links
+  const bool interior = (2 * i0 - 1 > g.rx0) && (2 * (i0 + TX) + 1 < g.rx1) && (2 * j0 - 1 > g.ry0) &&
+                        (2 * (j0 + TY) + 1 < g.ry1);
+  if (interior)
+    grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, wv * RW);
+  else
+    grad_rows<false>(T, g, sk, sl, i0, j0, bx, lane, wv * RW);
   // rows past the ext edge hold no entries
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
@@ -162,39 +175,42 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   }
   __syncthreads();
 
-  // 4. pointer jumping.  Links only ever move further along their path, so
-  //    the entries are updated without barriers; gradient paths are acyclic
-  //    and every round moves an unresolved entry at least one step, so the
-  //    loop ends within the longest path (the bound guards bad input).
+  // 3. pointer jumping.  Links only ever move further along their path, so
+  //    the entries are updated without barriers: whatever value a read
+  //    returns (the original link, a later one or the terminal) is a valid
+  //    state.  Each round reads all of a thread's pending entries, then all
+  //    their targets, then writes, so the LDS latency is paid twice per
+  //    round rather than per entry.  Gradient paths are acyclic and every
+  //    round moves a pending entry at least one step, so the loop ends
+  //    within the longest path (the bound guards bad input).
   {
-    volatile uint16_t *vl = sl;
     constexpr int PER = 2 * NV / 256;
     uint32_t pend = (1u << PER) - 1u;
     for (int round = 0; pend && round < 2 * NV; ++round) {
+      uint32_t e1[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) e1[k] = (pend >> k) & 1u ? (uint32_t)sl[tid + 256 * k] : T_NONE;
+      uint32_t e2[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) e2[k] = (e1[k] & 0x8000u) ? e1[k] : (uint32_t)sl[e1[k]];
+      uint32_t np = 0;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
-        if (!(pend & (1u << k))) continue;
-        const int e = tid + 256 * k;
-        const uint32_t w = vl[e];
-        if (w & 0x8000u) {
-          pend &= ~(1u << k);
-          continue;
-        }
-        const uint32_t w2 = vl[w];
-        vl[e] = (uint16_t)w2;
-        if (w2 & 0x8000u) pend &= ~(1u << k);
+        if (!(e1[k] & 0x8000u)) sl[tid + 256 * k] = (uint16_t)e2[k];
+        np |= (uint32_t)((e2[k] & 0x8000u) == 0) << k;
       }
+      pend = np;
     }
     if (pend) {
       atomicOr(err, 2u);
 #pragma unroll
       for (int k = 0; k < PER; ++k)
-        if (pend & (1u << k)) vl[tid + 256 * k] = (uint16_t)T_NONE;
+        if ((pend >> k) & 1u) sl[tid + 256 * k] = (uint16_t)T_NONE;
     }
   }
   __syncthreads();
 
-  // 5. terminals and ring words
+  // 4. terminals and ring words
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
     const int j = j0 + lj;
