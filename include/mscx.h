@@ -185,6 +185,37 @@ int mscx_batch_grid(const mscx_batch *b, int32_t i, const mscx_grid **out, int32
 const char *mscx_batch_error(const mscx_batch *b, int32_t i);
 int mscx_batch_destroy(mscx_batch *b);
 
+/* ---- serving: continuous batching over a request queue --------------------
+ * No reference counterpart (the reference runs one grid per GridDataManager);
+ * every result equals mscx_grid_run on the same grid, 2-D tiling and
+ * threshold.  A scheduler thread admits queued requests against an HBM
+ * budget (bytes of device memory the library may hold) and launches the GPU
+ * stages of all grids admitted in one round together; their host phases run
+ * on a pool of host_threads workers (0 = one per core, at most 16) while the
+ * next round is admitted, so requests that arrive while others are in
+ * flight overlap with them.  A request with a nonzero key leaves its grid
+ * resident (owner images in HBM, merged complexes on the host) until the
+ * budget needs the memory (least recently used first); a later request with
+ * the same key is served from it -- simplified at its own threshold without
+ * the GPU stages -- and may pass data = NULL while the key is resident.  The
+ * key names the grid and its tiling.  A grid larger than the budget fails
+ * with MSCX_ENOMEM. */
+typedef struct mscx_server mscx_server;
+int mscx_server_create(size_t hbm_budget, int32_t host_threads, mscx_server **out);
+int mscx_server_submit(mscx_server *s, const float *data, int32_t size_x, int32_t size_y, int32_t lx, int32_t ly,
+                       double threshold, uint64_t key, uint32_t flags, uint64_t *ticket);
+/* Waits up to timeout_s seconds (< 0: no limit) for a request; *out is
+ * borrowed until mscx_server_release, *status its own status, *latency_s
+ * the time from submission to completion. */
+int mscx_server_wait(mscx_server *s, uint64_t ticket, double timeout_s, const mscx_grid **out, int32_t *status,
+                     double *latency_s);
+const char *mscx_server_error(mscx_server *s, uint64_t ticket);
+int mscx_server_release(mscx_server *s, uint64_t ticket);
+/* done, scheduling rounds, repeat requests, evictions, resident grids,
+ * device bytes held, budget, queued requests */
+int mscx_server_stats(mscx_server *s, uint64_t out[8]);
+int mscx_server_destroy(mscx_server *s);
+
 /* ---- synthetic inputs (seeded, host) -------------------------------------- */
 
 /* Sum of n Gaussians, centres U[0,sx)xU[0,sy), sigma U[8,64], amplitude

@@ -345,6 +345,63 @@ class Batch:
         return _GridView(g, self)
 
 
+class Server:
+    """Continuous-batching request loop (mscx_server_*): requests are
+    admitted against an HBM budget, the GPU stages of every grid admitted in
+    one round run together, host phases overlap the next round.  A nonzero
+    key keeps the grid resident for repeat requests at other thresholds."""
+
+    def __init__(self, hbm_budget, host_threads=0):
+        h = ctypes.c_void_p()
+        check(lib().mscx_server_create(int(hbm_budget), int(host_threads), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            check(lib().mscx_server_destroy(h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, data, lx=0, ly=0, threshold=0.0, key=0, discs=True, size=None):
+        """data: 2-D float32 array, or None for a resident key (then size=(sx, sy))."""
+        t = ctypes.c_uint64()
+        if data is None:
+            sx, sy = size
+            p = None
+        else:
+            data = np.ascontiguousarray(data, np.float32)
+            sy, sx = data.shape
+            p = ptr(data)
+        check(lib().mscx_server_submit(self._h, p, sx, sy, lx, ly, float(threshold), int(key),
+                                       0 if discs else 2, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """(status, error, latency seconds, grid view or None)."""
+        g = ctypes.c_void_p()
+        st = ctypes.c_int32()
+        lat = ctypes.c_double()
+        check(lib().mscx_server_wait(self._h, ticket, float(timeout), ctypes.byref(g), ctypes.byref(st),
+                                     ctypes.byref(lat)))
+        err = lib().mscx_server_error(self._h, ticket).decode()
+        return st.value, err, lat.value, (_GridView(g, self) if st.value == 0 and g.value else None)
+
+    def release(self, ticket):
+        check(lib().mscx_server_release(self._h, ticket))
+
+    def stats(self):
+        o = np.zeros(8, np.uint64)
+        check(lib().mscx_server_stats(self._h, ptr(o)))
+        keys = ("done", "rounds", "repeats", "evicted", "resident", "device_bytes", "budget", "queued")
+        return {k: int(v) for k, v in zip(keys, o)}
+
+
 class GridRun:
     """A whole GridDataManager compute run over one grid (mscx_grid_run)."""
 

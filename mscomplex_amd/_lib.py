@@ -61,6 +61,15 @@ _SIGS = {
     "mscx_batch_grid": (c_int, [P, c_int32, ctypes.POINTER(P), ctypes.POINTER(c_int32)]),
     "mscx_batch_error": (ctypes.c_char_p, [P, c_int32]),
     "mscx_batch_destroy": (c_int, [P]),
+    "mscx_server_create": (c_int, [c_size_t, c_int32, ctypes.POINTER(P)]),
+    "mscx_server_submit": (c_int, [P, P, c_int32, c_int32, c_int32, c_int32, c_double, c_uint64, c_uint32,
+                                   ctypes.POINTER(c_uint64)]),
+    "mscx_server_wait": (c_int, [P, c_uint64, c_double, ctypes.POINTER(P), ctypes.POINTER(c_int32),
+                                 ctypes.POINTER(c_double)]),
+    "mscx_server_error": (ctypes.c_char_p, [P, c_uint64]),
+    "mscx_server_release": (c_int, [P, c_uint64]),
+    "mscx_server_stats": (c_int, [P, P]),
+    "mscx_server_destroy": (c_int, [P]),
     "mscx_gen_gaussians": (c_int, [P, c_int32, c_int32, c_int32, c_double, c_uint64]),
     "mscx_gen_value_noise": (c_int, [P, c_int32, c_int32, c_int32, c_double, c_double, c_double, c_uint64]),
     "mscx_gen_white": (c_int, [P, c_int32, c_int32, c_int32, c_uint64]),

@@ -23,7 +23,7 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
               "-Wno-unused-command-line-argument"]
 
 DEVICE_SOURCES = ["kernels.hip"]
-HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline.cpp", "gen.cpp"]
+HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline.cpp", "server.cpp", "gen.cpp"]
 
 
 def _hipcc():

@@ -98,9 +98,12 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
 void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   if (n <= 0) return;
   ensure_init();
-  Runtime &rt = runtime();
-  std::lock_guard<std::mutex> lk(rt.mu);
-  hipStream_t s = rt.stream;
+  // The label pass runs on the auxiliary stream under its own lock, so it
+  // overlaps the tile stages of other grids on the work stream; the owner
+  // images and codes it reads were completed (and synchronised) by run_tiles.
+  static std::mutex disc_mu;  // guards g_disc / g_stage
+  std::lock_guard<std::mutex> lk(disc_mu);
+  hipStream_t s = runtime().aux;
   for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i]->worked, MSCX_ESTATE, "tile has not been worked");
 
   std::vector<Rect> rect(n), ext(n);

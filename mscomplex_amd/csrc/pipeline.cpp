@@ -3,7 +3,10 @@
 // at once.  Every leaf tile of every grid goes through each HIP stage in one
 // launch; the complex algebra runs on host threads (merges of one tree level
 // in parallel for a single grid, whole grids in parallel for a batch); the
-// manifold label pass of every leaf is again one launch.
+// manifold label pass of every leaf is again one launch.  The stages are
+// also driven by the serving loop (server.cpp).
+#include "pipeline.hpp"
+
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -12,19 +15,13 @@
 #include <thread>
 #include <vector>
 
-#include "complex.hpp"
-#include "runtime.hpp"
-#include "tile.hpp"
-
 namespace mscx {
 
-static double now_s() {
+double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static int host_threads() {
-  return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
-}
+int host_threads() { return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u); }
 
 // Dynamic parallel loop over [0, n) on up to `nt` threads; the first
 // exception is rethrown after all threads have joined.
@@ -52,15 +49,11 @@ static void parallel_for(int n, F f, int nt = 0) {
     if (e) std::rethrow_exception(e);
 }
 
-struct LeafGeom {
-  Rect rect, ext;
-};
-
 // createPieces_quadtree (grid_datamanager.cpp:53-89): split along y into
 // 2^levels slabs; with lx/ly a 2-D tiling split along y while it has more
 // levels left than x, else along x.  Children share the split line; each
 // ext grows 2 cells (one vertex) past it.
-static void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d, std::vector<LeafGeom> &out) {
+void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d, std::vector<LeafGeom> &out) {
   if (level == 0) {
     out.push_back({r, e});
     return;
@@ -86,43 +79,21 @@ static void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d,
   create_pieces(r2, e2, level - 1, nlx, nly, two_d, out);
 }
 
-}  // namespace mscx
-
-using namespace mscx;
-
-struct mscx_grid {
-  int32_t sx = 0, sy = 0;
-  double threshold = 0;
-  uint32_t flags = 0;
-  std::vector<LeafGeom> leaves;
-  std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
-  std::vector<mscx_tile *> tiles;
-  void *dev_input = nullptr;
-  double t[5] = {0, 0, 0, 0, 0};
-  ~mscx_grid() {
-    for (mscx_complex *c : pieces) delete c;
-    for (mscx_tile *t : tiles) mscx_tile_destroy(t);
-    if (dev_input) mscx_dev_free(dev_input);
-  }
-};
-
-struct mscx_batch {
-  std::vector<mscx_grid *> grids;
-  std::vector<int> status;
-  std::vector<std::string> error;
-  ~mscx_batch() {
-    for (mscx_grid *g : grids) delete g;
-  }
-};
-
-namespace mscx {
-
-// Validates arguments, uploads host input, creates and binds the leaf tiles.
-static void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx,
-                         int32_t ly, double threshold, uint32_t flags) {
-  MSCX_REQUIRE(data, MSCX_EINVAL, "null input");
+static void check_grid_args(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly) {
   MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
   MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
+}
+
+static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, std::vector<LeafGeom> &out) {
+  const bool two_d = lx > 0 || ly > 0;
+  const Rect whole{0, 0, 2 * (sx - 1), 2 * (sy - 1)};
+  create_pieces(whole, whole, two_d ? lx + ly : levels, lx, ly, two_d, out);
+}
+
+void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
+                  double threshold, uint32_t flags) {
+  MSCX_REQUIRE(data, MSCX_EINVAL, "null input");
+  check_grid_args(sx, sy, levels, lx, ly);
   MSCX_REQUIRE(threshold >= 0.0 && !std::isnan(threshold), MSCX_EINVAL, "threshold must be >= 0");
   G->sx = sx;
   G->sy = sy;
@@ -136,10 +107,7 @@ static void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy
     MSCX_HIP(hipMemcpy(G->dev_input, data, n * 4, hipMemcpyHostToDevice));
     dev = (const float *)G->dev_input;
   }
-  bool two_d = lx > 0 || ly > 0;
-  int L = two_d ? lx + ly : levels;
-  Rect whole{0, 0, 2 * (sx - 1), 2 * (sy - 1)};
-  create_pieces(whole, whole, L, lx, ly, two_d, G->leaves);
+  leaves_of(sx, sy, levels, lx, ly, G->leaves);
   for (const LeafGeom &lg : G->leaves) {
     int32_t r[4] = {lg.rect.x0, lg.rect.y0, lg.rect.x1, lg.rect.y1};
     int32_t e[4] = {lg.ext.x0, lg.ext.y0, lg.ext.x1, lg.ext.y1};
@@ -152,62 +120,26 @@ static void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy
   }
 }
 
-// Merge up, simplify, merge down for one grid whose leaf complexes are built.
-static void grid_tree(mscx_grid *G, bool inner_parallel) {
-  const int nleaf = (int)G->leaves.size();
-  const int nt = inner_parallel ? 0 : 1;
-  double t1 = now_s();
-  G->pieces.resize(2 * nleaf - 1, nullptr);
-  for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
-  // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
-  for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
-    parallel_for(
-        width / 2,
-        [&](int k) {
-          int j = lo + 2 * k;
-          merge_up(G->pieces[j]->c, G->pieces[j + 1]->c, G->pieces[nleaf + j / 2]->c);
-        },
-        nt);
-  double t2 = now_s();
-  G->t[1] += t2 - t1;
-  if (G->threshold > 0.0) {
-    std::vector<std::pair<uint32_t, uint32_t>> canc;
-    simplify(G->pieces.back()->c, canc, G->threshold);
-    un_simplify(G->pieces.back()->c, canc);
+size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, bool with_input) {
+  check_grid_args(sx, sy, levels, lx, ly);
+  std::vector<LeafGeom> leaves;
+  leaves_of(sx, sy, levels, lx, ly, leaves);
+  size_t b = with_input ? (size_t)sx * sy * 4 : 0;
+  for (const LeafGeom &lg : leaves) {
+    int32_t r[4] = {lg.rect.x0, lg.rect.y0, lg.rect.x1, lg.rect.y1};
+    int32_t e[4] = {lg.ext.x0, lg.ext.y0, lg.ext.x1, lg.ext.y1};
+    b += tile_fixed_bytes(r, e);
+    const size_t cells = (size_t)(e[2] - e[0] + 1) * (e[3] - e[1] + 1);
+    b += CritLayout(cells / 4).total;
   }
-  double t3 = now_s();
-  G->t[2] += t3 - t2;
-  // mergePiecesDown + finalMergeDownPiecesInRange (:308-395): parents before
-  // children, one tree level at a time
-  if (nleaf > 1) {
-    std::vector<int> level{2 * nleaf - 2};
-    while (!level.empty()) {
-      parallel_for(
-          (int)level.size(),
-          [&](int k) {
-            int p = level[k];
-            int c = 2 * (p - nleaf);
-            merge_down(G->pieces[p]->c, G->pieces[c]->c, G->pieces[c + 1]->c);
-          },
-          nt);
-      std::vector<int> next;
-      for (int p : level) {
-        int c = 2 * (p - nleaf);
-        if (c >= nleaf) next.push_back(c);
-        if (c + 1 >= nleaf) next.push_back(c + 1);
-      }
-      level.swap(next);
-    }
-  }
-  G->t[3] += now_s() - t3;
+  return b;
 }
 
+// Host staging of the critical-point block of one compute call.
+static std::mutex g_crit_mu;
 static Scratch g_crit_host;
 
-// The whole compute order for a set of prepared grids.  A failure of one
-// grid in the host phases is recorded in status/error (when given) so the
-// other grids complete; device failures fail the call.
-static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status, std::vector<std::string> *error) {
+void grids_compute(std::vector<mscx_grid *> &grids) {
   const int ng = (int)grids.size();
   if (!ng) return;
   const double t0 = now_s();
@@ -216,6 +148,7 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
   run_tiles(tiles.data(), (int)tiles.size());
 
   // every tile's critical data lives in one block: one copy to host
+  std::lock_guard<std::mutex> lk(g_crit_mu);
   const size_t bytes = tiles[0]->crit_block_bytes;
   char *host = (char *)g_crit_host.reserve_host(bytes);
   MSCX_HIP(hipMemcpyAsync(host, tiles[0]->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
@@ -245,11 +178,121 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
   });
   const double t1 = now_s();
   for (mscx_grid *G : grids) G->t[0] += t1 - t0;
+}
 
+// mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
+void grid_merge_up(mscx_grid *G, bool inner_parallel) {
+  const int nleaf = (int)G->leaves.size();
+  const int nt = inner_parallel ? 0 : 1;
+  const double t1 = now_s();
+  G->pieces.resize(2 * nleaf - 1, nullptr);
+  for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
+  for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
+    parallel_for(
+        width / 2,
+        [&](int k) {
+          int j = lo + 2 * k;
+          merge_up(G->pieces[j]->c, G->pieces[j + 1]->c, G->pieces[nleaf + j / 2]->c);
+        },
+        nt);
+  G->t[1] += now_s() - t1;
+}
+
+// simplify_un_simplify at the root, then mergePiecesDown +
+// finalMergeDownPiecesInRange (:308-395): parents before children, one tree
+// level at a time
+void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
+  const int nleaf = (int)G->leaves.size();
+  const int nt = inner_parallel ? 0 : 1;
+  const double t2 = now_s();
+  if (G->threshold > 0.0) {
+    std::vector<std::pair<uint32_t, uint32_t>> canc;
+    simplify(G->pieces.back()->c, canc, G->threshold);
+    un_simplify(G->pieces.back()->c, canc);
+  }
+  const double t3 = now_s();
+  G->t[2] += t3 - t2;
+  if (nleaf > 1) {
+    std::vector<int> level{2 * nleaf - 2};
+    while (!level.empty()) {
+      parallel_for(
+          (int)level.size(),
+          [&](int k) {
+            int p = level[k];
+            int c = 2 * (p - nleaf);
+            merge_down(G->pieces[p]->c, G->pieces[c]->c, G->pieces[c + 1]->c);
+          },
+          nt);
+      std::vector<int> next;
+      for (int p : level) {
+        int c = 2 * (p - nleaf);
+        if (c >= nleaf) next.push_back(c);
+        if (c + 1 >= nleaf) next.push_back(c + 1);
+      }
+      level.swap(next);
+    }
+  }
+  G->t[3] += now_s() - t3;
+}
+
+// collectSubdomainManifolds / postMergeFillDiscs (:397-490) for every leaf
+// of every grid that asked for discs
+void grids_discs(std::vector<mscx_grid *> &grids) {
+  const double t4 = now_s();
+  std::vector<mscx_tile *> dt;
+  std::vector<Complex *> dc;
+  for (mscx_grid *G : grids) {
+    if (G->flags & MSCX_RUN_NO_DISCS) continue;
+    for (size_t i = 0; i < G->leaves.size(); ++i) {
+      dt.push_back(G->tiles[i]);
+      dc.push_back(&G->pieces[i]->c);
+    }
+  }
+  fill_discs_device(dt.data(), dc.data(), (int)dt.size());
+  const double t5 = now_s();
+  for (mscx_grid *G : grids) G->t[4] += t5 - t4;
+}
+
+void copy_pieces(const mscx_grid *src, mscx_grid *dst) {
+  for (mscx_complex *c : dst->pieces) delete c;
+  dst->pieces.clear();
+  for (const mscx_complex *c : src->pieces) dst->pieces.push_back(new mscx_complex(*c));
+}
+
+}  // namespace mscx
+
+using namespace mscx;
+
+mscx_grid::~mscx_grid() {
+  for (mscx_complex *c : pieces) delete c;
+  if (owns_tiles)
+    for (mscx_tile *t : tiles) mscx_tile_destroy(t);
+  if (dev_input) mscx_dev_free(dev_input);
+}
+
+struct mscx_batch {
+  std::vector<mscx_grid *> grids;
+  std::vector<int> status;
+  std::vector<std::string> error;
+  ~mscx_batch() {
+    for (mscx_grid *g : grids) delete g;
+  }
+};
+
+namespace mscx {
+
+// The whole compute order for a set of prepared grids.  A failure of one
+// grid in the host phases is recorded in status/error (when given) so the
+// other grids complete; device failures fail the call.
+static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status, std::vector<std::string> *error) {
+  const int ng = (int)grids.size();
+  if (!ng) return;
+  grids_compute(grids);
   std::vector<char> ok(ng, 1);
   auto tree = [&](int gi, bool inner) {
     try {
-      grid_tree(grids[gi], inner);
+      grid_merge_up(grids[gi], inner);
+      grid_simplify_down(grids[gi], inner);
     } catch (const std::exception &e) {
       if (!status) throw;
       ok[gi] = 0;
@@ -263,23 +306,10 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
     tree(0, true);
   else
     parallel_for(ng, [&](int gi) { tree(gi, false); });
-
-  // collectSubdomainManifolds / postMergeFillDiscs (:397-490) for every
-  // leaf of every grid that asked for discs and has not failed
-  const double t4 = now_s();
-  std::vector<mscx_tile *> dt;
-  std::vector<Complex *> dc;
-  for (int gi = 0; gi < ng; ++gi) {
-    mscx_grid *G = grids[gi];
-    if (!ok[gi] || (G->flags & MSCX_RUN_NO_DISCS)) continue;
-    for (size_t i = 0; i < G->leaves.size(); ++i) {
-      dt.push_back(G->tiles[i]);
-      dc.push_back(&G->pieces[i]->c);
-    }
-  }
-  fill_discs_device(dt.data(), dc.data(), (int)dt.size());
-  const double t5 = now_s();
-  for (mscx_grid *G : grids) G->t[4] += t5 - t4;
+  std::vector<mscx_grid *> done;
+  for (int gi = 0; gi < ng; ++gi)
+    if (ok[gi]) done.push_back(grids[gi]);
+  grids_discs(done);
 }
 
 }  // namespace mscx

@@ -1,0 +1,64 @@
+// The GridDataManager compute order (grid_datamanager.cpp:596-662) as stages
+// over sets of grids, shared by mscx_grid_run / mscx_batch_run
+// (pipeline.cpp) and the serving loop (server.cpp).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "complex.hpp"
+#include "runtime.hpp"
+#include "tile.hpp"
+
+namespace mscx {
+
+struct LeafGeom {
+  Rect rect, ext;
+};
+
+// Leaf (rect, ext) pairs of a grid's tiling (createPieces_quadtree).
+void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d, std::vector<LeafGeom> &out);
+
+}  // namespace mscx
+
+// One GridDataManager run over one grid.
+struct mscx_grid {
+  int32_t sx = 0, sy = 0;
+  double threshold = 0;
+  uint32_t flags = 0;
+  std::vector<mscx::LeafGeom> leaves;
+  std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
+  std::vector<mscx_tile *> tiles;
+  bool owns_tiles = true;         // false when the tiles belong to a resident grid of the server
+  std::shared_ptr<void> keep;     // keeps the tiles' owner alive (server results)
+  void *dev_input = nullptr;
+  double t[5] = {0, 0, 0, 0, 0};  // compute, merge-up, simplify, merge-down, discs (seconds)
+  ~mscx_grid();
+};
+
+namespace mscx {
+
+double now_s();
+int host_threads();
+
+// Validates arguments, uploads host input (unless MSCX_RUN_DEVICE_INPUT),
+// creates and binds the leaf tiles.
+void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
+                  double threshold, uint32_t flags);
+// GPU stages of every leaf of every grid (one launch per stage) and the
+// leaf complexes (writeout_connectivity).
+void grids_compute(std::vector<mscx_grid *> &grids);
+// mergePiecesUp; then simplification at G->threshold and mergePiecesDown.
+void grid_merge_up(mscx_grid *G, bool inner_parallel);
+void grid_simplify_down(mscx_grid *G, bool inner_parallel);
+// postMergeFillDiscs for every leaf of the given grids (label pass in one launch).
+void grids_discs(std::vector<mscx_grid *> &grids);
+// HBM a grid's tiles and input take (for admission against a budget);
+// the critical-point block is estimated at one critical cell in four.
+size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, bool with_input);
+
+// Deep copy of a grid's pieces (the complexes); tiles are not copied.
+void copy_pieces(const mscx_grid *src, mscx_grid *dst);
+
+}  // namespace mscx

@@ -19,11 +19,18 @@ void ensure_init() {
     MSCX_HIP(hipSetDevice(rt.device));
     return;
   }
+  static std::mutex init_mu;  // first use may come from several threads
+  std::lock_guard<std::mutex> lk(init_mu);
+  if (rt.device >= 0) {
+    MSCX_HIP(hipSetDevice(rt.device));
+    return;
+  }
   int n = 0;
   MSCX_HIP(hipGetDeviceCount(&n));
   MSCX_REQUIRE(n > 0, MSCX_EDEVICE, "no HIP device is visible");
   MSCX_HIP(hipSetDevice(0));
   MSCX_HIP(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
+  MSCX_HIP(hipStreamCreateWithFlags(&rt.aux, hipStreamNonBlocking));
   rt.device = 0;
 }
 
@@ -168,6 +175,7 @@ int mscx_init(int device) {
     MSCX_REQUIRE(rt.device < 0, MSCX_ESTATE, "mscx_init: already initialised on another device");
     MSCX_HIP(hipSetDevice(device));
     MSCX_HIP(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
+    MSCX_HIP(hipStreamCreateWithFlags(&rt.aux, hipStreamNonBlocking));
     rt.device = device;
   });
 }

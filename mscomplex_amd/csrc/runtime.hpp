@@ -58,7 +58,8 @@ int guarded(F &&f) {
 // serialises batched launches issued from several host threads.
 struct Runtime {
   int device = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // tile stages (run_tiles), serialised by mu
+  hipStream_t aux = nullptr;     // manifold label pass, serialised by its own lock
   std::mutex mu;
 };
 Runtime &runtime();

@@ -24,19 +24,21 @@ static void check_rects(const int32_t *rect, const int32_t *ext) {
                "ext must contain rect");
 }
 
-static void alloc_fixed(mscx_tile *t) {
-  TileDesc &d = t->d;
+// Geometry of a tile over ext and the layout of its one device allocation:
+// codes, counts, offsets, scan partials, critical flags, owners, local
+// terminals, ring words.  Returns the allocation size; offsets in o[8].
+static size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[8]) {
   TileGeom &g = d.g;
-  g.vx = (t->ext[2] - t->ext[0]) / 2 + 1;
-  g.vy = (t->ext[3] - t->ext[1]) / 2 + 1;
+  g.vx = (ext[2] - ext[0]) / 2 + 1;
+  g.vy = (ext[3] - ext[1]) / 2 + 1;
   g.cw = 2 * g.vx - 1;
   g.ch = 2 * g.vy - 1;
-  g.rx0 = t->rect[0] - t->ext[0];
-  g.ry0 = t->rect[1] - t->ext[1];
-  g.rx1 = t->rect[2] - t->ext[0];
-  g.ry1 = t->rect[3] - t->ext[1];
-  d.ox = t->ext[0];
-  d.oy = t->ext[1];
+  g.rx0 = rect[0] - ext[0];
+  g.ry0 = rect[1] - ext[1];
+  g.rx1 = rect[2] - ext[0];
+  g.ry1 = rect[3] - ext[1];
+  d.ox = ext[0];
+  d.oy = ext[1];
   // one collation segment per block tile column: cw = 2 vx - 1 is odd, so
   // ceil(cw / SEG) == ceil(vx / GRAD_TX)
   d.nseg = (g.cw + SEG - 1) / SEG;
@@ -48,28 +50,46 @@ static void alloc_fixed(mscx_tile *t) {
   const int64_t nblk = (int64_t)d.nseg * ((g.vy + GRAD_TY - 1) / GRAD_TY);
   size_t off = 0;
   auto take = [&](size_t bytes) {
-    size_t o = off;
+    size_t at = off;
     off = align_up(off + bytes, 256);
-    return o;
+    return at;
   };
-  size_t o_codes = take((size_t)g.ch * d.pitch);
-  size_t o_cnt = take(nrs * 4), o_offs = take(nrs * 4), o_part = take(nparts * 4 + 4);
-  size_t o_bits = take(nrs * 16);
-  size_t o_own = take((size_t)g.ch * d.own_pitch * 4);
-  size_t o_loc = take((size_t)g.vy * d.loc_pitch * 4);
-  size_t o_ring = take((size_t)nblk * RING_W * 4), o_ringidx = take((size_t)nblk * RING_W * 4);
-  t->fixed_bytes = off;
-  t->fixed = device_pool().alloc(off);
+  o[0] = take((size_t)g.ch * d.pitch);
+  o[1] = take(nrs * 4);
+  o[2] = take(nrs * 4);
+  o[3] = take(nparts * 4 + 4);
+  o[4] = take(nrs * 16);
+  o[5] = take((size_t)g.ch * d.own_pitch * 4);
+  o[6] = take((size_t)g.vy * d.loc_pitch * 4);
+  o[7] = take((size_t)nblk * RING_W * 4);
+  size_t o_ringidx = take((size_t)nblk * RING_W * 4);
+  (void)o_ringidx;
+  return off;
+}
+
+size_t tile_fixed_bytes(const int32_t rect[4], const int32_t ext[4]) {
+  TileDesc d{};
+  size_t o[8];
+  return fixed_layout(rect, ext, d, o);
+}
+
+static void alloc_fixed(mscx_tile *t) {
+  TileDesc &d = t->d;
+  size_t o[8];
+  const size_t bytes = fixed_layout(t->rect, t->ext, d, o);
+  const int64_t nblk = (int64_t)d.nseg * ((d.g.vy + GRAD_TY - 1) / GRAD_TY);
+  t->fixed_bytes = bytes;
+  t->fixed = device_pool().alloc(bytes);
   char *base = (char *)t->fixed;
-  d.codes = (uint8_t *)(base + o_codes);
-  d.seg_counts = (uint32_t *)(base + o_cnt);
-  d.seg_offs = (uint32_t *)(base + o_offs);
-  d.scan_part = (uint32_t *)(base + o_part);
-  d.critbits = (uint64_t *)(base + o_bits);
-  d.own = (uint32_t *)(base + o_own);
-  d.loc = (uint32_t *)(base + o_loc);
-  d.ring = (uint32_t *)(base + o_ring);
-  d.ringidx = (uint32_t *)(base + o_ringidx);
+  d.codes = (uint8_t *)(base + o[0]);
+  d.seg_counts = (uint32_t *)(base + o[1]);
+  d.seg_offs = (uint32_t *)(base + o[2]);
+  d.scan_part = (uint32_t *)(base + o[3]);
+  d.critbits = (uint64_t *)(base + o[4]);
+  d.own = (uint32_t *)(base + o[5]);
+  d.loc = (uint32_t *)(base + o[6]);
+  d.ring = (uint32_t *)(base + o[7]);
+  d.ringidx = d.ring + nblk * RING_W;
 }
 
 CritLayout::CritLayout(size_t K) {

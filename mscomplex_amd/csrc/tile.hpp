@@ -31,6 +31,9 @@ namespace mscx {
 // launch per stage, then synchronises.  Sets K / worked on every tile.
 void run_tiles(mscx_tile *const *tiles, int n);
 
+// Size of the device allocation a tile over (rect, ext) makes at creation.
+size_t tile_fixed_bytes(const int32_t rect[4], const int32_t ext[4]);
+
 // Byte offsets of the crit / fn / conn / pair arrays for K critical cells.
 struct CritLayout {
   size_t fn, conn, pair, total;
