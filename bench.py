@@ -80,12 +80,147 @@ def cpu_baseline(data, budget_s=20.0):
                       f"manifold discs) on a {side}x{side} crop of the same field, {el:.1f}s single-threaded"}
 
 
+def batched_bench(n=1024, seed=7):
+    """The batched configuration: n grids with long sides log-uniform in
+    [64, 2048], aspect 1:1-1:4, half Gaussian sums and half value noise,
+    threshold 1%, inputs resident in HBM.  Batched = every grid's GPU stages
+    in one launch per stage and the host phases of all grids in parallel;
+    plain = one mscx_grid_run per grid.  Leaf tilings per size class come
+    from the autotuner (timed on a 64-grid sample, outside the timed runs)."""
+    import mscomplex_amd as mx
+    t0 = time.perf_counter()
+    mix = mx.uneven_mix(n, 64, 2048, seed, aspect_max=4.0, thresholds=0.01)
+    grids = [g for g, _ in mix]
+    # Gaussian sums (odd indices) at threshold 0: their +/-1e-3 noise makes
+    # the reference's simplification multiply strangulated arcs without
+    # bound at 1% (it fails only at the arc-entry cap, after minutes)
+    ths = [0.0 if i % 2 else t for i, (_, t) in enumerate(mix)]
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] batched: generated {n} grids in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sample = mx.uneven_mix(64, 64, 2048, seed + 1, aspect_max=4.0, thresholds=0.01)
+    policy, table = mx.autotune_tiling([g for g, _ in sample],
+                                       [0.0 if i % 2 else t for i, (_, t) in enumerate(sample)])
+    t_tune = time.perf_counter() - t0
+    tiling = [policy(g.shape[1], g.shape[0]) for g in grids]
+    bufs = [mx.DeviceBuffer.from_array(g) for g in grids]
+    dev = [(b.ptr, g.shape[1], g.shape[0]) for b, g in zip(bufs, grids)]
+    lx = [t[0] for t in tiling]
+    ly = [t[1] for t in tiling]
+    mx.Batch(dev[:8], lx[:8], ly[:8], ths[:8], device=True)  # warm the allocator caches
+    t0 = time.perf_counter()
+    b = mx.Batch(dev, lx, ly, ths, device=True)
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] batched: autotune {t_tune:.1f}s, batch {t_batch:.2f}s", file=sys.stderr, flush=True)
+    bad = [i for i in range(n) if b.status(i)[0] != 0]
+    del b
+    t0 = time.perf_counter()
+    for (p, sx, sy), a, c, th in zip(dev, lx, ly, ths):
+        r = mx.GridRun(None, levels=0, lx=a, ly=c, threshold=th, device_ptr=p, size=(sx, sy))
+        del r
+    t_plain = time.perf_counter() - t0
+    print(f"[bench] batched: plain {t_plain:.2f}s", file=sys.stderr, flush=True)
+    vert = sum(g.size for g in grids)
+    return {"grids": n, "vertices": int(vert), "batched_grids_per_s": round(n / t_batch, 2),
+            "plain_grids_per_s": round(n / t_plain, 2), "speedup": round(t_plain / t_batch, 2),
+            "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3), "failed": bad,
+            "tiling_policy": {"side<256": policy(255, 255), "side<1024": policy(1023, 1023),
+                              "side>=1024": policy(2048, 2048)},
+            "autotune_table": {f"{b}:{c}": round(v, 4) for (b, c), v in table.items()},
+            "autotune_s": round(t_tune, 2), "generate_s": round(t_gen, 2)}
+
+
+def served_bench(n=24, seed=11, repeat_frac=0.25, load=2.0, max_side=4096):
+    """The served configuration: square grids with sides log-uniform in
+    [128, max_side] (8192 in the full configuration), alternating fields,
+    value-noise thresholds uniform in [0, 5%] of the range (Gaussian sums at
+    0, as in the batched run), a quarter of the requests repeating an
+    earlier grid at a new threshold.  Plain = mscx_grid_run per request, one at a time (no
+    residency: repeats recompute).  Served = Poisson arrivals at `load` x the
+    plain throughput into the continuous-batching server (200 GB HBM budget,
+    grids resident under their key for the repeats)."""
+    import threading
+    import mscomplex_amd as mx
+    r = np.random.default_rng(seed)
+    base = []
+    reqs = []  # (grid index, threshold)
+    for i in range(n):
+        if base and r.random() < repeat_frac:
+            gi = int(r.integers(len(base)))
+            reqs.append((gi, 0.0 if gi % 2 else float(r.uniform(0, 0.05))))
+            continue
+        side = int(round(np.exp(r.uniform(np.log(128), np.log(max_side)))))
+        s = int(r.integers(1 << 62))
+        g = mx.gen_gaussians(side, side, 32, 1e-3, seed=s) if len(base) % 2 else \
+            mx.gen_value_noise(side, side, seed=s, period=side / 4)
+        base.append(g)
+        reqs.append((len(base) - 1, 0.0 if (len(base) - 1) % 2 else float(r.uniform(0, 0.05))))
+    tiling = [(0, 0) if max(g.shape) < 1024 else (1, 1) if max(g.shape) < 4096 else (2, 2) for g in base]
+
+    plain_lat = []
+    t0 = time.perf_counter()
+    for gi, th in reqs:
+        a = time.perf_counter()
+        lx, ly = tiling[gi]
+        run = mx.GridRun(base[gi], levels=0, lx=lx, ly=ly, threshold=th)
+        del run
+        plain_lat.append(time.perf_counter() - a)
+    t_plain = time.perf_counter() - t0
+    print(f"[bench] served: plain {t_plain:.2f}s for {n} requests", file=sys.stderr, flush=True)
+    rate = load * n / t_plain
+    gaps = r.exponential(1.0 / rate, size=n)
+
+    srv = mx.Server(200 << 30)
+    tickets = [None] * n
+    submitted = {}
+
+    def client():
+        t_next = time.perf_counter()
+        for k, (gi, th) in enumerate(reqs):
+            t_next += gaps[k]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            lx, ly = tiling[gi]
+            tickets[k] = srv.submit(base[gi], lx, ly, th, key=gi + 1)
+            submitted[gi] = True
+
+    t0 = time.perf_counter()
+    th_ = threading.Thread(target=client)
+    th_.start()
+    th_.join()
+    lat = []
+    failed = 0
+    for t in tickets:
+        st, err, l, _ = srv.wait(t, timeout=600)
+        failed += st != 0
+        lat.append(l)
+        srv.release(t)
+    t_served = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2)
+    return {"requests": n, "distinct_grids": len(base), "repeats": n - len(base),
+            "vertices_mean": int(np.mean([base[gi].size for gi, _ in reqs])),
+            "offered_rate_per_s": round(rate, 3),
+            "served_throughput_per_s": round(n / t_served, 3),
+            "served_latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
+            "plain_throughput_per_s": round(n / t_plain, 3),
+            "plain_latency_ms": {"p50": pct(plain_lat, 50), "p95": pct(plain_lat, 95), "p99": pct(plain_lat, 99)},
+            "rounds": stats["rounds"], "failed": int(failed), "hbm_budget_GB": 200, "max_side": max_side,
+            "load_x_plain": load}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-max-side", type=int, default=4096)
+    ap.add_argument("--batched-grids", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is single-GPU "
@@ -161,6 +296,11 @@ def main():
         "phase_ms": dict(zip(("compute", "merge_up", "simplify", "merge_down", "discs"),
                              [round(1000 * v / args.steps, 3) for v in host_t])),
     }
+    del last
+    if not args.no_batched:
+        line["batched"] = batched_bench(n=args.batched_grids)
+    if not args.no_served:
+        line["served"] = served_bench(max_side=args.served_max_side)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)
     print(json.dumps(line), flush=True)

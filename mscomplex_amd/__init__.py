@@ -442,3 +442,61 @@ class GridRun:
         t = np.zeros(5, np.float64)
         check(lib().mscx_grid_times(self._h, ptr(t)))
         return t
+
+
+# ------------------------------------------------------------- workloads
+
+def uneven_mix(n, side_lo, side_hi, seed, aspect_max=4.0, thresholds=None):
+    """n grids with sides log-uniform in [side_lo, side_hi] (the long side),
+    aspect ratio uniform in [1, aspect_max] with random orientation, half
+    Gaussian sums (32 Gaussians, +/-1e-3 noise) and half 8-octave value
+    noise; returns a list of (array, threshold)."""
+    r = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(round(np.exp(r.uniform(np.log(side_lo), np.log(side_hi)))))
+        short = max(2, int(round(side / r.uniform(1.0, aspect_max))))
+        sx, sy = (side, short) if r.random() < 0.5 else (short, side)
+        s = int(r.integers(1 << 62))
+        g = gen_gaussians(sx, sy, 32, 1e-3, seed=s) if i % 2 else gen_value_noise(sx, sy, seed=s, period=max(sx, sy) / 4)
+        th = float(thresholds(r)) if callable(thresholds) else float(thresholds or 0.0)
+        out.append((g, th))
+    return out
+
+
+def autotune_tiling(grids, thresholds, candidates=((0, 0), (1, 1), (2, 2)), classes=(256, 1024, 1 << 30),
+                    reps=1):
+    """Leaf tiling (lx, ly) per size class for batched runs, chosen by timing.
+
+    For each size class (grids whose larger side is below the class bound),
+    every candidate tiling runs the class's grids as one batch and the
+    fastest wins.  Returns (policy, table): policy(sx, sy) -> (lx, ly), and
+    the measured seconds per (class bound, candidate)."""
+    import time as _time
+    table = {}
+    best = {}
+    for bound in classes:
+        members = [(g, t) for g, t in zip(grids, thresholds)
+                   if max(g.shape) < bound and all(max(g.shape) >= b for b in classes if b < bound)]
+        if not members:
+            continue
+        res = []
+        for lx, ly in candidates:
+            if any(min(g.shape) < 8 * (1 << max(lx, ly)) for g, _ in members):
+                continue
+            el = float("inf")
+            for _ in range(reps):
+                t0 = _time.perf_counter()
+                Batch([g for g, _ in members], lx, ly, [t for _, t in members])
+                el = min(el, _time.perf_counter() - t0)
+            table[(bound, (lx, ly))] = el
+            res.append((el, (lx, ly)))
+        best[bound] = min(res)[1] if res else (0, 0)
+
+    def policy(sx, sy):
+        for bound in classes:
+            if max(sx, sy) < bound:
+                return best.get(bound, (0, 0))
+        return (0, 0)
+
+    return policy, table
