@@ -435,46 +435,86 @@ MSCX_HD int ecand_both(const KeyWin &K, int x, int y) {
              : DIR_NX;
 }
 
+// Candidates at the window's centre (i, j) = (K.i, K.j): the vertex and
+// the vertical edge (i, j)-(i, j+1).
+template <bool INTERIOR>
+MSCX_HD int vc_center(const KeyWin &K, const TileGeom &g) {
+  const VClass c0 = vclass(K.i, g.vx, g.rx0, g.rx1, g.cw - 1), r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1);
+  return vcand_at<INTERIOR>(K.w11, K.w10, K.w01, K.w21, K.w12, c0, r0);
+}
+template <bool INTERIOR>
+MSCX_HD int ec_center(const KeyWin &K, const TileGeom &g) {
+  int e = ecand_keys(K.w11, K.w21, K.w12, K.w22, K.w10, K.w20);
+  if (!INTERIOR) {
+    const VClass c0 = vclass(K.i, g.vx, g.rx0, g.rx1, g.cw - 1);
+    const VClass r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(K.j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
+    if (!(c0.in && r1.in && c0.rv && r0.re && c0.lo && c0.hi)) e = DIR_NONE;
+  }
+  if (e == CAND_BOTH) e = ecand_both(K, K.i, K.j);
+  return e;
+}
+
+// ... and the horizontal edge (i, j)-(i+1, j).
+template <bool INTERIOR>
+MSCX_HD int hc_center(const KeyWin &K, const TileGeom &g) {
+  int h = hcand_keys(K.w11, K.w12, K.w21, K.w22, K.w01, K.w02);
+  if (!INTERIOR) {
+    const VClass c0 = vclass(K.i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(K.i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
+    const VClass r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1);
+    if (!(c1.in && r0.in && c0.re && r0.rv && r0.lo && r0.hi)) h = DIR_NONE;
+  }
+  if (h == CAND_BOTH) h = hcand_both(K, K.i, K.j);
+  return h;
+}
+
 // Candidates of the first row of a walk: vertices (i, j), (i+1, j) and the
 // edge (i, j)-(i+1, j).
 template <bool INTERIOR>
 MSCX_HD void row_init(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, int &hc00) {
-  const int i = K.i, j = K.j;
-  const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
-  const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1);
-  vc00 = vcand_at<INTERIOR>(K.w11, K.w10, K.w01, K.w21, K.w12, c0, r0);
+  const VClass c1 = vclass(K.i + 1, g.vx, g.rx0, g.rx1, g.cw - 1), r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1);
+  vc00 = vc_center<INTERIOR>(K, g);
   vc10 = vcand_at<INTERIOR>(K.w12, K.w11, K.w02, K.w22, K.w13, c1, r0);
-  int h = hcand_keys(K.w11, K.w12, K.w21, K.w22, K.w01, K.w02);
-  if (!INTERIOR && !(c1.in && r0.in && c0.re && r0.rv && r0.lo && r0.hi)) h = DIR_NONE;
-  if (h == CAND_BOTH) h = hcand_both(K, i, j);
-  hc00 = h;
+  hc00 = hc_center<INTERIOR>(K, g);
 }
 
-// One row: the codes of cells (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1)
-// (code_from_cands), and the carried candidates advanced to row j + 1.
-// Cells outside the ext get code 0.
+// New candidates of a row step in the lane's own column: vertex (i, j+1),
+// horizontal edge (i, j+1)-(i+1, j+1), vertical edge (i, j)-(i, j+1).
 template <bool INTERIOR>
-MSCX_HD void row_codes(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, int &hc00, uint8_t &cv, uint8_t &chh,
-                       uint8_t &ce, uint8_t &cq) {
+MSCX_HD void row_own(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
+  const int i = K.i, j = K.j;
+  const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
+  const VClass r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
+  vc01 = vcand_at<INTERIOR>(K.w21, K.w20, K.w11, K.w31, K.w22, c0, r1);
+  int h = hcand_keys(K.w21, K.w22, K.w31, K.w32, K.w11, K.w12);
+  if (!INTERIOR && !(c1.in && r1.in && c0.re && r1.rv && r1.lo && r1.hi)) h = DIR_NONE;
+  if (h == CAND_BOTH) h = hcand_both(K, i, j + 1);
+  hc01 = h;
+  ec00 = ec_center<INTERIOR>(K, g);
+}
+
+// The same step's candidates of the next column (vertex (i+1, j+1), edge
+// (i+1, j)-(i+1, j+1)), which the kernel takes from the neighbouring lane.
+template <bool INTERIOR>
+MSCX_HD void row_next(const KeyWin &K, const TileGeom &g, int &vc11, int &ec10) {
+  const int i = K.i, j = K.j;
+  const VClass c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
+  const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
+  vc11 = vcand_at<INTERIOR>(K.w22, K.w21, K.w12, K.w32, K.w23, c1, r1);
+  int e = ecand_keys(K.w12, K.w22, K.w13, K.w23, K.w11, K.w21);
+  if (!INTERIOR && !(c1.in && r1.in && c1.rv && r0.re && c1.lo && c1.hi)) e = DIR_NONE;
+  if (e == CAND_BOTH) e = ecand_both(K, i + 1, j);
+  ec10 = e;
+}
+
+// The four codes of cells (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1)
+// (code_from_cands) from the seven candidates they depend on.  Cells
+// outside the ext get code 0.
+template <bool INTERIOR>
+MSCX_HD void row_cell_codes(const KeyWin &K, const TileGeom &g, int vc00, int vc10, int hc00, int vc01, int hc01,
+                            int ec00, int ec10, uint8_t &cv, uint8_t &chh, uint8_t &ce, uint8_t &cq) {
   const int i = K.i, j = K.j;
   const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
   const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
-  // new candidates: vertices (i, j+1), (i+1, j+1), edges (i, j+1)-(i+1, j+1),
-  // (i, j)-(i, j+1) and (i+1, j)-(i+1, j+1)
-  const int vc01 = vcand_at<INTERIOR>(K.w21, K.w20, K.w11, K.w31, K.w22, c0, r1);
-  const int vc11 = vcand_at<INTERIOR>(K.w22, K.w21, K.w12, K.w32, K.w23, c1, r1);
-  int hc01 = hcand_keys(K.w21, K.w22, K.w31, K.w32, K.w11, K.w12);
-  int ec00 = ecand_keys(K.w11, K.w21, K.w12, K.w22, K.w10, K.w20);
-  int ec10 = ecand_keys(K.w12, K.w22, K.w13, K.w23, K.w11, K.w21);
-  if (!INTERIOR) {
-    if (!(c1.in && r1.in && c0.re && r1.rv && r1.lo && r1.hi)) hc01 = DIR_NONE;
-    if (!(c0.in && r1.in && c0.rv && r0.re && c0.lo && c0.hi)) ec00 = DIR_NONE;
-    if (!(c1.in && r1.in && c1.rv && r0.re && c1.lo && c1.hi)) ec10 = DIR_NONE;
-  }
-  if (hc01 == CAND_BOTH) hc01 = hcand_both(K, i, j + 1);
-  if (ec00 == CAND_BOTH) ec00 = ecand_both(K, i, j);
-  if (ec10 == CAND_BOTH) ec10 = ecand_both(K, i + 1, j);
-
   // codes (code_from_cands)
   int dv = vc00;
   int dh = hc00;
@@ -546,6 +586,17 @@ MSCX_HD void row_codes(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10,
     if (!(c0.in && r1.in)) ce = 0;
     if (!(c1.in && r1.in)) cq = 0;
   }
+}
+
+// One full row step as the CPU checks run it: own and next-column
+// candidates, the four codes, and the carried candidates advanced to row j+1.
+template <bool INTERIOR>
+MSCX_HD void row_codes(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, int &hc00, uint8_t &cv, uint8_t &chh,
+                       uint8_t &ce, uint8_t &cq) {
+  int vc01, hc01, ec00, vc11, ec10;
+  row_own<INTERIOR>(K, g, vc01, hc01, ec00);
+  row_next<INTERIOR>(K, g, vc11, ec10);
+  row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
   vc00 = vc01;
   vc10 = vc11;
   hc00 = hc01;

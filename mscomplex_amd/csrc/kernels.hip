@@ -47,10 +47,13 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 //     terminals of the tile's perimeter entries go to its ring words, in the
 //     form the ring resolution chases.
 
-// trace link of a vertex (q = 0) or quad (q = 1) entry from its code
-__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q) {
+// trace link of a vertex (q = 0) or quad (q = 1) entry from its code; a
+// critical entry becomes its own root terminal, recorded as its cell row in
+// the block (2 lj + q) and its rank among the critical cells of that row's
+// segment
+__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY;
-  if (c & CF_CRITICAL) return T_ROOT | (uint32_t)(lj * TX + li);
+  if (c & CF_CRITICAL) return T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
   if (!(c & CF_PAIRED)) return T_NONE;
   const int d = code_dir(c);
   const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
@@ -61,12 +64,33 @@ __device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q) {
 }
 
 // Step 2 for one wave: rows lj0 .. lj0 + GRAD_TY/4 - 1 of the block tile,
-// with the row-step rules (row_init / row_codes) on a register window.
+// with the row-step rules on a register window.  A lane computes the new
+// candidates of its own column (row_own) and takes the next column's from
+// the neighbouring lane; lane 63's next column lies outside the wave, so its
+// candidates for all the wave's rows are computed once up front, one row per
+// lane, and read with readlane.
 template <bool INTERIOR>
 __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, const uint32_t *__restrict__ sk,
                                           uint16_t *__restrict__ sl, int i0, int j0, int bx, int lane, int lj0) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, NV = TX * TY, RW = TY / 4;
   const int li = lane, i = i0 + li;
+  // column i0 + TX: vertex candidate at row j0 + lj0 + L and vertical-edge
+  // candidate at the same row, lane L in 0..RW (keys from LDS rows
+  // lj0+L .. lj0+L+2, columns TX .. TX+2)
+  int xv, xe;
+  {
+    const int L = lane < RW ? lane : RW;
+    const uint32_t *s0 = sk + (lj0 + L) * KW + TX;
+    KeyWin P;
+    P.w00 = s0[0], P.w01 = s0[1], P.w02 = s0[2], P.w03 = 0;
+    P.w10 = s0[KW], P.w11 = s0[KW + 1], P.w12 = s0[KW + 2], P.w13 = 0;
+    P.w20 = s0[2 * KW], P.w21 = s0[2 * KW + 1], P.w22 = s0[2 * KW + 2], P.w23 = 0;
+    P.w30 = P.w31 = P.w32 = P.w33 = 0;
+    P.i = i0 + TX;
+    P.j = j0 + lj0 + L;
+    xv = vc_center<INTERIOR>(P, g);
+    xe = ec_center<INTERIOR>(P, g);
+  }
   KeyWin K;
   {
     const uint32_t *s0 = sk + lj0 * KW + li;
@@ -77,13 +101,24 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   }
   K.i = i;
   K.j = j0 + lj0;
-  int vc00, vc10, hc00;
-  row_init<INTERIOR>(K, g, vc00, vc10, hc00);
+  int vc00 = vc_center<INTERIOR>(K, g);
+  int hc00 = hc_center<INTERIOR>(K, g);
+  int vc10 = __shfl_down(vc00, 1, 64);
+  if (lane == 63) vc10 = __builtin_amdgcn_readlane(xv, 0);
+  const uint64_t below = lanes_below(lane);
   for (int rr = 0; rr < RW; ++rr) {
     const int lj = lj0 + rr, j = K.j;
     if (j >= g.vy) break;  // uniform across the wave
+    int vc01, hc01, ec00;
+    row_own<INTERIOR>(K, g, vc01, hc01, ec00);
+    int vc11 = __shfl_down(vc01, 1, 64), ec10 = __shfl_down(ec00, 1, 64);
+    const int xv1 = __builtin_amdgcn_readlane(xv, rr + 1), xe0 = __builtin_amdgcn_readlane(xe, rr);
+    if (lane == 63) {
+      vc11 = xv1;
+      ec10 = xe0;
+    }
     uint8_t cv, chh, ce, cq;
-    row_codes<INTERIOR>(K, g, vc00, vc10, hc00, cv, chh, ce, cq);
+    row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
     const bool ein = INTERIOR || j + 1 < g.vy;
     uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
     *(uint16_t *)crow = (uint16_t)(cv | (chh << 8));
@@ -94,6 +129,7 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
       T.critbits[2 * rs0 + 1] = b1;
       T.seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
     }
+    uint32_t rank_q = 0;
     if (ein) {
       *(uint16_t *)(crow + T.pitch) = (uint16_t)(ce | (cq << 8));
       const int64_t rs1 = rs0 + T.nseg;
@@ -103,10 +139,15 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
         T.critbits[2 * rs1 + 1] = b3;
         T.seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
       }
+      rank_q = (uint32_t)(__popcll(b2 & below) + __popcll(b3 & below)) + (uint32_t)((b2 >> lane) & 1u);
     }
+    const uint32_t rank_v = (uint32_t)(__popcll(b0 & below) + __popcll(b1 & below));
     // codes outside the ext are 0, so those entries get T_NONE
-    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0);
-    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1);
+    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
+    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
+    vc00 = vc01;
+    vc10 = vc11;
+    hc00 = hc01;
     if (rr + 1 < RW) {
       const uint32_t *sn = sk + (lj + 4) * KW + li;
       K.slide(sn[0], sn[1], sn[2], sn[3]);
@@ -229,8 +270,9 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
       const uint32_t w = sl[q * NV + lj * TX + li];
       const uint32_t kind = term_kind(w);
       if (kind == 0) {
-        const int e = (int)(w & 0x7FFu);
-        out = pack_cell(2 * (i0 + e % TX) + q, 2 * (j0 + e / TX) + q);
+        // root: ext-local cell row, segment (= block column), rank in the segment
+        const uint32_t y = (uint32_t)(2 * j0) + ((w >> 7) & 63u);
+        out = (y << 15) | ((uint32_t)bx << 7) | (w & 127u);
       } else if (kind == 1) {
         const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
         int tbx = bx, tby = by, tli, tlj;
@@ -250,7 +292,8 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
 }
 
 // Ring resolution: every perimeter entry follows ring words (one per block
-// tile crossed) to its root and stores the root's critical index.
+// tile crossed) to its root and stores the root's critical index (its
+// segment's scanned offset plus its rank there).
 __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                       int ntiles, uint32_t *err) {
   const int b = blockIdx.x;
@@ -271,22 +314,18 @@ __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict
     w = T.ring[w & ~RING_MARK];
   }
   uint32_t out = OWN_NONE;
-  if (w != OWN_NONE) {
-    const int x = (int)(w & 0xFFFFu), y = (int)(w >> 16);
-    const int64_t rs = (int64_t)y * T.nseg + (x / SEG);
-    out = crit_rank(T.seg_offs[rs], T.critbits[2 * rs], T.critbits[2 * rs + 1], x);
-  }
+  if (w != OWN_NONE) out = T.seg_offs[(int64_t)(w >> 15) * T.nseg + ((w >> 7) & 255u)] + (w & 127u);
   T.ringidx[e] = out;
 }
 
 // Owner images: the terminals of the block's entries become critical indices
-// (a root by its rank in the scanned critical flags, an exit by the resolved
-// ring word it leads to); an edge takes the owner of the vertex or quad it
-// is paired with, a critical cell owns itself.
+// (a root is its segment row's scanned offset plus its rank there, an exit
+// the resolved ring word it leads to); an edge takes the owner of the vertex
+// or quad it is paired with, a critical cell owns itself (its rank comes from
+// ballots over the codes of its row).
 __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                int ntiles) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY, RW = TY / 4;
-  __shared__ uint64_t sb[2 * TY * 2];
   __shared__ uint32_t so[2 * TY];
   __shared__ uint32_t dv[NV], dq[NV];
 
@@ -302,15 +341,9 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
 
   for (int r = tid; r < 2 * TY; r += 256) {
     const int cy = 2 * j0 + r;
-    if (cy < g.ch) {
-      const int64_t rs = (int64_t)cy * T.nseg + bx;
-      sb[2 * r] = T.critbits[2 * rs];
-      sb[2 * r + 1] = T.critbits[2 * rs + 1];
-      so[r] = T.seg_offs[rs];
-    }
+    if (cy < g.ch) so[r] = T.seg_offs[(int64_t)cy * T.nseg + bx];
   }
   __syncthreads();
-  auto rank_local = [&](int lx, int ly) { return crit_rank(so[ly], sb[2 * ly], sb[2 * ly + 1], lx); };
   auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
     // the neighbour always exists for a valid gradient (paths stay in ext)
     if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return OWN_NONE;
@@ -318,10 +351,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   };
   auto decode = [&](uint32_t w, int q) -> uint32_t {
     const uint32_t kind = term_kind(w);
-    if (kind == 0) {
-      const int e = (int)(w & 0x7FFu);
-      return rank_local(2 * (e % TX) + q, 2 * (e / TX) + q);
-    }
+    if (kind == 0) return so[(w >> 7) & 63u] + (w & 127u);
     if (kind == 1) {
       const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
       if (s == 0) return ring_val(bx + 1, by, q, 0, pos);
@@ -345,19 +375,22 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   }
   __syncthreads();
 
+  const uint64_t below = lanes_below(lane);
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r, li = lane;
     const int i = i0 + li, j = j0 + lj;
-    if (j >= g.vy) break;
+    if (j >= g.vy) break;  // uniform across the wave
     const bool ein = j + 1 < g.vy;
     const uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
     const uint16_t w0 = *(const uint16_t *)crow;
     const uint16_t w1 = ein ? *(const uint16_t *)(crow + T.pitch) : (uint16_t)0;
-    const uint8_t chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1;
+    const uint8_t cv = (uint8_t)w0, chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1, cq = (uint8_t)(w1 >> 8);
+    const uint64_t bv = __ballot(cv & CF_CRITICAL), bh = __ballot(chh & CF_CRITICAL);
+    const uint64_t be = __ballot(ce & CF_CRITICAL), bq = __ballot(cq & CF_CRITICAL);
     const uint32_t ov = dv[lj * TX + li], oq = dq[lj * TX + li];
     uint32_t oh = OWN_NONE, oe = OWN_NONE;
     if (chh & CF_CRITICAL) {
-      oh = rank_local(2 * li + 1, 2 * lj);
+      oh = so[2 * lj] + (uint32_t)(__popcll(bv & below) + __popcll(bh & below)) + (uint32_t)((bv >> lane) & 1u);
     } else if (chh & CF_PAIRED) {
       const int d = code_dir(chh);
       if (d == DIR_PX) oh = li + 1 < TX ? dv[lj * TX + li + 1] : ring_val(bx + 1, by, 0, 0, lj);
@@ -366,7 +399,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
       else oh = lj > 0 ? dq[(lj - 1) * TX + li] : ring_val(bx, by - 1, 1, li, TY - 1);
     }
     if (ce & CF_CRITICAL) {
-      oe = rank_local(2 * li, 2 * lj + 1);
+      oe = so[2 * lj + 1] + (uint32_t)(__popcll(be & below) + __popcll(bq & below));
     } else if (ce & CF_PAIRED) {
       const int d = code_dir(ce);
       if (d == DIR_PY) oe = lj + 1 < TY ? dv[(lj + 1) * TX + li] : ring_val(bx, by + 1, 0, li, 0);
@@ -513,11 +546,19 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ ti
   const int64_t n = (int64_t)T.g.ch * T.nseg;
   const int64_t first = ((int64_t)(b - prefix[t]) * 4 + (threadIdx.x >> 6)) * COMPACT_ITEMS;
   const uint64_t lt = lanes_below(lane);
+  // the flags of all items first, so their loads are in flight together
+  uint64_t fw[2 * COMPACT_ITEMS];
+#pragma unroll
   for (int it = 0; it < COMPACT_ITEMS; ++it) {
-    const int64_t item = first + it;
-    if (item >= n) break;
-    const uint64_t w0 = T.critbits[2 * item], w1 = T.critbits[2 * item + 1];
+    const bool in = first + it < n;
+    fw[2 * it] = in ? T.critbits[2 * (first + it)] : 0ull;
+    fw[2 * it + 1] = in ? T.critbits[2 * (first + it) + 1] : 0ull;
+  }
+#pragma unroll
+  for (int it = 0; it < COMPACT_ITEMS; ++it) {
+    const uint64_t w0 = fw[2 * it], w1 = fw[2 * it + 1];
     if (!(w0 | w1)) continue;
+    const int64_t item = first + it;
     const int row = (int)(item / T.nseg), seg = (int)(item % T.nseg);
     const uint32_t c0 = (uint32_t)(w0 >> lane) & 1u, c1 = (uint32_t)(w1 >> lane) & 1u;
     uint32_t k = T.seg_offs[item] + (uint32_t)(__popcll(w0 & lt) + __popcll(w1 & lt));

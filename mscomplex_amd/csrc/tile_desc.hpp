@@ -46,7 +46,8 @@ MSCX_HD void ring_at(int p, int &li, int &lj) {
 
 // Local terminal of a vertex or quad entry after the block-local trace
 // (16 bits; in LDS an entry with bit 15 clear is a link to another entry):
-//   0x8000 | e          root: the critical entry e (li + lj*TX) of the same forest
+//   0x8000 | r<<7 | k   root: the critical cell in cell row r (0..63) of the
+//                       block, k-th critical cell of that row's segment
 //   0xA000 | s<<6 | p   exit: the path leaves through side s (0 +x, 1 -x,
 //                       2 +y, 3 -y) at position p along that side
 //   0xC000              none: the cell has no owner
@@ -75,9 +76,10 @@ struct TileDesc {
   // local terminals: per vertex position, vertex terminal | quad terminal << 16
   uint32_t *loc;
   int loc_pitch;  // words per vertex row (nseg * GRAD_TX)
-  // ring words per block (RING_W each): after the gradient kernel an
-  // ext-local packed root cell, OWN_NONE or RING_MARK | ring index of the
-  // entry the path continues at; ringidx holds the resolved critical index
+  // ring words per block (RING_W each): after the gradient kernel a root
+  // (ext-local cell row << 15 | segment << 7 | rank in the segment), OWN_NONE
+  // or RING_MARK | ring index of the entry the path continues at; ringidx
+  // holds the resolved critical index
   uint32_t *ring;
   uint32_t *ringidx;
   // critical list (raster order), their function values, saddle incidence
@@ -95,7 +97,7 @@ struct TileDesc {
 };
 
 constexpr uint32_t OWN_NONE = 0xFFFFFFFFu;   // (-1,-1) / no owner
-constexpr uint32_t RING_MARK = 0x80000000u;  // ext-local cell ids never have bit 31 set
+constexpr uint32_t RING_MARK = 0x80000000u;  // root words never have bit 31 set
 
 // Critical index of ext-local cell (x, y), known to be critical: the scanned
 // offset of its (row, segment) plus the critical cells before it there.
