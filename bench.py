@@ -80,6 +80,53 @@ def cpu_baseline(data, budget_s=20.0):
                       f"manifold discs) on a {side}x{side} crop of the same field, {el:.1f}s single-threaded"}
 
 
+def worstcase_bench(reps=3):
+    """The worst-case configuration's GPU tile stages: 16384x16384 white
+    noise U[0,1) and its 256-level quantisation, 8x8 tiling, inputs
+    resident in HBM; every leaf tile through gradient, collation, trace and
+    incidence in one launch per stage (mscx_tiles_work).  The host merge
+    tree at ~2e8 critical points is not run here (its host phases alone
+    would take many minutes)."""
+    import mscomplex_amd as mx
+    side = 16384
+    out = {}
+    for levels in (0, 256):
+        data = mx.gen_white(side, side, levels=levels, seed=4242 + levels)
+        buf = mx.DeviceBuffer.from_array(data)
+        pieces = split_pieces(side, side, 3, 3)
+        tiles = []
+        for rect, ext in pieces:
+            t = mx.GridDataset(rect, ext)
+            t.bind_device(buf.ptr + 4 * ((ext[1] // 2) * side + ext[0] // 2), side)
+            tiles.append(t)
+        mx.work_tiles(tiles)
+        mx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mx.work_tiles(tiles)
+        el = (time.perf_counter() - t0) / reps
+        prof = mx.profile_read()
+        mx.profile(False)
+        K = sum(t.num_critical() for t in tiles)
+        geo = [((e[2] - e[0]) // 2 + 1, (e[3] - e[1]) // 2 + 1) for _, e in pieces]
+        nm = sum(vx * vy for vx, vy in geo)
+        cells = sum((2 * vx - 1) * (2 * vy - 1) for vx, vy in geo)
+        per = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
+        g_ms = per["gradient"]
+        gt_ms = per["gradient"] + per["trace"] + per["owner"]
+        gt_bytes = 4 * nm + 6 * cells
+        out["white" if levels == 0 else f"quantized{levels}"] = {
+            "critical_cells": int(K), "tile_stages_ms": round(1e3 * el, 3),
+            "tile_stages_Mcells_per_s": round(side * side / 1e6 / el, 1),
+            "stage_ms": {k: round(v, 4) for k, v in per.items()},
+            "k_grad_local_GBs": round((4 * nm + cells) / (g_ms * 1e-3) / 1e9, 1),
+            "gradient_trace_GBs": round(gt_bytes / (gt_ms * 1e-3) / 1e9, 1),
+            "gradient_trace_frac": round(gt_bytes / (gt_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        print(f"[bench] worst case levels={levels}: {1e3 * el:.2f} ms, K={K}", file=sys.stderr, flush=True)
+        del tiles, buf, data
+    return out
+
+
 def batched_bench(n=1024, seed=7):
     """The batched configuration: n grids with long sides log-uniform in
     [64, 2048], aspect 1:1-1:4, half Gaussian sums and half value noise,
@@ -219,6 +266,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-worst", action="store_true")
     ap.add_argument("--served-max-side", type=int, default=4096)
     ap.add_argument("--batched-grids", type=int, default=1024)
     args = ap.parse_args()
@@ -297,6 +345,8 @@ def main():
                              [round(1000 * v / args.steps, 3) for v in host_t])),
     }
     del last
+    if not args.no_worst:
+        line["worst_case"] = worstcase_bench()
     if not args.no_batched:
         line["batched"] = batched_bench(n=args.batched_grids)
     if not args.no_served:

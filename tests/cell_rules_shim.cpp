@@ -116,3 +116,30 @@ extern "C" int cr_codes(const float *win, const int *rect, const int *ext, uint8
   }
   return 0;
 }
+
+// Codes of the cells [x0, x0+w) x [y0, y0+h) of a tile (ext-local cell
+// coordinates) from the per-cell rules, reading the tile's whole window.
+extern "C" int cr_codes_region(const float *win, const int *rect, const int *ext, int x0, int y0, int w, int h,
+                               uint8_t *codes) {
+  mscx::TileGeom g;
+  g.vx = (ext[2] - ext[0]) / 2 + 1;
+  g.vy = (ext[3] - ext[1]) / 2 + 1;
+  g.cw = 2 * g.vx - 1;
+  g.ch = 2 * g.vy - 1;
+  g.rx0 = rect[0] - ext[0];
+  g.ry0 = rect[1] - ext[1];
+  g.rx1 = rect[2] - ext[0];
+  g.ry1 = rect[3] - ext[1];
+  auto K = [&](int i, int j) -> uint64_t {
+    if (i < -1 || j < -1 || i > g.vx || j > g.vy) throw 1;
+    if (i < 0 || j < 0 || i >= g.vx || j >= g.vy) return 0x12345ull;
+    return mscx::vkey(mscx::fkey(win[(size_t)j * g.vx + i]), i, j);
+  };
+  try {
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) codes[(size_t)y * w + x] = mscx::cell_code(K, g, x0 + x, y0 + y);
+  } catch (int) {
+    return -1;
+  }
+  return 0;
+}

@@ -68,3 +68,27 @@ def rng_field(kind, sx, sy, seed):
     if kind == "denormal":
         return (r.random((sy, sx)) * 1e-39).astype(np.float32)
     raise ValueError(kind)
+
+
+def load_shim():
+    """Builds (once per process) and loads the test-only CPU build of the
+    gradient rules (tests/cell_rules_shim.cpp)."""
+    import ctypes
+    import os
+    import subprocess
+    global _SHIM
+    if _SHIM is None:
+        here = os.path.dirname(os.path.abspath(__file__))
+        d = os.path.join(here, "_shim")
+        os.makedirs(d, exist_ok=True)
+        so = os.path.join(d, "libcellrules.so")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", so,
+                        os.path.join(here, "cell_rules_shim.cpp")], check=True)
+        L = ctypes.CDLL(so)
+        L.cr_codes.argtypes = [ctypes.c_void_p] * 4
+        L.cr_codes_region.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [ctypes.c_void_p]
+        _SHIM = L
+    return _SHIM
+
+
+_SHIM = None

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.helpers import rng_field, split_pieces, window
+from tests.helpers import load_shim, rng_field, split_pieces, window
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SHIM_DIR = os.path.join(HERE, "_shim")
@@ -21,12 +21,7 @@ DIRS = {0: (0, 0), 1: (1, 0), 2: (-1, 0), 3: (0, 1), 4: (0, -1)}
 
 @pytest.fixture(scope="module")
 def shim():
-    os.makedirs(SHIM_DIR, exist_ok=True)
-    src = os.path.join(HERE, "cell_rules_shim.cpp")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", SHIM, src], check=True)
-    L = ctypes.CDLL(SHIM)
-    L.cr_codes.argtypes = [ctypes.c_void_p] * 4
-    return L
+    return load_shim()
 
 
 def codes_to_flags_pairs(codes, ext):
