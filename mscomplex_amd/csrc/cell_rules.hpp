@@ -478,18 +478,32 @@ MSCX_HD void row_init(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, 
 }
 
 // New candidates of a row step in the lane's own column: vertex (i, j+1),
-// horizontal edge (i, j+1)-(i+1, j+1), vertical edge (i, j)-(i, j+1).
+// horizontal edge (i, j+1)-(i+1, j+1), vertical edge (i, j)-(i, j+1).  The
+// edges may come back as CAND_BOTH (both cofacets usable); row_own_resolve
+// decides those with quad_lt.
 template <bool INTERIOR>
-MSCX_HD void row_own(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
+MSCX_HD void row_own_raw(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
   const int i = K.i, j = K.j;
   const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
-  const VClass r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
+  const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
   vc01 = vcand_at<INTERIOR>(K.w21, K.w20, K.w11, K.w31, K.w22, c0, r1);
   int h = hcand_keys(K.w21, K.w22, K.w31, K.w32, K.w11, K.w12);
-  if (!INTERIOR && !(c1.in && r1.in && c0.re && r1.rv && r1.lo && r1.hi)) h = DIR_NONE;
-  if (h == CAND_BOTH) h = hcand_both(K, i, j + 1);
+  int e = ecand_keys(K.w11, K.w21, K.w12, K.w22, K.w10, K.w20);
+  if (!INTERIOR) {
+    if (!(c1.in && r1.in && c0.re && r1.rv && r1.lo && r1.hi)) h = DIR_NONE;
+    if (!(c0.in && r1.in && c0.rv && r0.re && c0.lo && c0.hi)) e = DIR_NONE;
+  }
   hc01 = h;
-  ec00 = ec_center<INTERIOR>(K, g);
+  ec00 = e;
+}
+MSCX_HD void row_own_resolve(const KeyWin &K, int &hc01, int &ec00) {
+  if (hc01 == CAND_BOTH) hc01 = hcand_both(K, K.i, K.j + 1);
+  if (ec00 == CAND_BOTH) ec00 = ecand_both(K, K.i, K.j);
+}
+template <bool INTERIOR>
+MSCX_HD void row_own(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
+  row_own_raw<INTERIOR>(K, g, vc01, hc01, ec00);
+  row_own_resolve(K, hc01, ec00);
 }
 
 // The same step's candidates of the next column (vertex (i+1, j+1), edge

@@ -509,8 +509,10 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
 
   while (!pq.empty()) {
     uint64_t hp = pq.top();
-    pq.pop();
     const P2 cp = slots[ExactHeap::payload(hp)];
+    pq.pop();
+    // the next element's pair is a random access into a large array
+    if (!pq.empty()) __builtin_prefetch(&slots[ExactHeap::payload(pq.top())]);
     uint32_t v1 = cp.first, v2 = cp.second;
     float pers = std::abs(m.fn[v1] - m.fn[v2]);
     if ((double)pers / (double)max_persistence > thresh) break;

@@ -10,6 +10,20 @@
 
 namespace mscx {
 
+// Device pointers taken from a tile descriptor are generic; marking them as
+// global lets the compiler emit global (not flat) loads and stores.
+#define GLB __attribute__((address_space(1)))
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ GLB T *glb(T *p) {
+  return (GLB T *)p;
+}
+template <class T>
+__device__ __forceinline__ const GLB T *glb(const T *p) {
+  return (const GLB T *)p;
+}
+
 __device__ __forceinline__ int find_tile(const int *__restrict__ prefix, int ntiles, int b) {
   int lo = 0, hi = ntiles - 1;
   while (lo < hi) {
@@ -106,11 +120,19 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   int vc10 = __shfl_down(vc00, 1, 64);
   if (lane == 63) vc10 = __builtin_amdgcn_readlane(xv, 0);
   const uint64_t below = lanes_below(lane);
+  // this lane's two code bytes in cell row 2j, and this wave's critical
+  // words / counts of cell row 2j; both advance by two cell rows per step
+  const int64_t pitch = T.pitch, nseg = T.nseg;
+  GLB uint8_t *crow = glb(T.codes) + (int64_t)(2 * K.j) * pitch + 2 * i;
+  int64_t rs0 = (int64_t)(2 * K.j) * nseg + bx;
+  GLB uint64_t *const critbits = glb(T.critbits);
+  GLB uint32_t *const seg_counts = glb(T.seg_counts);
   for (int rr = 0; rr < RW; ++rr) {
     const int lj = lj0 + rr, j = K.j;
     if (j >= g.vy) break;  // uniform across the wave
     int vc01, hc01, ec00;
-    row_own<INTERIOR>(K, g, vc01, hc01, ec00);
+    row_own_raw<INTERIOR>(K, g, vc01, hc01, ec00);
+    if (hc01 == CAND_BOTH || ec00 == CAND_BOTH) row_own_resolve(K, hc01, ec00);
     int vc11 = __shfl_down(vc01, 1, 64), ec10 = __shfl_down(ec00, 1, 64);
     const int xv1 = __builtin_amdgcn_readlane(xv, rr + 1), xe0 = __builtin_amdgcn_readlane(xe, rr);
     if (lane == 63) {
@@ -120,27 +142,25 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
     uint8_t cv, chh, ce, cq;
     row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
     const bool ein = INTERIOR || j + 1 < g.vy;
-    uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
-    *(uint16_t *)crow = (uint16_t)(cv | (chh << 8));
-    const int64_t rs0 = (int64_t)(2 * j) * T.nseg + bx;
+    *(GLB uint16_t *)crow = (uint16_t)(cv | (chh << 8));
     const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
     if (lane == 0) {
-      T.critbits[2 * rs0] = b0;
-      T.critbits[2 * rs0 + 1] = b1;
-      T.seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
+      *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
+      seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
     }
     uint32_t rank_q = 0;
     if (ein) {
-      *(uint16_t *)(crow + T.pitch) = (uint16_t)(ce | (cq << 8));
-      const int64_t rs1 = rs0 + T.nseg;
+      *(GLB uint16_t *)(crow + pitch) = (uint16_t)(ce | (cq << 8));
+      const int64_t rs1 = rs0 + nseg;
       const uint64_t b2 = __ballot(ce & CF_CRITICAL), b3 = __ballot(cq & CF_CRITICAL);
       if (lane == 0) {
-        T.critbits[2 * rs1] = b2;
-        T.critbits[2 * rs1 + 1] = b3;
-        T.seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
+        *(GLB u64x2 *)(critbits + 2 * rs1) = u64x2{b2, b3};
+        seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
       }
       rank_q = (uint32_t)(__popcll(b2 & below) + __popcll(b3 & below)) + (uint32_t)((b2 >> lane) & 1u);
     }
+    crow += 2 * pitch;
+    rs0 += 2 * nseg;
     const uint32_t rank_v = (uint32_t)(__popcll(b0 & below) + __popcll(b1 & below));
     // codes outside the ext are 0, so those entries get T_NONE
     sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
@@ -168,7 +188,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const TileGeom g = T.g;
   const int lb = b - prefix[t];
   const int nbx = T.nseg;
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict
                                                       int ntiles, uint32_t *err) {
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const int nblk = T.nseg * ((T.g.vy + GRAD_TY - 1) / GRAD_TY);
   const int64_t e = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
   if (e >= (int64_t)nblk * RING_W) return;
@@ -331,7 +351,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const TileGeom g = T.g;
   const int lb = b - prefix[t];
   const int nbx = T.nseg;
@@ -339,15 +359,20 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   const int i0 = bx * TX, j0 = by * TY;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
+  const int64_t pitch = T.pitch, own_pitch = T.own_pitch, loc_pitch = T.loc_pitch;
+  const GLB uint8_t *const codes = glb((const uint8_t *)T.codes);
+  const GLB uint32_t *const ringidx = glb((const uint32_t *)T.ringidx);
+  const GLB uint32_t *const loc = glb((const uint32_t *)T.loc);
+  GLB uint32_t *const own = glb(T.own);
   for (int r = tid; r < 2 * TY; r += 256) {
     const int cy = 2 * j0 + r;
-    if (cy < g.ch) so[r] = T.seg_offs[(int64_t)cy * T.nseg + bx];
+    if (cy < g.ch) so[r] = glb((const uint32_t *)T.seg_offs)[(int64_t)cy * nbx + bx];
   }
   __syncthreads();
   auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
     // the neighbour always exists for a valid gradient (paths stay in ext)
     if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return OWN_NONE;
-    return T.ringidx[(int64_t)(tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj)];
+    return ringidx[(int64_t)(tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj)];
   };
   auto decode = [&](uint32_t w, int q) -> uint32_t {
     const uint32_t kind = term_kind(w);
@@ -366,7 +391,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     const int lj = wv * RW + r, j = j0 + lj;
     uint32_t a = OWN_NONE, c = OWN_NONE;
     if (j < g.vy && i0 + lane < g.vx) {
-      const uint32_t w = T.loc[(int64_t)j * T.loc_pitch + i0 + lane];
+      const uint32_t w = loc[(int64_t)j * loc_pitch + i0 + lane];
       a = decode(w & 0xFFFFu, 0);
       c = decode(w >> 16, 1);
     }
@@ -381,9 +406,9 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     const int i = i0 + li, j = j0 + lj;
     if (j >= g.vy) break;  // uniform across the wave
     const bool ein = j + 1 < g.vy;
-    const uint8_t *crow = T.codes + (int64_t)(2 * j) * T.pitch + 2 * i;
-    const uint16_t w0 = *(const uint16_t *)crow;
-    const uint16_t w1 = ein ? *(const uint16_t *)(crow + T.pitch) : (uint16_t)0;
+    const GLB uint8_t *crow = codes + (int64_t)(2 * j) * pitch + 2 * i;
+    const uint16_t w0 = *(const GLB uint16_t *)crow;
+    const uint16_t w1 = ein ? *(const GLB uint16_t *)(crow + pitch) : (uint16_t)0;
     const uint8_t cv = (uint8_t)w0, chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1, cq = (uint8_t)(w1 >> 8);
     const uint64_t bv = __ballot(cv & CF_CRITICAL), bh = __ballot(chh & CF_CRITICAL);
     const uint64_t be = __ballot(ce & CF_CRITICAL), bq = __ballot(cq & CF_CRITICAL);
@@ -407,9 +432,9 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
       else if (d == DIR_PX) oe = oq;
       else oe = li > 0 ? dq[lj * TX + li - 1] : ring_val(bx - 1, by, 1, TX - 1, lj);
     }
-    uint32_t *orow = T.own + (int64_t)(2 * j) * T.own_pitch + 2 * i;
-    *(uint2 *)orow = make_uint2(ov, oh);
-    if (ein) *(uint2 *)(orow + T.own_pitch) = make_uint2(oe, oq);
+    GLB uint32_t *orow = own + (int64_t)(2 * j) * own_pitch + 2 * i;
+    *(GLB u32x2 *)orow = u32x2{ov, oh};
+    if (ein) *(GLB u32x2 *)(orow + own_pitch) = u32x2{oe, oq};
   }
 }
 
@@ -443,7 +468,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const TileDesc *__restrict_
   __shared__ uint32_t s_wave[4];
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const int lb = b - prefix[t];
   const int64_t n = (int64_t)T.g.ch * T.nseg;
   const int64_t base = (int64_t)lb * SCAN_CHUNK;
@@ -459,7 +484,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const TileDesc *__restrict_
 
 __global__ __launch_bounds__(256) void k_scan_parts(const TileDesc *__restrict__ tiles) {
   __shared__ uint32_t s_wave[4];
-  const TileDesc &T = tiles[blockIdx.x];
+  const TileDesc T = tiles[blockIdx.x];
   const int64_t n = (int64_t)T.g.ch * T.nseg;
   const int nparts = (int)((n + SCAN_CHUNK - 1) / SCAN_CHUNK);
   uint32_t carry = 0;
@@ -479,7 +504,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const TileDesc *__restrict__ 
   __shared__ uint32_t s_wave[4];
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const int lb = b - prefix[t];
   const int64_t n = (int64_t)T.g.ch * T.nseg;
   const int64_t base = (int64_t)lb * SCAN_CHUNK + (int64_t)threadIdx.x * (SCAN_CHUNK / 256);
@@ -541,7 +566,7 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ ti
                                                  int ntiles) {
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const int lane = threadIdx.x & 63;
   const int64_t n = (int64_t)T.g.ch * T.nseg;
   const int64_t first = ((int64_t)(b - prefix[t]) * 4 + (threadIdx.x >> 6)) * COMPACT_ITEMS;
@@ -594,7 +619,7 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
                                                    int ntiles) {
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
-  const TileDesc &T = tiles[t];
+  const TileDesc T = tiles[t];
   const TileGeom g = T.g;
   const uint32_t K = T.nkrit[0];
   const uint32_t k = (uint32_t)(b - prefix[t]) * 256 + threadIdx.x;
