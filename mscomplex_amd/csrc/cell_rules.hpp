@@ -327,7 +327,6 @@ namespace mscx {
 // above (checked exhaustively on the CPU against cell_code).
 
 constexpr uint32_t KEY_INF = 0xFFFFFFFFu;  // above every fkey() value (NaN-free input)
-constexpr int CAND_BOTH = 7;               // both cofacets usable: decided by quad_lt
 
 struct KeyWin {
   uint32_t w00, w01, w02, w03, w10, w11, w12, w13, w20, w21, w22, w23, w30, w31, w32, w33;
@@ -359,23 +358,45 @@ MSCX_HD int vcand_keys(uint32_t fv, uint32_t fnx, uint32_t fny, uint32_t fpy, ui
   const uint32_t eny = (any && fny <= fv) ? fny : KEY_INF;
   const uint32_t epy = (apy && fpy < fv) ? fpy : KEY_INF;
   const uint32_t epx = (apx && fpx < fv) ? fpx : KEY_INF;
+  // every value is computed before it is selected, so the compiler keeps
+  // this (and the rules below) free of divergent branches
   const bool l1 = enx <= eny, l2 = epy <= epx;
   const uint32_t e1 = l1 ? enx : eny, e2 = l2 ? epy : epx;
   const bool l = e1 <= e2;
-  const uint32_t e = l ? e1 : e2;
-  return e == KEY_INF ? DIR_NONE : (l ? (l1 ? DIR_NX : DIR_NY) : (l2 ? DIR_PY : DIR_PX));
+  const int d1 = l1 ? DIR_NX : DIR_NY, d2 = l2 ? DIR_PY : DIR_PX;
+  const int d = l ? d1 : d2;
+  return (e1 & e2) == KEY_INF ? DIR_NONE : d;  // no usable edge: all four are KEY_INF
 }
-// hedge_cand of the edge A=(x,y)-B=(x+1,y): Au, Bu the row above, Ad, Bd below.
+// Edge candidates.  A quad is a usable cofacet of one of its edges exactly
+// when that edge is its largest, i.e. joins its largest corner to the larger
+// of that corner's two neighbours; equivalently, each end of the edge is
+// above the corner diagonally across from it.  When both quads of an edge
+// are usable, the end of the edge that is not the quad's largest corner is
+// above its neighbour off the edge, so each quad's smallest corner is one of
+// its two corners off the edge; those are distinct vertices, so quad_lt on
+// the sorted corners is decided by the smaller of them alone.
+//
+// hedge_cand of the edge A=(x,y)-B=(x+1,y): Au, Bu the row above, Ad, Bd
+// below.  Between equal values the key in column x, then the lower row, is
+// the smaller.
 MSCX_HD int hcand_keys(uint32_t fA, uint32_t fB, uint32_t fAu, uint32_t fBu, uint32_t fAd, uint32_t fBd) {
   const bool up = fAu <= fB && fBu < fA;
   const bool dn = fAd <= fB && fBd < fA;
-  return up ? (dn ? CAND_BOTH : DIR_PY) : (dn ? DIR_NY : DIR_NONE);
+  const bool cu = fAu <= fBu, cd = fAd <= fBd;  // smaller corner in column x
+  const uint32_t mu = cu ? fAu : fBu, md = cd ? fAd : fBd;
+  const bool up_first = mu < md || (mu == md && cu && !cd);
+  const int d_up = (dn && !up_first) ? DIR_NY : DIR_PY, d_dn = dn ? DIR_NY : DIR_NONE;
+  return up ? d_up : d_dn;
 }
-// vedge_cand of the edge A=(x,y)-B=(x,y+1): Ra, Rb the column right, La, Lb left.
+// vedge_cand of the edge A=(x,y)-B=(x,y+1): Ra, Rb the column right, La, Lb
+// left.  Between equal values the key in the left column is the smaller.
 MSCX_HD int ecand_keys(uint32_t fA, uint32_t fB, uint32_t fRa, uint32_t fRb, uint32_t fLa, uint32_t fLb) {
   const bool rt = fRa < fB && fRb < fA;
   const bool lf = fLa <= fB && fLb <= fA;
-  return rt ? (lf ? CAND_BOTH : DIR_PX) : (lf ? DIR_NX : DIR_NONE);
+  const uint32_t mr = fRa < fRb ? fRa : fRb, ml = fLa < fLb ? fLa : fLb;
+  const bool rt_first = mr < ml;
+  const int d_rt = (lf && !rt_first) ? DIR_NX : DIR_PX, d_lf = lf ? DIR_NX : DIR_NONE;
+  return rt ? d_rt : d_lf;
 }
 
 // Boundary classes of vertex column x / row y (INTERIOR: none applies).
@@ -415,26 +436,6 @@ MSCX_HD int vcand_at(uint32_t fv, uint32_t fnx, uint32_t fny, uint32_t fpy, uint
   return vcand_keys(fv, fnx, fny, fpy, fpx, cx.lo && hx, cy.lo && vy, cy.hi && vy, cx.hi && hx);
 }
 
-// quad_lt decision of an edge with both cofacets usable, on split keys
-MSCX_HD int hcand_both(const KeyWin &K, int x, int y) {
-  const VKey a = K(x, y), b = K(x + 1, y);
-  const bool a_hi = a > b;
-  const VKey hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
-  const VKey ua = K(x, y + 1), ub = K(x + 1, y + 1), da = K(x, y - 1), db = K(x + 1, y - 1);
-  return quad_lt(lo, hi, ksel(a_hi, ua, ub), ksel(a_hi, ub, ua), lo, hi, ksel(a_hi, da, db), ksel(a_hi, db, da))
-             ? DIR_PY
-             : DIR_NY;
-}
-MSCX_HD int ecand_both(const KeyWin &K, int x, int y) {
-  const VKey a = K(x, y), b = K(x, y + 1);
-  const bool a_hi = a > b;
-  const VKey hi = ksel(a_hi, a, b), lo = ksel(a_hi, b, a);
-  const VKey ra = K(x + 1, y), rb = K(x + 1, y + 1), la = K(x - 1, y), lb = K(x - 1, y + 1);
-  return quad_lt(lo, hi, ksel(a_hi, ra, rb), ksel(a_hi, rb, ra), lo, hi, ksel(a_hi, la, lb), ksel(a_hi, lb, la))
-             ? DIR_PX
-             : DIR_NX;
-}
-
 // Candidates at the window's centre (i, j) = (K.i, K.j): the vertex and
 // the vertical edge (i, j)-(i, j+1).
 template <bool INTERIOR>
@@ -450,7 +451,6 @@ MSCX_HD int ec_center(const KeyWin &K, const TileGeom &g) {
     const VClass r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(K.j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
     if (!(c0.in && r1.in && c0.rv && r0.re && c0.lo && c0.hi)) e = DIR_NONE;
   }
-  if (e == CAND_BOTH) e = ecand_both(K, K.i, K.j);
   return e;
 }
 
@@ -463,7 +463,6 @@ MSCX_HD int hc_center(const KeyWin &K, const TileGeom &g) {
     const VClass r0 = vclass(K.j, g.vy, g.ry0, g.ry1, g.ch - 1);
     if (!(c1.in && r0.in && c0.re && r0.rv && r0.lo && r0.hi)) h = DIR_NONE;
   }
-  if (h == CAND_BOTH) h = hcand_both(K, K.i, K.j);
   return h;
 }
 
@@ -478,11 +477,9 @@ MSCX_HD void row_init(const KeyWin &K, const TileGeom &g, int &vc00, int &vc10, 
 }
 
 // New candidates of a row step in the lane's own column: vertex (i, j+1),
-// horizontal edge (i, j+1)-(i+1, j+1), vertical edge (i, j)-(i, j+1).  The
-// edges may come back as CAND_BOTH (both cofacets usable); row_own_resolve
-// decides those with quad_lt.
+// horizontal edge (i, j+1)-(i+1, j+1), vertical edge (i, j)-(i, j+1).
 template <bool INTERIOR>
-MSCX_HD void row_own_raw(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
+MSCX_HD void row_own(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
   const int i = K.i, j = K.j;
   const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
   const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
@@ -496,15 +493,6 @@ MSCX_HD void row_own_raw(const KeyWin &K, const TileGeom &g, int &vc01, int &hc0
   hc01 = h;
   ec00 = e;
 }
-MSCX_HD void row_own_resolve(const KeyWin &K, int &hc01, int &ec00) {
-  if (hc01 == CAND_BOTH) hc01 = hcand_both(K, K.i, K.j + 1);
-  if (ec00 == CAND_BOTH) ec00 = ecand_both(K, K.i, K.j);
-}
-template <bool INTERIOR>
-MSCX_HD void row_own(const KeyWin &K, const TileGeom &g, int &vc01, int &hc01, int &ec00) {
-  row_own_raw<INTERIOR>(K, g, vc01, hc01, ec00);
-  row_own_resolve(K, hc01, ec00);
-}
 
 // The same step's candidates of the next column (vertex (i+1, j+1), edge
 // (i+1, j)-(i+1, j+1)), which the kernel takes from the neighbouring lane.
@@ -516,7 +504,6 @@ MSCX_HD void row_next(const KeyWin &K, const TileGeom &g, int &vc11, int &ec10) 
   vc11 = vcand_at<INTERIOR>(K.w22, K.w21, K.w12, K.w32, K.w23, c1, r1);
   int e = ecand_keys(K.w12, K.w22, K.w13, K.w23, K.w11, K.w21);
   if (!INTERIOR && !(c1.in && r1.in && c1.rv && r0.re && c1.lo && c1.hi)) e = DIR_NONE;
-  if (e == CAND_BOTH) e = ecand_both(K, i + 1, j);
   ec10 = e;
 }
 
@@ -529,31 +516,35 @@ MSCX_HD void row_cell_codes(const KeyWin &K, const TileGeom &g, int vc00, int vc
   const int i = K.i, j = K.j;
   const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
   const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
-  // codes (code_from_cands)
-  int dv = vc00;
-  int dh = hc00;
-  if (dh == DIR_NONE) {
-    const bool a_hi = K.w11 > K.w12;
-    if (a_hi ? vc00 == DIR_PX : vc10 == DIR_NX) dh = a_hi ? DIR_NX : DIR_PX;
-  }
-  int de = ec00;
-  if (de == DIR_NONE) {
-    const bool a_hi = K.w11 > K.w21;
-    if (a_hi ? vc00 == DIR_PY : vc01 == DIR_NY) de = a_hi ? DIR_NY : DIR_PY;
-  }
-  int dq = DIR_NONE;
+  // codes (code_from_cands), all values computed before they are selected
+  const int dv = vc00;
+  // an edge without a usable quad takes its higher end's vertex if that
+  // vertex's candidate is this edge
+  const bool h_hi = K.w11 > K.w12;
+  const bool h_v = (h_hi & (vc00 == DIR_PX)) | (!h_hi & (vc10 == DIR_NX));
+  const int h_d = h_hi ? DIR_NX : DIR_PX;
+  const int dh = hc00 != DIR_NONE ? hc00 : (h_v ? h_d : DIR_NONE);
+  const bool e_hi = K.w11 > K.w21;
+  const bool e_v = (e_hi & (vc00 == DIR_PY)) | (!e_hi & (vc01 == DIR_NY));
+  const int e_d = e_hi ? DIR_NY : DIR_PY;
+  const int de = ec00 != DIR_NONE ? ec00 : (e_v ? e_d : DIR_NONE);
+  int dq;
   {
+    // the quad is paired with its largest edge when that edge's candidate is
+    // this quad
     const uint32_t k00 = K.w11, k10 = K.w12, k01 = K.w21, k11 = K.w22;
     // position order 00 < 01 < 10 < 11 (x, then y)
     const bool g00_10 = k00 > k10, g00_01 = k00 > k01, g00_11 = k00 > k11;
     const bool g10_01 = k10 >= k01, g10_11 = k10 > k11, g01_11 = k01 > k11;
-    int edir;
-    if (g00_10 && g00_01 && g00_11) edir = g10_01 ? DIR_NY : DIR_NX;
-    else if (g10_01 && g10_11) edir = g00_11 ? DIR_NY : DIR_PX;
-    else if (g01_11) edir = g00_11 ? DIR_NX : DIR_PY;
-    else edir = g10_01 ? DIR_PX : DIR_PY;
-    const int ec = edir == DIR_NY ? hc00 : edir == DIR_PY ? hc01 : edir == DIR_NX ? ec00 : ec10;
-    if (ec == dir_flip(edir)) dq = edir;
+    const int m00 = g10_01 ? DIR_NY : DIR_NX, m10 = g00_11 ? DIR_NY : DIR_PX;
+    const int m01 = g00_11 ? DIR_NX : DIR_PY, m11 = g10_01 ? DIR_PX : DIR_PY;
+    const int r2 = g01_11 ? m01 : m11;
+    const int r1 = (g10_01 && g10_11) ? m10 : r2;
+    const int edir = (g00_10 && g00_01 && g00_11) ? m00 : r1;
+    const int ea = edir == DIR_NX ? ec00 : ec10, eb = edir == DIR_PY ? hc01 : ea;
+    const int ec = edir == DIR_NY ? hc00 : eb;
+    const int flip = ((edir - 1) ^ 1) + 1;  // edir is 1..4
+    dq = ec == flip ? edir : DIR_NONE;
   }
   if (INTERIOR) {
     cv = (uint8_t)((dv ? CF_PAIRED : CF_CRITICAL) | (dv << 2));

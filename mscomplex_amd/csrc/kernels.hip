@@ -24,6 +24,11 @@ __device__ __forceinline__ const GLB T *glb(const T *p) {
   return (const GLB T *)p;
 }
 
+// number of set bits of b in the lanes below this one, plus acc
+__device__ __forceinline__ uint32_t rank_below(uint64_t b, uint32_t acc) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, acc));
+}
+
 __device__ __forceinline__ int find_tile(const int *__restrict__ prefix, int ntiles, int b) {
   int lo = 0, hi = ntiles - 1;
   while (lo < hi) {
@@ -65,16 +70,25 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 // critical entry becomes its own root terminal, recorded as its cell row in
 // the block (2 lj + q) and its rank among the critical cells of that row's
 // segment
-__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY;
-  if (c & CF_CRITICAL) return T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
-  if (!(c & CF_PAIRED)) return T_NONE;
-  const int d = code_dir(c);
-  const int ni = li + dir_dx(d), nj = lj + dir_dy(d);
-  if (ni >= 0 && nj >= 0 && ni < TX && nj < TY) return (uint32_t)(q * NV + nj * TX + ni);
-  const uint32_t s = ni >= TX ? 0u : (ni < 0 ? 1u : (nj >= TY ? 2u : 3u));
-  const uint32_t p = (s < 2) ? (uint32_t)lj : (uint32_t)li;
-  return T_EXIT | (s << 6) | p;
+// Trace link of a vertex (q = 0) or quad (q = 1) entry (li, lj) from its
+// code: its root terminal when critical, none when unpaired, else the entry
+// across its pair (DIR_PX, NX, PY, NY step by +1, -1, +TX, -TX) or, when
+// that leaves the tile through side s = dir - 1, the exit terminal.
+// exits: bit s set when this entry's step in direction s + 1 leaves the
+// tile.  Every value is computed before it is selected (no divergent
+// branches).
+__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank, uint32_t exits) {
+  constexpr int TX = GRAD_TX, NV = TX * GRAD_TY;
+  static_assert(TX == 64, "step table holds +-TX in a signed byte");
+  constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)TX << 16) | ((uint32_t)(256 - TX) << 24);
+  const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
+  const uint32_t s = (uint32_t)code_dir(c) - 1u;  // side 0..3 when paired
+  const int step = __builtin_amdgcn_sbfe((int)STEP, (int)(8 * s), 8);
+  const uint32_t in_link = (uint32_t)(q * NV + lj * TX + li + step);
+  const uint32_t exit_link = T_EXIT | (s << 6) | (s < 2 ? (uint32_t)lj : (uint32_t)li);
+  const uint32_t link = __builtin_amdgcn_ubfe(exits, s, 1) ? exit_link : in_link;
+  const uint32_t paired = (c & CF_PAIRED) ? link : T_NONE;
+  return (c & CF_CRITICAL) ? root : paired;
 }
 
 // Step 2 for one wave: rows lj0 .. lj0 + GRAD_TY/4 - 1 of the block tile,
@@ -119,7 +133,6 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   int hc00 = hc_center<INTERIOR>(K, g);
   int vc10 = __shfl_down(vc00, 1, 64);
   if (lane == 63) vc10 = __builtin_amdgcn_readlane(xv, 0);
-  const uint64_t below = lanes_below(lane);
   // this lane's two code bytes in cell row 2j, and this wave's critical
   // words / counts of cell row 2j; both advance by two cell rows per step
   const int64_t pitch = T.pitch, nseg = T.nseg;
@@ -127,44 +140,43 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   int64_t rs0 = (int64_t)(2 * K.j) * nseg + bx;
   GLB uint64_t *const critbits = glb(T.critbits);
   GLB uint32_t *const seg_counts = glb(T.seg_counts);
+  // sides a step from this lane's column leaves the tile through (link_of)
+  const uint32_t ex_col = (li == TX - 1 ? 1u : 0u) | (li == 0 ? 2u : 0u);
   for (int rr = 0; rr < RW; ++rr) {
     const int lj = lj0 + rr, j = K.j;
     if (j >= g.vy) break;  // uniform across the wave
     int vc01, hc01, ec00;
-    row_own_raw<INTERIOR>(K, g, vc01, hc01, ec00);
-    if (hc01 == CAND_BOTH || ec00 == CAND_BOTH) row_own_resolve(K, hc01, ec00);
-    int vc11 = __shfl_down(vc01, 1, 64), ec10 = __shfl_down(ec00, 1, 64);
+    row_own<INTERIOR>(K, g, vc01, hc01, ec00);
+    // the next column's candidates from the next lane; lane 63's next
+    // column is the prologue's column i0 + TX
+    const int sv = __shfl_down(vc01, 1, 64), se = __shfl_down(ec00, 1, 64);
     const int xv1 = __builtin_amdgcn_readlane(xv, rr + 1), xe0 = __builtin_amdgcn_readlane(xe, rr);
-    if (lane == 63) {
-      vc11 = xv1;
-      ec10 = xe0;
-    }
+    const int vc11 = lane == 63 ? xv1 : sv, ec10 = lane == 63 ? xe0 : se;
     uint8_t cv, chh, ce, cq;
     row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
-    const bool ein = INTERIOR || j + 1 < g.vy;
+    const bool ein = INTERIOR || j + 1 < g.vy;  // uniform across the wave
     *(GLB uint16_t *)crow = (uint16_t)(cv | (chh << 8));
     const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
+    // with no odd row below (ext edge) its words are not written
+    const uint64_t b2 = __ballot(ein && (ce & CF_CRITICAL)), b3 = __ballot(ein && (cq & CF_CRITICAL));
+    if (ein) *(GLB uint16_t *)(crow + pitch) = (uint16_t)(ce | (cq << 8));
     if (lane == 0) {
       *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
       seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
-    }
-    uint32_t rank_q = 0;
-    if (ein) {
-      *(GLB uint16_t *)(crow + pitch) = (uint16_t)(ce | (cq << 8));
-      const int64_t rs1 = rs0 + nseg;
-      const uint64_t b2 = __ballot(ce & CF_CRITICAL), b3 = __ballot(cq & CF_CRITICAL);
-      if (lane == 0) {
-        *(GLB u64x2 *)(critbits + 2 * rs1) = u64x2{b2, b3};
-        seg_counts[rs1] = (uint32_t)(__popcll(b2) + __popcll(b3));
+      if (ein) {
+        *(GLB u64x2 *)(critbits + 2 * (rs0 + nseg)) = u64x2{b2, b3};
+        seg_counts[rs0 + nseg] = (uint32_t)(__popcll(b2) + __popcll(b3));
       }
-      rank_q = (uint32_t)(__popcll(b2 & below) + __popcll(b3 & below)) + (uint32_t)((b2 >> lane) & 1u);
     }
+    // ranks among the row segment's critical cells before this lane's
+    const uint32_t rank_v = rank_below(b1, rank_below(b0, 0));
+    const uint32_t rank_q = rank_below(b3, rank_below(b2, 0)) + (uint32_t)((b2 >> lane) & 1u);
     crow += 2 * pitch;
     rs0 += 2 * nseg;
-    const uint32_t rank_v = (uint32_t)(__popcll(b0 & below) + __popcll(b1 & below));
     // codes outside the ext are 0, so those entries get T_NONE
-    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
-    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
+    const uint32_t ex = ex_col | (lj == TY - 1 ? 4u : 0u) | (lj == 0 ? 8u : 0u);
+    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0, rank_v, ex);
+    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1, rank_q, ex);
     vc00 = vc01;
     vc10 = vc11;
     hc00 = hc01;
@@ -239,34 +251,39 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   // 3. pointer jumping.  Links only ever move further along their path, so
   //    the entries are updated without barriers: whatever value a read
   //    returns (the original link, a later one or the terminal) is a valid
-  //    state.  Each round reads all of a thread's pending entries, then all
-  //    their targets, then writes, so the LDS latency is paid twice per
-  //    round rather than per entry.  Gradient paths are acyclic and every
-  //    round moves a pending entry at least one step, so the loop ends
-  //    within the longest path (the bound guards bad input).
+  //    state.  A thread keeps its entries' current values in registers (it
+  //    is the only writer of its entries); each round reads every target,
+  //    then writes every entry, so the LDS latency is paid once per round.
+  //    Terminal entries read a harmless in-range slot and keep their value.
+  //    Gradient paths are acyclic and every round moves a pending entry at
+  //    least one step, so the loop ends within the longest path (the bound
+  //    guards bad input).
   {
     constexpr int PER = 2 * NV / 256;
-    uint32_t pend = (1u << PER) - 1u;
-    for (int round = 0; pend && round < 2 * NV; ++round) {
-      uint32_t e1[PER];
+    static_assert((2 * NV & (2 * NV - 1)) == 0 && 2 * NV <= 0x8000, "entry indices below the terminal bit");
+    uint32_t cur[PER];
 #pragma unroll
-      for (int k = 0; k < PER; ++k) e1[k] = (pend >> k) & 1u ? (uint32_t)sl[tid + 256 * k] : T_NONE;
-      uint32_t e2[PER];
+    for (int k = 0; k < PER; ++k) cur[k] = sl[tid + 256 * k];
+    uint32_t all = 0x8000u;
 #pragma unroll
-      for (int k = 0; k < PER; ++k) e2[k] = (e1[k] & 0x8000u) ? e1[k] : (uint32_t)sl[e1[k]];
-      uint32_t np = 0;
+    for (int k = 0; k < PER; ++k) all &= cur[k];
+    for (int round = 0; !all && round < 2 * NV; ++round) {
+      uint32_t nxt[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) nxt[k] = sl[cur[k] & (2 * NV - 1)];
+      all = 0x8000u;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
-        if (!(e1[k] & 0x8000u)) sl[tid + 256 * k] = (uint16_t)e2[k];
-        np |= (uint32_t)((e2[k] & 0x8000u) == 0) << k;
+        cur[k] = (cur[k] & 0x8000u) ? cur[k] : nxt[k];
+        sl[tid + 256 * k] = (uint16_t)cur[k];
+        all &= cur[k];
       }
-      pend = np;
     }
-    if (pend) {
+    if (!all) {
       atomicOr(err, 2u);
 #pragma unroll
       for (int k = 0; k < PER; ++k)
-        if ((pend >> k) & 1u) sl[tid + 256 * k] = (uint16_t)T_NONE;
+        if (!(cur[k] & 0x8000u)) sl[tid + 256 * k] = (uint16_t)T_NONE;
     }
   }
   __syncthreads();
@@ -364,74 +381,106 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   const GLB uint32_t *const ringidx = glb((const uint32_t *)T.ringidx);
   const GLB uint32_t *const loc = glb((const uint32_t *)T.loc);
   GLB uint32_t *const own = glb(T.own);
+  uint32_t wl[RW];  // this lane's terminals (vertex | quad << 16) of its rows
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int j = j0 + wv * RW + r;
+    wl[r] = (j < g.vy && i0 + lane < g.vx) ? loc[(int64_t)j * loc_pitch + i0 + lane] : (T_NONE | (T_NONE << 16));
+  }
   for (int r = tid; r < 2 * TY; r += 256) {
     const int cy = 2 * j0 + r;
     if (cy < g.ch) so[r] = glb((const uint32_t *)T.seg_offs)[(int64_t)cy * nbx + bx];
   }
   __syncthreads();
-  auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
-    // the neighbour always exists for a valid gradient (paths stay in ext)
-    if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return OWN_NONE;
-    return ringidx[(int64_t)(tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj)];
+  // ring word of entry (tli, tlj) of forest q in block (tbx, tby), or
+  // UINT32_MAX when that block does not exist (a valid gradient's paths stay
+  // inside ext, so this only guards bad input)
+  auto ring_index = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
+    if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return 0xFFFFFFFFu;
+    return (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)(q * RING_Q + ring_pos(tli, tlj));
   };
-  auto decode = [&](uint32_t w, int q) -> uint32_t {
-    const uint32_t kind = term_kind(w);
-    if (kind == 0) return so[(w >> 7) & 63u] + (w & 127u);
-    if (kind == 1) {
-      const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
-      if (s == 0) return ring_val(bx + 1, by, q, 0, pos);
-      if (s == 1) return ring_val(bx - 1, by, q, TX - 1, pos);
-      if (s == 2) return ring_val(bx, by + 1, q, pos, 0);
-      return ring_val(bx, by - 1, q, pos, TY - 1);
-    }
-    return OWN_NONE;
+  auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
+    const uint32_t x = ring_index(tbx, tby, q, tli, tlj);
+    return x == 0xFFFFFFFFu ? OWN_NONE : ringidx[x];
   };
 
-  for (int r = 0; r < RW; ++r) {
-    const int lj = wv * RW + r, j = j0 + lj;
-    uint32_t a = OWN_NONE, c = OWN_NONE;
-    if (j < g.vy && i0 + lane < g.vx) {
-      const uint32_t w = loc[(int64_t)j * loc_pitch + i0 + lane];
-      a = decode(w & 0xFFFFu, 0);
-      c = decode(w >> 16, 1);
+  // 1. terminals -> critical indices in LDS.  A lane issues all its
+  //    terminal loads, then all its ring loads, before it uses any of them.
+  uint32_t rx[2 * RW];
+#pragma unroll
+  for (int k = 0; k < 2 * RW; ++k) {
+    const int q = k & 1;
+    const uint32_t w = (wl[k >> 1] >> (16 * q)) & 0xFFFFu;
+    uint32_t x = 0xFFFFFFFFu;
+    if (term_kind(w) == 1) {
+      const int sd = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
+      if (sd == 0) x = ring_index(bx + 1, by, q, 0, pos);
+      else if (sd == 1) x = ring_index(bx - 1, by, q, TX - 1, pos);
+      else if (sd == 2) x = ring_index(bx, by + 1, q, pos, 0);
+      else x = ring_index(bx, by - 1, q, pos, TY - 1);
     }
-    dv[lj * TX + lane] = a;
-    dq[lj * TX + lane] = c;
+    rx[k] = x;
+  }
+  uint32_t rv[2 * RW];
+#pragma unroll
+  for (int k = 0; k < 2 * RW; ++k) rv[k] = rx[k] != 0xFFFFFFFFu ? ringidx[rx[k]] : OWN_NONE;
+  // owners across the block's sides that this lane's edges may be paired
+  // with: per row, lane TX-1 the right block's vertex and lane 0 the left
+  // block's quad; the block's first row the quads above it, its last row
+  // the vertices below it
+  uint32_t xs[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int lj = wv * RW + r;
+    const uint32_t x = lane == TX - 1 ? ring_index(bx + 1, by, 0, 0, lj)
+                                      : (lane == 0 ? ring_index(bx - 1, by, 1, TX - 1, lj) : 0xFFFFFFFFu);
+    xs[r] = x != 0xFFFFFFFFu ? ringidx[x] : OWN_NONE;
+  }
+  const uint32_t xt = wv == 0 ? ring_val(bx, by - 1, 1, lane, TY - 1) : OWN_NONE;
+  const uint32_t xb = wv == 3 ? ring_val(bx, by + 1, 0, lane, 0) : OWN_NONE;
+#pragma unroll
+  for (int k = 0; k < 2 * RW; ++k) {
+    const int q = k & 1, lj = wv * RW + (k >> 1);
+    const uint32_t w = (wl[k >> 1] >> (16 * q)) & 0xFFFFu;
+    const uint32_t kind = term_kind(w);
+    const uint32_t o = kind == 0 ? so[(w >> 7) & 63u] + (w & 127u) : kind == 1 ? rv[k] : OWN_NONE;
+    (q ? dq : dv)[lj * TX + lane] = o;
+  }
+  // this lane's codes of all its rows, loaded while the others write LDS
+  uint16_t cw0[RW], cw1[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int j = j0 + wv * RW + r;
+    const GLB uint8_t *crow = codes + (int64_t)(2 * j) * pitch + 2 * (i0 + lane);
+    cw0[r] = j < g.vy ? *(const GLB uint16_t *)crow : (uint16_t)0;
+    cw1[r] = j + 1 < g.vy ? *(const GLB uint16_t *)(crow + pitch) : (uint16_t)0;
   }
   __syncthreads();
 
-  const uint64_t below = lanes_below(lane);
+  // 2. owners of the four cells of each vertex: vertices and quads from
+  //    the terminals, critical edges by rank, paired edges from the vertex or
+  //    quad they are paired with (every value is computed, then selected)
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r, li = lane;
     const int i = i0 + li, j = j0 + lj;
     if (j >= g.vy) break;  // uniform across the wave
     const bool ein = j + 1 < g.vy;
-    const GLB uint8_t *crow = codes + (int64_t)(2 * j) * pitch + 2 * i;
-    const uint16_t w0 = *(const GLB uint16_t *)crow;
-    const uint16_t w1 = ein ? *(const GLB uint16_t *)(crow + pitch) : (uint16_t)0;
+    const uint16_t w0 = cw0[r], w1 = cw1[r];
     const uint8_t cv = (uint8_t)w0, chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1, cq = (uint8_t)(w1 >> 8);
     const uint64_t bv = __ballot(cv & CF_CRITICAL), bh = __ballot(chh & CF_CRITICAL);
     const uint64_t be = __ballot(ce & CF_CRITICAL), bq = __ballot(cq & CF_CRITICAL);
     const uint32_t ov = dv[lj * TX + li], oq = dq[lj * TX + li];
-    uint32_t oh = OWN_NONE, oe = OWN_NONE;
-    if (chh & CF_CRITICAL) {
-      oh = so[2 * lj] + (uint32_t)(__popcll(bv & below) + __popcll(bh & below)) + (uint32_t)((bv >> lane) & 1u);
-    } else if (chh & CF_PAIRED) {
-      const int d = code_dir(chh);
-      if (d == DIR_PX) oh = li + 1 < TX ? dv[lj * TX + li + 1] : ring_val(bx + 1, by, 0, 0, lj);
-      else if (d == DIR_NX) oh = ov;
-      else if (d == DIR_PY) oh = oq;
-      else oh = lj > 0 ? dq[(lj - 1) * TX + li] : ring_val(bx, by - 1, 1, li, TY - 1);
-    }
-    if (ce & CF_CRITICAL) {
-      oe = so[2 * lj + 1] + (uint32_t)(__popcll(be & below) + __popcll(bq & below));
-    } else if (ce & CF_PAIRED) {
-      const int d = code_dir(ce);
-      if (d == DIR_PY) oe = lj + 1 < TY ? dv[(lj + 1) * TX + li] : ring_val(bx, by + 1, 0, li, 0);
-      else if (d == DIR_NY) oe = ov;
-      else if (d == DIR_PX) oe = oq;
-      else oe = li > 0 ? dq[lj * TX + li - 1] : ring_val(bx - 1, by, 1, TX - 1, lj);
-    }
+    const uint32_t up_l = dq[(lj > 0 ? lj - 1 : 0) * TX + li], dn_l = dv[(lj + 1 < TY ? lj + 1 : lj) * TX + li];
+    const uint32_t up_q = lj > 0 ? up_l : xt, dn_v = lj + 1 < TY ? dn_l : xb;
+    const uint32_t rt_s = __shfl_down(ov, 1, 64), lf_s = __shfl_up(oq, 1, 64);
+    const uint32_t rt_v = li == TX - 1 ? xs[r] : rt_s, lf_q = li == 0 ? xs[r] : lf_s;
+    const uint32_t rh = so[2 * lj] + rank_below(bh, rank_below(bv, 0)) + (uint32_t)((bv >> lane) & 1u);
+    const uint32_t re = so[2 * lj + 1] + rank_below(bq, rank_below(be, 0));
+    const int dh = code_dir(chh), de = code_dir(ce);
+    const uint32_t ph1 = dh == DIR_PY ? oq : up_q, ph2 = dh == DIR_NX ? ov : ph1, ph = dh == DIR_PX ? rt_v : ph2;
+    const uint32_t pe1 = de == DIR_PX ? oq : lf_q, pe2 = de == DIR_NY ? ov : pe1, pe = de == DIR_PY ? dn_v : pe2;
+    const uint32_t hp = (chh & CF_PAIRED) ? ph : OWN_NONE, ep = (ce & CF_PAIRED) ? pe : OWN_NONE;
+    const uint32_t oh = (chh & CF_CRITICAL) ? rh : hp, oe = (ce & CF_CRITICAL) ? re : ep;
     GLB uint32_t *orow = own + (int64_t)(2 * j) * own_pitch + 2 * i;
     *(GLB u32x2 *)orow = u32x2{ov, oh};
     if (ein) *(GLB u32x2 *)(orow + own_pitch) = u32x2{oe, oq};

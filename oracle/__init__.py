@@ -37,7 +37,7 @@ def lib():
         L.orc_tile_complex.argtypes = [P, P, P, P, ctypes.c_size_t]
         L.orc_pipeline_run.restype = P
         L.orc_pipeline_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_double, ctypes.c_int]
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_pipeline_npieces.argtypes = [P]
         L.orc_pipeline_nleaves.argtypes = [P]
         L.orc_pipeline_dump.restype = ctypes.c_long
@@ -93,13 +93,15 @@ def tile_complex(data, rect, ext=None):
 class Pipeline:
     """The GridDataManager compute order on a whole grid; dumps of every
     piece's complex after each phase (0 compute, 1 merge-up, 2 simplify,
-    3 merge-down, 4 discs)."""
+    3 merge-down, 4 discs).  threads > 1 runs the independent pieces of each
+    phase in parallel (the reference's --num-parallel); results do not
+    depend on it."""
 
-    def __init__(self, data, levels=0, lx=0, ly=0, threshold=0.0, keep=True):
+    def __init__(self, data, levels=0, lx=0, ly=0, threshold=0.0, keep=True, threads=1):
         d = np.ascontiguousarray(data, np.float32)
         sy, sx = d.shape
         self._data = d
-        h = lib().orc_pipeline_run(_p(d), sx, sy, levels, lx, ly, float(threshold), 1 if keep else 0)
+        h = lib().orc_pipeline_run(_p(d), sx, sy, levels, lx, ly, float(threshold), 1 if keep else 0, int(threads))
         if not h:
             raise RuntimeError(_err())
         self._h = h

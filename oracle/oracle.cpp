@@ -8,6 +8,10 @@
 #include "oracle.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
+#include <functional>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -902,33 +906,65 @@ void DataManager::snapshot(std::vector<std::vector<uint32_t>> &dst) {
   }
 }
 
+// f(0..n-1) on up to nthreads threads (items taken in index order)
+static void par_for(uint n, int nthreads, const std::function<void(uint)> &f) {
+  if (nthreads <= 1 || n <= 1) {
+    for (uint i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<uint> next(0);
+  std::vector<std::exception_ptr> errs(n);
+  auto worker = [&]() {
+    for (uint i; (i = next.fetch_add(1)) < n;) {
+      try {
+        f(i);
+      } catch (...) {
+        errs[i] = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads && (uint)t < n; ++t) th.emplace_back(worker);
+  for (auto &t : th) t.join();
+  for (auto &e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
 // GridDataManager::GridDataManager compute order (grid_datamanager.cpp:
-// 643-660), single-threaded (-st): computeSubdomainMsgraphs, mergePiecesUp,
-// root simplify_un_simplify, mergePiecesDown, collectSubdomainManifolds.
-void DataManager::run(PipelineSnapshots *snaps) {
+// 643-660): computeSubdomainMsgraphs, mergePiecesUp, root
+// simplify_un_simplify, mergePiecesDown, collectSubdomainManifolds.  With
+// nthreads == 1 this is the -st order; otherwise the leaf phases and each
+// merge-tree level run their independent pieces in parallel
+// (grid_datamanager.cpp:213-238, 258-306, 308-357).
+void DataManager::run(PipelineSnapshots *snaps, int nthreads) {
   PipelineSnapshots local;
   PipelineSnapshots &S = snaps ? *snaps : local;
   uint num_leafs = 1u << m_num_levels;
 
   double t0 = now_s();
   // computeSubdomainMsgraphs / computeMsGraph (:121-140, :493-528)
-  for (uint i = 0; i < num_leafs; ++i) {
+  par_for(num_leafs, nthreads, [&](uint i) {
     Piece *dp = m_pieces[i];
     initPiece(dp);
     dp->dataset->work();
     dp->dataset->writeout_connectivity(dp->msgraph);
-  }
+  });
   double t1 = now_s();
   S.t_compute = t1 - t0;
   snapshot(S.after_compute);
 
   // mergePiecesUp (:258-306), single-threaded order
   for (uint i = 0; i + 1 < num_leafs; ++i) m_pieces.push_back(new Piece);
-  for (uint j = 0; j + 1 < m_pieces.size(); j += 2) {
-    Piece *dp1 = m_pieces[j], *dp2 = m_pieces[j + 1], *dp = m_pieces[num_leafs + j / 2];
-    if (dp1->level != dp2->level) throw std::logic_error("dps must have same level");
-    dp->level = dp1->level + 1;
-    dp->msgraph = GridMSComplex::merge_up(*dp1->msgraph, *dp2->msgraph);
+  // pairs (j, j+1) of one level are merged into num_leafs + j/2; a level's
+  // pairs start where the previous level's outputs start
+  for (uint lo = 0, w = num_leafs; w > 1; lo += w, w /= 2) {
+    par_for(w / 2, nthreads, [&](uint k) {
+      uint j = lo + 2 * k;
+      Piece *dp1 = m_pieces[j], *dp2 = m_pieces[j + 1], *dp = m_pieces[num_leafs + j / 2];
+      if (dp1->level != dp2->level) throw std::logic_error("dps must have same level");
+      dp->level = dp1->level + 1;
+      dp->msgraph = GridMSComplex::merge_up(*dp1->msgraph, *dp2->msgraph);
+    });
   }
   double t2 = now_s();
   S.t_merge_up = t2 - t1;
@@ -944,20 +980,27 @@ void DataManager::run(PipelineSnapshots *snaps) {
   // reference's loop bound shifts by -1; no iteration runs.
   uint num_graphs = (1u << (m_num_levels + 1)) - 1;
   if (m_num_levels >= 1) {
-    for (uint j = 1; j < (1u << (m_num_levels - 1)); ++j) {
-      Piece *dp = m_pieces[num_graphs - j];
-      Piece *dp1 = m_pieces[num_graphs - 2 * j];
-      Piece *dp2 = m_pieces[num_graphs - 2 * j - 1];
-      dp->msgraph->merge_down(*dp1->msgraph, *dp2->msgraph);
+    // node j's children are 2j and 2j+1: one level [a, 2a) at a time
+    const uint jn = 1u << (m_num_levels - 1);
+    for (uint a = 1; a < jn; a *= 2) {
+      const uint b = std::min(2 * a, jn);
+      par_for(b - a, nthreads, [&](uint k) {
+        uint j = a + k;
+        Piece *dp = m_pieces[num_graphs - j];
+        Piece *dp1 = m_pieces[num_graphs - 2 * j];
+        Piece *dp2 = m_pieces[num_graphs - 2 * j - 1];
+        dp->msgraph->merge_down(*dp1->msgraph, *dp2->msgraph);
+      });
     }
   }
   // finalMergeDownPiecesInRange (:359-395), run for even leaves in
   // collectSubdomainManifolds (:466-485)
   if (m_num_levels != 0) {
-    for (uint j = 0; j < num_leafs; j += 2) {
+    par_for(num_leafs / 2, nthreads, [&](uint k) {
+      uint j = 2 * k;
       Piece *dp = m_pieces[num_leafs + j / 2];
       dp->msgraph->merge_down(*m_pieces[j]->msgraph, *m_pieces[j + 1]->msgraph);
-    }
+    });
   }
   double t4 = now_s();
   S.t_merge_down = t4 - t3;
@@ -965,12 +1008,12 @@ void DataManager::run(PipelineSnapshots *snaps) {
 
   // collectManifold (:397-422): re-init (flags reset, critical list kept)
   // and re-run work(), then postMergeFillDiscs.
-  for (uint i = 0; i < num_leafs; ++i) {
+  par_for(num_leafs, nthreads, [&](uint i) {
     Piece *dp = m_pieces[i];
     initPiece(dp);
     dp->dataset->work();
     dp->dataset->postMergeFillDiscs(dp->msgraph);
-  }
+  });
   double t5 = now_s();
   S.t_manifolds = t5 - t4;
   snapshot(S.after_discs);

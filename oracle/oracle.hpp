@@ -199,7 +199,10 @@ class DataManager {
   DataManager(const fn_t *data, int size_x, int size_y, int levels, int lx, int ly,
               double simp_tresh, bool keep_snapshots);
   ~DataManager();
-  void run(PipelineSnapshots *snaps);
+  // nthreads > 1: independent pieces of one phase (leaf tiles, merges of
+  // one tree level) run on that many threads, as with the reference's
+  // --num-parallel; results do not depend on it.
+  void run(PipelineSnapshots *snaps, int nthreads = 1);
 
   const fn_t *m_data;
   int m_size_x, m_size_y;

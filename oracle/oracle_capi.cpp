@@ -73,13 +73,15 @@ struct OrcPipeline {
 // The whole GridDataManager compute order on a size_x x size_y grid (x
 // fastest).  levels: slab levels (split along y); lx/ly > 0 selects a 2-D
 // tiling of 2^lx x 2^ly tiles instead.  keep != 0 stores complex dumps of
-// every piece after every phase.
-void *orc_pipeline_run(const float *data, int sx, int sy, int levels, int lx, int ly, double thresh, int keep) {
+// every piece after every phase.  nthreads > 1 runs the independent pieces
+// of each phase on that many threads (the reference's --num-parallel).
+void *orc_pipeline_run(const float *data, int sx, int sy, int levels, int lx, int ly, double thresh, int keep,
+                       int nthreads) {
   OrcPipeline *p = new OrcPipeline;
   try {
     p->dm = new DataManager(data, sx, sy, levels, lx, ly, thresh, keep != 0);
     p->nleaves = 1 << p->dm->m_num_levels;
-    p->dm->run(&p->snaps);
+    p->dm->run(&p->snaps, nthreads);
     return p;
   } catch (std::exception &e) {
     g_err = e.what();

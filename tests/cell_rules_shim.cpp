@@ -143,3 +143,42 @@ extern "C" int cr_codes_region(const float *win, const int *rect, const int *ext
   }
   return 0;
 }
+
+// The row-step edge candidates (hcand_keys / ecand_keys, which decide an
+// edge with both quads usable by the smaller off-edge corners) against the
+// per-cell rules (hedge_cand / vedge_cand with quad_lt on sorted corners),
+// over every assignment of nv values to the six vertices around a
+// horizontal and a vertical edge.  Returns the number of disagreements.
+extern "C" long cr_edge_forms_check(int nv) {
+  mscx::TileGeom g;
+  g.vx = g.vy = 4;
+  g.cw = g.ch = 7;
+  g.rx0 = g.ry0 = 0;
+  g.rx1 = g.ry1 = 6;
+  float win[16];
+  auto KS = [&](int i, int j) { return mscx::VKey{mscx::fkey(win[4 * j + i]), i, j}; };
+  auto fk = [&](int i, int j) { return mscx::fkey(win[4 * j + i]); };
+  long total = 1;
+  for (int k = 0; k < 6; ++k) total *= nv;
+  long bad = 0;
+  for (long code = 0; code < total; ++code) {
+    int v[6];
+    long c = code;
+    for (int k = 0; k < 6; ++k) v[k] = (int)(c % nv), c /= nv;
+    // horizontal edge (1,1)-(2,1): rows 0..2 of columns 1, 2
+    for (float &w : win) w = 0.5f;
+    win[4 * 1 + 1] = (float)v[0], win[4 * 1 + 2] = (float)v[1];
+    win[4 * 2 + 1] = (float)v[2], win[4 * 2 + 2] = (float)v[3];
+    win[4 * 0 + 1] = (float)v[4], win[4 * 0 + 2] = (float)v[5];
+    if (mscx::hedge_cand(KS, g, 1, 1) != mscx::hcand_keys(fk(1, 1), fk(2, 1), fk(1, 2), fk(2, 2), fk(1, 0), fk(2, 0)))
+      ++bad;
+    // vertical edge (1,1)-(1,2): columns 0..2 of rows 1, 2
+    for (float &w : win) w = 0.5f;
+    win[4 * 1 + 1] = (float)v[0], win[4 * 2 + 1] = (float)v[1];
+    win[4 * 1 + 2] = (float)v[2], win[4 * 2 + 2] = (float)v[3];
+    win[4 * 1 + 0] = (float)v[4], win[4 * 2 + 0] = (float)v[5];
+    if (mscx::vedge_cand(KS, g, 1, 1) != mscx::ecand_keys(fk(1, 1), fk(1, 2), fk(2, 1), fk(2, 2), fk(0, 1), fk(0, 2)))
+      ++bad;
+  }
+  return bad;
+}

@@ -74,3 +74,14 @@ def test_random_tiles(shim, kind, tiling):
         data = rng_field(kind, sx, sy, seed)
         for rect, ext in split_pieces(sx, sy, **tiling):
             assert run_case(shim, window(data, ext), rect, ext), (kind, tiling, seed, rect, ext)
+
+
+def test_edge_candidate_closed_form(shim):
+    """The row-step edge candidates decide an edge whose two quads are both
+    usable by comparing the quads' smaller off-edge corners; every value
+    pattern (with ties) of the six vertices around an edge agrees with the
+    per-cell rule's quad_lt on sorted corners (grid_dataset.cpp:1354-1372,
+    1519-1536)."""
+    shim.cr_edge_forms_check.restype = ctypes.c_long
+    assert shim.cr_edge_forms_check(6) == 0
+    assert shim.cr_edge_forms_check(3) == 0

@@ -163,3 +163,17 @@ def test_simplify_cancels_adjacent_dimensions_and_keeps_euler():
         assert abs(_dim(*unpack(after[i]["cell"])) - _dim(*unpack(after[j]["cell"]))) == 1
     dims = [_dim(*unpack(after[i]["cell"])) for i in live]
     assert dims.count(0) - dims.count(1) + dims.count(2) == 1
+
+
+@pytest.mark.parametrize("levels,lx,ly", [(3, 0, 0), (0, 2, 2), (0, 2, 1)])
+def test_parallel_pipeline_equals_single_threaded(levels, lx, ly):
+    """The oracle's --num-parallel form (independent tiles and merge-tree
+    nodes of one level on several threads) gives the same complexes after
+    every phase as the -st order."""
+    f = rng_field("smooth", 200, 170, seed=12)
+    a = oracle.Pipeline(f, levels=levels, lx=lx, ly=ly, threshold=0.02)
+    b = oracle.Pipeline(f, levels=levels, lx=lx, ly=ly, threshold=0.02, threads=6)
+    assert a.num_pieces() == b.num_pieces()
+    for st in range(5):
+        for p in range(a.num_pieces()):
+            assert np.array_equal(a.dump(st, p), b.dump(st, p)), (st, p)
