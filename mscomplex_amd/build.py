@@ -4,6 +4,7 @@ Usage: python -m mscomplex_amd.build   (or mscomplex_amd.build.build())
 """
 import concurrent.futures as _cf
 import os
+import shutil
 import subprocess
 import sys
 
@@ -33,12 +34,21 @@ def _hipcc():
     return "hipcc"
 
 
+def _host_cxx():
+    c = os.environ.get("MSCX_HOST_CXX")
+    if c:
+        return c
+    return shutil.which("g++") or _hipcc()
+
+
 def _compile(src, device):
     out = os.path.join(OBJ, os.path.basename(src) + ".o")
     if device:
         cmd = [_hipcc(), *HIP_FLAGS, "-c", src, "-o", out]
     else:
-        cmd = [_hipcc(), "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", out]
+        # host code goes through the system C++ compiler: its code for the
+        # simplification heap's sift-down is measurably faster than clang's
+        cmd = [_host_cxx(), *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -702,33 +702,38 @@ __global__ __launch_bounds__(256) void k_incidence(const TileDesc *__restrict__ 
 
 __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict__ descs, const int *__restrict__ prefix,
                                                      int n) {
+  // one block = 64 x 4 vertices of the interior rect; a thread labels its
+  // vertex (2x, 2y) and the quad (2x+1, 2y+1) above and right of it
   const int b = blockIdx.x;
   const int t = find_tile(prefix, n, b);
-  const DiscDesc &D = descs[t];
+  const DiscDesc D = descs[t];
   const int nvx = (D.rx1 - D.rx0) / 2 + 1, nvy = (D.ry1 - D.ry0) / 2 + 1;
-  const int64_t nv = (int64_t)nvx * nvy, nq = (int64_t)(nvx - 1) * (nvy - 1);
-  const int64_t item = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
-  if (item >= nv + nq) return;
-  const bool vert = item < nv;
-  const int64_t r = vert ? item : item - nv;
-  const int w = vert ? nvx : nvx - 1;
-  const int lx = D.rx0 - D.ox + (vert ? 0 : 1) + 2 * (int)(r % w);
-  const int ly = D.ry0 - D.oy + (vert ? 0 : 1) + 2 * (int)(r / w);
-  const uint32_t o = D.own[(int64_t)ly * D.own_pitch + lx];
-  uint32_t lab = 0xFFFFFFFFu;
-  if (o != OWN_NONE) {
+  const int nbx = (nvx + 63) / 64, lb = b - prefix[t];
+  const int x = (lb % nbx) * 64 + (int)(threadIdx.x & 63), y = (lb / nbx) * 4 + (int)(threadIdx.x >> 6);
+  if (x >= nvx || y >= nvy) return;
+  const bool hasq = x + 1 < nvx && y + 1 < nvy;
+  const int lx = D.rx0 - D.ox + 2 * x, ly = D.ry0 - D.oy + 2 * y;
+  const GLB uint32_t *own = glb(D.own);
+  const GLB uint8_t *codes = glb(D.codes);
+  const int64_t ov_at = (int64_t)ly * D.own_pitch + lx, cv_at = (int64_t)ly * D.pitch + lx;
+  const uint32_t ov = own[ov_at], oq = hasq ? own[ov_at + D.own_pitch + 1] : OWN_NONE;
+  const uint8_t cv = codes[cv_at], cq = hasq ? codes[cv_at + D.pitch + 1] : (uint8_t)0;
+  auto label = [&](uint32_t o, uint8_t c) -> uint32_t {
+    if (o == OWN_NONE) return 0xFFFFFFFFu;
     if (o >= D.nlab) {
       atomicOr(D.err, 2u);
-    } else {
-      const uint8_t c = D.codes[(int64_t)ly * D.pitch + lx];
-      lab = (c & CF_CRITICAL) ? D.lab_self[o] : D.lab_owned[o];
-      if (lab == LAB_ERR) {
-        atomicOr(D.err, 1u);
-        lab = 0xFFFFFFFFu;
-      }
+      return 0xFFFFFFFFu;
     }
-  }
-  (vert ? D.lab_v : D.lab_q)[r] = lab;
+    const uint32_t lab = (c & CF_CRITICAL) ? D.lab_self[o] : D.lab_owned[o];
+    if (lab == LAB_ERR) {
+      atomicOr(D.err, 1u);
+      return 0xFFFFFFFFu;
+    }
+    return lab;
+  };
+  const uint32_t lv = label(ov, cv), lq = label(oq, cq);
+  D.lab_v[(int64_t)y * nvx + x] = lv;
+  if (hasq) D.lab_q[(int64_t)y * (nvx - 1) + x] = lq;
 }
 
 // -------------------------------------------------------------- launchers
@@ -769,9 +774,8 @@ int plan_disc_blocks(const DiscDesc *d, int n, int *prefix) {
   int acc = 0;
   for (int t = 0; t < n; ++t) {
     prefix[t] = acc;
-    int64_t nvx = (d[t].rx1 - d[t].rx0) / 2 + 1, nvy = (d[t].ry1 - d[t].ry0) / 2 + 1;
-    int64_t items = nvx * nvy + (nvx - 1) * (nvy - 1);
-    acc += (int)((items + 255) / 256);
+    const int nvx = (d[t].rx1 - d[t].rx0) / 2 + 1, nvy = (d[t].ry1 - d[t].ry0) / 2 + 1;
+    acc += ((nvx + 63) / 64) * ((nvy + 3) / 4);  // k_disc_labels: 64 x 4 vertices per block
   }
   prefix[n] = acc;
   return acc;
