@@ -576,38 +576,11 @@ __global__ __launch_bounds__(256) void k_scan_down(const TileDesc *__restrict__ 
 }
 
 // ---------------------------------------------------------------- compaction
-// One wave per 8 consecutive (cell row, segment) items: critical cells are
-// written in raster order at their scanned offsets with their cell function
-// value (grid_dataset.cpp:1538-1552: float sum in getCellPoints order, then
-// the division).
-
-__device__ __forceinline__ float vval(const TileDesc &T, int i, int j) { return T.verts[(int64_t)j * T.vpitch + i]; }
-
-__device__ __forceinline__ float cell_fn(const TileDesc &T, int cx, int cy) {
-  int i = cx >> 1, j = cy >> 1;
-  int dx = cx & 1, dy = cy & 1;
-  float fn = 0.0f;
-  if (!dx && !dy) {
-    fn += vval(T, i, j);
-    return fn / 1.0f;
-  }
-  if (dx && !dy) {  // points (x+1,y), (x-1,y)
-    fn += vval(T, i + 1, j);
-    fn += vval(T, i, j);
-    return fn / 2.0f;
-  }
-  if (!dx && dy) {  // points (x,y+1), (x,y-1)
-    fn += vval(T, i, j + 1);
-    fn += vval(T, i, j);
-    return fn / 2.0f;
-  }
-  // points (x+1,y+1), (x+1,y-1), (x-1,y-1), (x-1,y+1)
-  fn += vval(T, i + 1, j + 1);
-  fn += vval(T, i + 1, j);
-  fn += vval(T, i, j);
-  fn += vval(T, i, j + 1);
-  return fn / 4.0f;
-}
+// One wave per 8 consecutive (cell row, segment) items; lane l holds cells
+// 2l and 2l+1 of a segment: critical cells are written in raster order at
+// their scanned offsets with their cell function value
+// (grid_dataset.cpp:1538-1552: float sum in getCellPoints order, then the
+// division).
 
 constexpr int COMPACT_ITEMS = 8;  // (row, segment) items per wave
 
@@ -617,34 +590,76 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *__restrict__ ti
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc T = tiles[t];
   const int lane = threadIdx.x & 63;
-  const int64_t n = (int64_t)T.g.ch * T.nseg;
-  const int64_t first = ((int64_t)(b - prefix[t]) * 4 + (threadIdx.x >> 6)) * COMPACT_ITEMS;
-  const uint64_t lt = lanes_below(lane);
-  // the flags of all items first, so their loads are in flight together
+  const int nseg = T.nseg, n = T.g.ch * nseg;
+  const int first = __builtin_amdgcn_readfirstlane(((b - prefix[t]) * 4 + (int)(threadIdx.x >> 6)) * COMPACT_ITEMS);
+  const GLB uint64_t *critbits = glb((const uint64_t *)T.critbits);
+  const GLB uint32_t *seg_offs = glb((const uint32_t *)T.seg_offs);
+  const GLB float *verts = glb(T.verts);
+  const int64_t vpitch = T.vpitch;
+  // 1. every item's flags and offset; then, for the items with critical
+  //    cells, the corner values this lane's critical cells need (a = (i, j),
+  //    b = (i+1, j), c = (i, j+1), d = (i+1, j+1)), all in flight together
   uint64_t fw[2 * COMPACT_ITEMS];
+  uint32_t so[COMPACT_ITEMS];
 #pragma unroll
   for (int it = 0; it < COMPACT_ITEMS; ++it) {
     const bool in = first + it < n;
-    fw[2 * it] = in ? T.critbits[2 * (first + it)] : 0ull;
-    fw[2 * it + 1] = in ? T.critbits[2 * (first + it) + 1] : 0ull;
+    fw[2 * it] = in ? critbits[2 * (first + it)] : 0ull;
+    fw[2 * it + 1] = in ? critbits[2 * (first + it) + 1] : 0ull;
+    so[it] = in ? seg_offs[first + it] : 0u;
   }
+  // (row, segment) of each item, advanced in scalar registers
+  int srow[COMPACT_ITEMS], sseg[COMPACT_ITEMS];
+  {
+    int row = first / nseg, seg = first - row * nseg;
+#pragma unroll
+    for (int it = 0; it < COMPACT_ITEMS; ++it) {
+      srow[it] = row, sseg[it] = seg;
+      if (++seg == nseg) seg = 0, ++row;
+    }
+  }
+  float va[COMPACT_ITEMS], vb[COMPACT_ITEMS], vc[COMPACT_ITEMS], vd[COMPACT_ITEMS];
+#pragma unroll
+  for (int it = 0; it < COMPACT_ITEMS; ++it) {
+    va[it] = vb[it] = vc[it] = vd[it] = 0.0f;
+    const uint64_t w0 = fw[2 * it], w1 = fw[2 * it + 1];
+    if (!(w0 | w1)) continue;  // uniform across the wave
+    const int seg = sseg[it], row = srow[it];
+    const bool c0 = (w0 >> lane) & 1u, c1 = (w1 >> lane) & 1u;
+    const int i = seg * (SEG / 2) + lane, j = row >> 1;
+    const GLB float *p = verts + (int64_t)j * vpitch + i;
+    if (c0 || c1) va[it] = p[0];
+    if (c1) vb[it] = p[1];
+    if ((row & 1) && (c0 || c1)) vc[it] = p[vpitch];
+    if ((row & 1) && c1) vd[it] = p[vpitch + 1];
+  }
+  // 2. ranks, packed ids and cell values (get_cell_fn: float sum from 0 in
+  //    getCellPoints order, then the division)
 #pragma unroll
   for (int it = 0; it < COMPACT_ITEMS; ++it) {
     const uint64_t w0 = fw[2 * it], w1 = fw[2 * it + 1];
     if (!(w0 | w1)) continue;
-    const int64_t item = first + it;
-    const int row = (int)(item / T.nseg), seg = (int)(item % T.nseg);
+    const int seg = sseg[it], row = srow[it];
     const uint32_t c0 = (uint32_t)(w0 >> lane) & 1u, c1 = (uint32_t)(w1 >> lane) & 1u;
-    uint32_t k = T.seg_offs[item] + (uint32_t)(__popcll(w0 & lt) + __popcll(w1 & lt));
+    uint32_t k = rank_below(w1, rank_below(w0, so[it]));
     const int cx = seg * SEG + 2 * lane;
+    const float a = va[it], bb = vb[it], c = vc[it], d = vd[it];
+    float f0, f1;
+    if (row & 1) {  // vertical edge (c, a), quad (d, b, a, c)
+      f0 = ((0.0f + c) + a) / 2.0f;
+      f1 = ((((0.0f + d) + bb) + a) + c) / 4.0f;
+    } else {        // vertex (a), horizontal edge (b, a)
+      f0 = (0.0f + a) / 1.0f;
+      f1 = ((0.0f + bb) + a) / 2.0f;
+    }
     if (c0 && k < T.kcap) {
       T.crit[k] = pack_cell(T.ox + cx, T.oy + row);
-      T.crit_fn[k] = cell_fn(T, cx, row);
+      T.crit_fn[k] = f0;
     }
     k += c0;
     if (c1 && k < T.kcap) {
       T.crit[k] = pack_cell(T.ox + cx + 1, T.oy + row);
-      T.crit_fn[k] = cell_fn(T, cx + 1, row);
+      T.crit_fn[k] = f1;
     }
   }
 }
