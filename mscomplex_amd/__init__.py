@@ -71,6 +71,13 @@ class DeviceBuffer:
         check(lib().mscx_dev_upload(ctypes.c_void_p(b.ptr), ptr(a), a.nbytes))
         return b
 
+    def upload(self, a):
+        """Overwrite the start of the buffer with the bytes of `a`."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        check(lib().mscx_dev_upload(ctypes.c_void_p(self.ptr), ptr(a), a.nbytes))
+
     def download(self, out):
         check(lib().mscx_dev_download(ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
         return out
@@ -90,6 +97,21 @@ class DeviceBuffer:
 def profile(enable=True):
     """Enable (and reset) per-stage HIP-event timing inside the library."""
     check(lib().mscx_profile(1 if enable else 0))
+
+
+def set_graphs(on=True):
+    """Submit the tile stages that need no host decision as a replayed HIP
+    graph (True, the default) or as direct launches; returns the old setting."""
+    prev = ctypes.c_int(0)
+    check(lib().mscx_set_graphs(1 if on else 0, ctypes.byref(prev)))
+    return bool(prev.value)
+
+
+def graph_stats():
+    """(captures, replays) of the stage graph since initialisation."""
+    c, r = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    check(lib().mscx_graph_stats(ctypes.byref(c), ctypes.byref(r)))
+    return int(c.value), int(r.value)
 
 
 STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex")

@@ -192,6 +192,24 @@ int mscx_profile(int enable) {
   });
 }
 
+int mscx_set_graphs(int on, int *previous) {
+  return mscx::guarded([&] {
+    mscx::Runtime &rt = mscx::runtime();
+    std::lock_guard<std::mutex> lk(rt.mu);
+    if (previous) *previous = rt.graphs ? 1 : 0;
+    rt.graphs = on != 0;
+  });
+}
+
+int mscx_graph_stats(uint64_t *captures, uint64_t *replays) {
+  return mscx::guarded([&] {
+    mscx::Runtime &rt = mscx::runtime();
+    std::lock_guard<std::mutex> lk(rt.mu);
+    if (captures) *captures = rt.graph_captures;
+    if (replays) *replays = rt.graph_replays;
+  });
+}
+
 int mscx_profile_read(double *ms, uint32_t *counts, int n) {
   return mscx::guarded([&] {
     mscx::StageTimer &t = mscx::stage_timer();
@@ -240,6 +258,11 @@ int mscx_shutdown(void) {
     std::lock_guard<std::mutex> lk(rt.mu);
     if (rt.device < 0) return;
     MSCX_HIP(hipStreamSynchronize(rt.stream));
+    if (rt.graph_exec) {
+      MSCX_HIP(hipGraphExecDestroy(rt.graph_exec));
+      rt.graph_exec = nullptr;
+      rt.graph_key.clear();
+    }
   });
 }
 

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/mscx.h"
 
@@ -61,6 +62,11 @@ struct Runtime {
   hipStream_t stream = nullptr;  // tile stages (run_tiles), serialised by mu
   hipStream_t aux = nullptr;     // manifold label pass, serialised by its own lock
   std::mutex mu;
+  // stage graphs (run_tiles), guarded by mu
+  bool graphs = true;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uintptr_t> graph_key;  // launch configuration graph_exec was captured with
+  uint64_t graph_captures = 0, graph_replays = 0;
 };
 Runtime &runtime();
 void ensure_init();
