@@ -209,15 +209,16 @@ def batched_bench(n=1024, seed=7):
             "autotune_s": round(t_tune, 2), "generate_s": round(t_gen, 2)}
 
 
-def served_bench(n=24, seed=11, repeat_frac=0.25, load=2.0, max_side=4096):
+def served_bench(n=24, seed=11, repeat_frac=0.25, loads=((1.0, 5), (2.0, 6)), max_side=4096):
     """The served configuration: square grids with sides log-uniform in
     [128, max_side] (8192 in the full configuration), alternating fields,
     value-noise thresholds uniform in [0, 5%] of the range (Gaussian sums at
     0, as in the batched run), a quarter of the requests repeating an
     earlier grid at a new threshold.  Plain = mscx_grid_run per request, one at a time (no
-    residency: repeats recompute).  Served = Poisson arrivals at `load` x the
-    plain throughput into the continuous-batching server (200 GB HBM budget,
-    grids resident under their key for the repeats)."""
+    residency: repeats recompute).  Served = Poisson arrivals at each of
+    `loads` (x the plain throughput, with the arrival seed) into the
+    continuous-batching server (200 GB HBM budget, grids resident under their
+    key for the repeats), each run starting with an empty server."""
     import threading
     import mscomplex_amd as mx
     r = np.random.default_rng(seed)
@@ -246,48 +247,57 @@ def served_bench(n=24, seed=11, repeat_frac=0.25, load=2.0, max_side=4096):
         plain_lat.append(time.perf_counter() - a)
     t_plain = time.perf_counter() - t0
     print(f"[bench] served: plain {t_plain:.2f}s for {n} requests", file=sys.stderr, flush=True)
-    rate = load * n / t_plain
-    gaps = r.exponential(1.0 / rate, size=n)
-
     srv = mx.Server(200 << 30)
-    tickets = [None] * n
-    submitted = {}
 
-    def client():
-        t_next = time.perf_counter()
-        for k, (gi, th) in enumerate(reqs):
-            t_next += gaps[k]
-            d = t_next - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            lx, ly = tiling[gi]
-            tickets[k] = srv.submit(base[gi], lx, ly, th, key=gi + 1)
-            submitted[gi] = True
+    def run_at(load, gap_seed):
+        rate = load * n / t_plain
+        gaps = np.random.default_rng(gap_seed).exponential(1.0 / rate, size=n)
+        tickets = [None] * n
 
-    t0 = time.perf_counter()
-    th_ = threading.Thread(target=client)
-    th_.start()
-    th_.join()
-    lat = []
-    failed = 0
-    for t in tickets:
-        st, err, l, _ = srv.wait(t, timeout=600)
-        failed += st != 0
-        lat.append(l)
-        srv.release(t)
-    t_served = time.perf_counter() - t0
-    stats = srv.stats()
-    srv.close()
+        def client():
+            t_next = time.perf_counter()
+            for k, (gi, th) in enumerate(reqs):
+                t_next += gaps[k]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                lx, ly = tiling[gi]
+                tickets[k] = srv.submit(base[gi], lx, ly, th, key=gi + 1)
+
+        s0 = srv.stats()
+        t0 = time.perf_counter()
+        th_ = threading.Thread(target=client)
+        th_.start()
+        th_.join()
+        lat = []
+        failed = 0
+        for t in tickets:
+            st, err, l, _ = srv.wait(t, timeout=600)
+            failed += st != 0
+            lat.append(l)
+            srv.release(t)
+        t_served = time.perf_counter() - t0
+        s1 = srv.stats()
+        print(f"[bench] served: load {load}x plain, {t_served:.2f}s", file=sys.stderr, flush=True)
+        return {"load_x_plain": load, "offered_rate_per_s": round(rate, 3),
+                "throughput_per_s": round(n / t_served, 3),
+                "latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
+                "rounds": s1["rounds"] - s0["rounds"], "repeats_served_resident": s1["repeats"] - s0["repeats"],
+                "failed": int(failed)}
+
     pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2)
+    # the residents of the first run are evicted so that both runs start cold
+    runs = []
+    for load, gs in loads:
+        runs.append(run_at(load, gs))
+        srv.close()
+        srv = mx.Server(200 << 30)
+    srv.close()
     return {"requests": n, "distinct_grids": len(base), "repeats": n - len(base),
             "vertices_mean": int(np.mean([base[gi].size for gi, _ in reqs])),
-            "offered_rate_per_s": round(rate, 3),
-            "served_throughput_per_s": round(n / t_served, 3),
-            "served_latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
             "plain_throughput_per_s": round(n / t_plain, 3),
             "plain_latency_ms": {"p50": pct(plain_lat, 50), "p95": pct(plain_lat, 95), "p99": pct(plain_lat, 99)},
-            "rounds": stats["rounds"], "failed": int(failed), "hbm_budget_GB": 200, "max_side": max_side,
-            "load_x_plain": load}
+            "served": runs, "hbm_budget_GB": 200, "max_side": max_side}
 
 
 def main():

@@ -72,8 +72,9 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 // segment
 // Trace link of a vertex (q = 0) or quad (q = 1) entry (li, lj) from its
 // code: its root terminal when critical, none when unpaired, else the entry
-// across its pair (DIR_PX, NX, PY, NY step by +1, -1, +TX, -TX) or, when
-// that leaves the tile through side s = dir - 1, the exit terminal.
+// across its pair (DIR_PX, NX, PY, NY step by +1, -1, +TX, -TX), as the byte
+// offset of that entry in the block's 16-bit entry array, or, when the step
+// leaves the tile through side s = dir - 1, the exit terminal.
 // exits: bit s set when this entry's step in direction s + 1 leaves the
 // tile.  Every value is computed before it is selected (no divergent
 // branches).
@@ -84,7 +85,7 @@ __device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, ui
   const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
   const uint32_t s = (uint32_t)code_dir(c) - 1u;  // side 0..3 when paired
   const int step = __builtin_amdgcn_sbfe((int)STEP, (int)(8 * s), 8);
-  const uint32_t in_link = (uint32_t)(q * NV + lj * TX + li + step);
+  const uint32_t in_link = 2u * (uint32_t)(q * NV + lj * TX + li + step);  // LDS byte offset
   const uint32_t exit_link = T_EXIT | (s << 6) | (s < 2 ? (uint32_t)lj : (uint32_t)li);
   const uint32_t link = __builtin_amdgcn_ubfe(exits, s, 1) ? exit_link : in_link;
   const uint32_t paired = (c & CF_PAIRED) ? link : T_NONE;
@@ -189,6 +190,36 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   }
 }
 
+// Keys of the block's vertices and their one-vertex halo into LDS: all of a
+// lane's global loads are issued before its first LDS store.
+template <bool INTERIOR>
+__device__ __forceinline__ void load_keys(const TileDesc &T, const TileGeom &g, uint32_t *__restrict__ sk, int i0,
+                                          int j0, int lane, int wv) {
+  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, KH = TY + 3;
+  constexpr int NR = (KH + 3) / 4;  // rows per wave
+  const GLB float *verts = glb((const float *)T.verts);
+  const int64_t vpitch = T.vpitch;
+  const int gi = i0 - 1 + lane, gi2 = i0 + TX - 1 + lane;
+  float vm[NR], vh[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int r = wv + 4 * k;  // uniform across the wave
+    const int gj = j0 - 1 + r;
+    const bool rowok = r < KH && (INTERIOR || (gj >= 0 && gj < g.vy));
+    const GLB float *row = verts + (int64_t)gj * vpitch;
+    vm[k] = (rowok && (INTERIOR || (gi >= 0 && gi < g.vx))) ? row[gi] : 0.0f;
+    vh[k] = (rowok && lane < 3 && (INTERIOR || gi2 < g.vx)) ? row[gi2] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int r = wv + 4 * k;
+    if (r < KH) {
+      sk[r * KW + lane] = fkey(vm[k]);
+      if (lane < 3) sk[r * KW + TX + lane] = fkey(vh[k]);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                     int ntiles, uint32_t *err) {
   constexpr int TX = GRAD_TX, TY = GRAD_TY;
@@ -208,33 +239,19 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   const int i0 = bx * TX, j0 = by * TY;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
+  // A block is interior when every cell of it and of its one-vertex halo
+  // lies strictly inside the interior rect: no boundary rule applies and
+  // every key of the window is inside the ext.
+  const bool interior = (2 * i0 - 1 > g.rx0) && (2 * (i0 + TX) + 1 < g.rx1) && (2 * j0 - 1 > g.ry0) &&
+                        (2 * (j0 + TY) + 1 < g.ry1);
   // 1. keys (0 outside the ext window; the rules never read those)
-  {
-    constexpr int NR = (KH + 3) / 4;
-    float vm[NR], vh[NR];
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const int r = wv + 4 * k;
-      const int gj = j0 - 1 + r;
-      const bool rowok = r < KH && gj >= 0 && gj < g.vy;
-      const int gi = i0 - 1 + lane, gi2 = i0 + TX - 1 + lane;
-      vm[k] = (rowok && gi >= 0 && gi < g.vx) ? T.verts[(int64_t)gj * T.vpitch + gi] : 0.0f;
-      vh[k] = (rowok && lane < 3 && gi2 < g.vx) ? T.verts[(int64_t)gj * T.vpitch + gi2] : 0.0f;
-    }
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const int r = wv + 4 * k;
-      if (r < KH) {
-        sk[r * KW + lane] = fkey(vm[k]);
-        if (lane < 3) sk[r * KW + TX + lane] = fkey(vh[k]);
-      }
-    }
-  }
+  if (interior)
+    load_keys<true>(T, g, sk, i0, j0, lane, wv);
+  else
+    load_keys<false>(T, g, sk, i0, j0, lane, wv);
   __syncthreads();
 
   // 2. codes, critical flags per row segment, trace links
-  const bool interior = (2 * i0 - 1 > g.rx0) && (2 * (i0 + TX) + 1 < g.rx1) && (2 * j0 - 1 > g.ry0) &&
-                        (2 * (j0 + TY) + 1 < g.ry1);
   if (interior)
     grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, wv * RW);
   else
@@ -260,25 +277,28 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   //    guards bad input).
   {
     constexpr int PER = 2 * NV / 256;
-    static_assert((2 * NV & (2 * NV - 1)) == 0 && 2 * NV <= 0x8000, "entry indices below the terminal bit");
+    // links are byte offsets into sl, below the terminal bit
+    static_assert((4 * NV & (4 * NV - 1)) == 0 && 4 * NV <= 0x8000, "entry offsets below the terminal bit");
+    const char *slb = (const char *)sl;
     uint32_t cur[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) cur[k] = sl[tid + 256 * k];
-    uint32_t all = 0x8000u;
+    uint32_t least = 0xFFFFu;  // all terminal: every entry >= 0x8000
 #pragma unroll
-    for (int k = 0; k < PER; ++k) all &= cur[k];
-    for (int round = 0; !all && round < 2 * NV; ++round) {
+    for (int k = 0; k < PER; ++k) least = min(least, cur[k]);
+    for (int round = 0; least < 0x8000u && round < 2 * NV; ++round) {
       uint32_t nxt[PER];
 #pragma unroll
-      for (int k = 0; k < PER; ++k) nxt[k] = sl[cur[k] & (2 * NV - 1)];
-      all = 0x8000u;
+      for (int k = 0; k < PER; ++k) nxt[k] = *(const uint16_t *)(slb + (cur[k] & (4 * NV - 2)));
+      least = 0xFFFFu;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
-        cur[k] = (cur[k] & 0x8000u) ? cur[k] : nxt[k];
+        cur[k] = cur[k] < 0x8000u ? nxt[k] : cur[k];
         sl[tid + 256 * k] = (uint16_t)cur[k];
-        all &= cur[k];
+        least = min(least, cur[k]);
       }
     }
+    const uint32_t all = least >= 0x8000u;
     if (!all) {
       atomicOr(err, 2u);
 #pragma unroll
