@@ -143,9 +143,12 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   GLB uint32_t *const seg_counts = glb(T.seg_counts);
   // sides a step from this lane's column leaves the tile through (link_of)
   const uint32_t ex_col = (li == TX - 1 ? 1u : 0u) | (li == 0 ? 2u : 0u);
+  // interior blocks lie inside the ext: every row exists, and the walk is
+  // unrolled so that the window slides by register renaming
+#pragma unroll(INTERIOR ? RW : 1)
   for (int rr = 0; rr < RW; ++rr) {
     const int lj = lj0 + rr, j = K.j;
-    if (j >= g.vy) break;  // uniform across the wave
+    if (!INTERIOR && j >= g.vy) break;  // uniform across the wave
     int vc01, hc01, ec00;
     row_own<INTERIOR>(K, g, vc01, hc01, ec00);
     // the next column's candidates from the next lane; lane 63's next
