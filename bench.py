@@ -300,6 +300,19 @@ def served_bench(n=24, seed=11, repeat_frac=0.25, loads=((1.0, 5), (2.0, 6)), ma
             "served": runs, "hbm_budget_GB": 200, "max_side": max_side}
 
 
+def copy_probe():
+    """A plain copy of known size: the counters' bytes for it calibrate the
+    HBM traffic figures read from rocprofv3 (FETCH_SIZE, WRITE_SIZE)."""
+    import torch
+    n = 64 * 1024 * 1024  # 256 MiB of f32
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy probe: 3 copies of {4 * n} B", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +324,13 @@ def main():
     ap.add_argument("--no-worst", action="store_true")
     ap.add_argument("--served-max-side", type=int, default=4096)
     ap.add_argument("--batched-grids", type=int, default=1024)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the calibration kernel for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is single-GPU "
               f"in this build and multi-GPU sharding is design-only.", file=sys.stderr)
