@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_grad_local launch of the main workload, read by rocprofv3
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB device copy
+# gives; profiles/tile_stages_4096_vnoise_4x4_counters.md)
+GRAD_TRAFFIC_BYTES = 144.4e6 + 192.0e6
+
 SIDE = 4096
 LX = LY = 2          # 4x4 tiles
 OCTAVES, LACUNARITY, GAIN = 8, 2.0, 0.5
@@ -406,7 +411,8 @@ def main():
                    "grid": [SIDE, SIDE], "tiles": [4, 4], "threshold": THRESHOLD,
                    "leaf_critical_points": K, "root_critical_points": root_cps},
         "roofline": {"bound": "hbm", "kernel": "k_grad_local", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": GRAD_TRAFFIC_BYTES,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
                      "algorithmic_bytes": g_bytes, "avg_launch_ms": round(g_ms, 4)},
         "gradient_trace": {"kernels": ["k_grad_local", "k_ring_resolve", "k_owner"],
                            "algorithmic_bytes": g_bytes + t_bytes, "ms": round(gt_ms, 4),
