@@ -322,7 +322,6 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   for (int p = tid; p < RING_W; p += 256) {
     const int q = p >= RING_Q ? 1 : 0;
     const int pp = p - q * RING_Q;
-    if (pp >= RING_N) continue;
     int li, lj;
     ring_at(pp, li, lj);
     uint32_t out = OWN_NONE;
@@ -335,14 +334,10 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
         out = (y << 15) | ((uint32_t)bx << 7) | (w & 127u);
       } else if (kind == 1) {
         const int s = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
-        int tbx = bx, tby = by, tli, tlj;
-        if (s == 0) { tbx = bx + 1; tli = 0; tlj = pos; }
-        else if (s == 1) { tbx = bx - 1; tli = TX - 1; tlj = pos; }
-        else if (s == 2) { tby = by + 1; tli = pos; tlj = 0; }
-        else { tby = by - 1; tli = pos; tlj = TY - 1; }
+        const int tbx = bx + (s == 0) - (s == 1), tby = by + (s == 2) - (s == 3);
         // a path never leaves the ext window, so the neighbour exists
         if (tbx >= 0 && tby >= 0 && tbx < nbx && tby * TY < g.vy)
-          out = RING_MARK | (uint32_t)((tby * nbx + tbx) * RING_W + q * RING_Q + ring_pos(tli, tlj));
+          out = RING_MARK | (uint32_t)((tby * nbx + tbx) * RING_W + q * RING_Q + ring_entry_base(s) + pos);
         else
           atomicOr(err, 8u);
       }
@@ -362,8 +357,6 @@ __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict
   const int nblk = T.nseg * ((T.g.vy + GRAD_TY - 1) / GRAD_TY);
   const int64_t e = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
   if (e >= (int64_t)nblk * RING_W) return;
-  const int p = (int)(e % RING_W);
-  if ((p % RING_Q) >= RING_N) return;
   uint32_t w = T.ring[e];
   for (int steps = 0; w != OWN_NONE && (w & RING_MARK); ++steps) {
     if (steps > nblk) {
@@ -415,34 +408,39 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     if (cy < g.ch) so[r] = glb((const uint32_t *)T.seg_offs)[(int64_t)cy * nbx + bx];
   }
   __syncthreads();
-  // ring word of entry (tli, tlj) of forest q in block (tbx, tby), or
-  // UINT32_MAX when that block does not exist (a valid gradient's paths stay
-  // inside ext, so this only guards bad input)
-  auto ring_index = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
+  // ring word at offset off of forest q in block (tbx, tby), or UINT32_MAX
+  // when that block does not exist (a valid gradient's paths stay inside
+  // ext, so this only guards bad input)
+  auto ring_index = [&](int tbx, int tby, int q, int off) -> uint32_t {
     if (tbx < 0 || tby < 0 || tbx >= nbx || tby * TY >= g.vy) return 0xFFFFFFFFu;
-    return (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)(q * RING_Q + ring_pos(tli, tlj));
+    return (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)(q * RING_Q + off);
   };
-  auto ring_val = [&](int tbx, int tby, int q, int tli, int tlj) -> uint32_t {
-    const uint32_t x = ring_index(tbx, tby, q, tli, tlj);
+  auto ring_val = [&](int tbx, int tby, int q, int off) -> uint32_t {
+    const uint32_t x = ring_index(tbx, tby, q, off);
     return x == 0xFFFFFFFFu ? OWN_NONE : ringidx[x];
   };
+  // the neighbours across sides 0 (+x), 1 (-x), 2 (+y), 3 (-y): whether each
+  // exists, and its block offset
+  const uint32_t side_ok = (uint32_t)(bx + 1 < nbx) | (uint32_t)(bx > 0) << 1 | (uint32_t)((by + 1) * TY < g.vy) << 2 |
+                           (uint32_t)(by > 0) << 3;
 
   // 1. terminals -> critical indices in LDS.  A lane issues all its
   //    terminal loads, then all its ring loads, before it uses any of them.
+  // an exit through side s at position p continues at the neighbour's ring
+  // word ring_entry_base(s) + p (branch-free decode)
+  constexpr uint32_t ENTRY_BASE = (uint32_t)(2 * TX) | (uint32_t)(2 * TX + TY) << 8 | (uint32_t)TX << 24;
+  static_assert(2 * TX + TY < 256, "entry bases fit a byte");
   uint32_t rx[2 * RW];
 #pragma unroll
   for (int k = 0; k < 2 * RW; ++k) {
     const int q = k & 1;
     const uint32_t w = (wl[k >> 1] >> (16 * q)) & 0xFFFFu;
-    uint32_t x = 0xFFFFFFFFu;
-    if (term_kind(w) == 1) {
-      const int sd = (int)((w >> 6) & 3u), pos = (int)(w & 63u);
-      if (sd == 0) x = ring_index(bx + 1, by, q, 0, pos);
-      else if (sd == 1) x = ring_index(bx - 1, by, q, TX - 1, pos);
-      else if (sd == 2) x = ring_index(bx, by + 1, q, pos, 0);
-      else x = ring_index(bx, by - 1, q, pos, TY - 1);
-    }
-    rx[k] = x;
+    const uint32_t s = (w >> 6) & 3u, pos = w & 63u;
+    const int step = (s & 2u) ? nbx : 1, dblk = (s & 1u) ? -step : step;
+    const uint32_t x = (uint32_t)(lb + dblk) * RING_W + (uint32_t)(q * RING_Q) +
+                       __builtin_amdgcn_ubfe(ENTRY_BASE, 8 * s, 8) + pos;
+    const bool ok = term_kind(w) == 1 && __builtin_amdgcn_ubfe(side_ok, s, 1);
+    rx[k] = ok ? x : 0xFFFFFFFFu;
   }
   uint32_t rv[2 * RW];
 #pragma unroll
@@ -455,12 +453,12 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
-    const uint32_t x = lane == TX - 1 ? ring_index(bx + 1, by, 0, 0, lj)
-                                      : (lane == 0 ? ring_index(bx - 1, by, 1, TX - 1, lj) : 0xFFFFFFFFu);
+    const uint32_t x = lane == TX - 1 ? ring_index(bx + 1, by, 0, ring_entry_base(0) + lj)
+                                      : (lane == 0 ? ring_index(bx - 1, by, 1, ring_entry_base(1) + lj) : 0xFFFFFFFFu);
     xs[r] = x != 0xFFFFFFFFu ? ringidx[x] : OWN_NONE;
   }
-  const uint32_t xt = wv == 0 ? ring_val(bx, by - 1, 1, lane, TY - 1) : OWN_NONE;
-  const uint32_t xb = wv == 3 ? ring_val(bx, by + 1, 0, lane, 0) : OWN_NONE;
+  const uint32_t xt = wv == 0 ? ring_val(bx, by - 1, 1, ring_entry_base(3) + lane) : OWN_NONE;
+  const uint32_t xb = wv == 3 ? ring_val(bx, by + 1, 0, ring_entry_base(2) + lane) : OWN_NONE;
 #pragma unroll
   for (int k = 0; k < 2 * RW; ++k) {
     const int q = k & 1, lj = wv * RW + (k >> 1);

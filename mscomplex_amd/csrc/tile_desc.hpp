@@ -14,19 +14,20 @@ constexpr int GRAD_TY = 32;       // vertex rows per block tile
 constexpr int SEG = 2 * GRAD_TX;  // cells per collation segment: one block tile row
 constexpr int SCAN_CHUNK = 2048;  // counts per scan block
 
-// Perimeter ("ring") of a block tile: the vertex positions (and, separately,
-// the quad positions) with li in {0, TX-1} or lj in {0, TY-1}.  A gradient
-// path that leaves a block enters its neighbour on that neighbour's ring.
-constexpr int RING_N = 2 * GRAD_TX + 2 * GRAD_TY - 4;  // positions per forest
-constexpr int RING_Q = 192;                            // quad ring offset in a block's ring words
-constexpr int RING_W = 2 * RING_Q;                     // ring words per block
-static_assert(RING_N <= RING_Q, "ring does not fit");
+// Perimeter ("ring") of a block tile, per forest (vertices, then quads at
+// RING_Q): four side arrays, the bottom row (lj = 0) at [0, TX), the top row
+// (lj = TY-1) at [TX, 2TX), the left column (li = 0) at [2TX, 2TX+TY) and
+// the right column (li = TX-1) at [2TX+TY, 2TX+2TY); a corner has a word in
+// two arrays, both with the same value.  A gradient path that leaves a block
+// through side s at position p enters its neighbour on the array of the
+// opposite side, at ring_entry_base(s) + p.
+constexpr int RING_Q = 2 * GRAD_TX + 2 * GRAD_TY;  // ring words per forest
+constexpr int RING_W = 2 * RING_Q;                 // ring words per block
 
-MSCX_HD int ring_pos(int li, int lj) {
-  if (lj == 0) return li;
-  if (lj == GRAD_TY - 1) return GRAD_TX + li;
-  if (li == 0) return 2 * GRAD_TX + lj - 1;
-  return 2 * GRAD_TX + GRAD_TY - 2 + lj - 1;  // li == TX - 1
+// side arrays entered through side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y): the
+// neighbour's left column, right column, bottom row, top row
+MSCX_HD int ring_entry_base(int s) {
+  return s == 0 ? 2 * GRAD_TX : s == 1 ? 2 * GRAD_TX + GRAD_TY : s == 2 ? 0 : GRAD_TX;
 }
 MSCX_HD void ring_at(int p, int &li, int &lj) {
   if (p < GRAD_TX) {
@@ -35,12 +36,12 @@ MSCX_HD void ring_at(int p, int &li, int &lj) {
   } else if (p < 2 * GRAD_TX) {
     li = p - GRAD_TX;
     lj = GRAD_TY - 1;
-  } else if (p < 2 * GRAD_TX + GRAD_TY - 2) {
+  } else if (p < 2 * GRAD_TX + GRAD_TY) {
     li = 0;
-    lj = p - 2 * GRAD_TX + 1;
+    lj = p - 2 * GRAD_TX;
   } else {
     li = GRAD_TX - 1;
-    lj = p - (2 * GRAD_TX + GRAD_TY - 2) + 1;
+    lj = p - (2 * GRAD_TX + GRAD_TY);
   }
 }
 
