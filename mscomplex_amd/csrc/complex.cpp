@@ -505,7 +505,14 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   float max_persistence = max_val - min_val;
   std::vector<uint32_t> new_edges;
   const Rect &R = m.rect;
-  auto on_b = [&](uint32_t c) { return R.on_boundary(cx_of(c), cy_of(c)); };
+  // per node: dimension (bits 0-1) and rect-boundary flag (bit 2), so that
+  // a popped pair is classified from two bytes instead of its cell ids
+  std::vector<uint8_t> info(m.size());
+  for (uint32_t i = 0; i < m.size(); ++i) {
+    const uint32_t c = m.cell[i];
+    info[i] = (uint8_t)(cell_dim(c) | (R.on_boundary(cx_of(c), cy_of(c)) ? 4u : 0u));
+  }
+  const uint8_t *flags = m.flags.data();
 
   while (!pq.empty()) {
     uint64_t hp = pq.top();
@@ -513,17 +520,23 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     pq.pop();
     // the next element's pair is a random access into a large array
     if (!pq.empty()) __builtin_prefetch(&slots[ExactHeap::payload(pq.top())]);
-    uint32_t v1 = cp.first, v2 = cp.second;
-    float pers = std::abs(m.fn[v1] - m.fn[v2]);
+    // the key is the persistence |fn[v2] - fn[v1]| computed at the push
+    const uint32_t kb = ExactHeap::key(hp);
+    float pers;
+    memcpy(&pers, &kb, 4);
     if ((double)pers / (double)max_persistence > thresh) break;
-    if (cell_dim(m.cell[v2]) == 1) std::swap(v1, v2);
-    uint32_t d2 = cell_dim(m.cell[v2]);
-    if ((m.flags[v1] & CPF_CANCELLED) || (m.flags[v2] & CPF_CANCELLED)) continue;
-    bool b1 = on_b(m.cell[v1]), b2 = on_b(m.cell[v2]);
-    if (b1 && !b2) continue;
-    if (!b1 && b2) continue;
+    uint32_t v1 = cp.first, v2 = cp.second;
+    if ((flags[v1] | flags[v2]) & CPF_CANCELLED) continue;
+    uint8_t i1 = info[v1], i2 = info[v2];
+    if ((i2 & 3u) == 1) {
+      std::swap(v1, v2);
+      std::swap(i1, i2);
+    }
+    const uint32_t d2 = i2 & 3u;
+    const bool b1 = i1 & 4u, b2 = i2 & 4u;
+    if (b1 != b2) continue;
     const ConnSet &c2 = (d2 / 2 == 0) ? m.asc[v2] : m.des[v2];
-    bool strang = m.flags[v1] & CPF_STRANGULATION;
+    bool strang = flags[v1] & CPF_STRANGULATION;
     if (strang && c2.size() != 1) {
       resubmit.push_back(hp);
       continue;
