@@ -54,13 +54,14 @@ int mscx_profile_read(double *ms, uint32_t *counts, int n);
 /* Stage submission through HIP graphs (no reference counterpart).  With
  * on != 0 (the default) the part of a tile batch that needs no host decision
  * (descriptor upload, gradient, scan, ring resolution, owner images, count
- * download) is captured into a HIP graph once per launch configuration and
- * replayed by later batches with the same configuration; with 0 every stage
+ * download) is built into a HIP graph (node by node: no stream is put in
+ * capture mode) once per launch configuration and replayed by later batches
+ * with the same configuration; with 0 every stage
  * is launched directly.  While mscx_profile is enabled stages are launched
  * directly.  *previous (may be NULL) receives the old setting;
- * mscx_graph_stats counts captures and replays since mscx_init. */
+ * mscx_graph_stats counts graph builds and replays since mscx_init. */
 int mscx_set_graphs(int on, int *previous);
-int mscx_graph_stats(uint64_t *captures, uint64_t *replays);
+int mscx_graph_stats(uint64_t *builds, uint64_t *replays);
 
 /* Device buffers (no reference counterpart: the reference keeps inputs in
  * host memory).  Lets a caller keep grids resident in HBM and bind tiles to

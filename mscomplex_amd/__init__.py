@@ -108,7 +108,7 @@ def set_graphs(on=True):
 
 
 def graph_stats():
-    """(captures, replays) of the stage graph since initialisation."""
+    """(builds, replays) of the stage graph since initialisation."""
     c, r = ctypes.c_uint64(0), ctypes.c_uint64(0)
     check(lib().mscx_graph_stats(ctypes.byref(c), ctypes.byref(r)))
     return int(c.value), int(r.value)

@@ -337,30 +337,46 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   out->cell.reserve(total);
   out->fn.reserve(total);
 
+  // at[i][j]: the index in `out` of the cell of m_i's node j (every
+  // non-cancelled node's cell is in `out` after this loop), so that the arcs
+  // below are translated without hashing their cells
+  std::vector<uint32_t> at[2];
   for (int i = 0; i < 2; ++i) {
     const Complex &m = *arr[i];
+    at[i].assign(m.size(), UINT32_MAX);
     for (uint32_t j = 0; j < m.size(); ++j) {
-      if (i == 1 && out->idx.contains(m.cell[j])) continue;
+      if (i == 1) {
+        const uint32_t e = out->idx.find(m.cell[j]);
+        if (e != UINT32_MAX) {
+          at[i][j] = e;
+          continue;
+        }
+      }
       if (m.flags[j] & CPF_CANCELLED) continue;
-      out->replicate(m, j);
+      at[i][j] = out->replicate(m, j);
       out->fn.push_back(m.fn[j]);
     }
   }
   for (int i = 0; i < 2; ++i) {
     const Complex &m = *arr[i];
+    const uint32_t *ai = at[i].data();
     for (uint32_t j = 0; j < m.size(); ++j) {
       if (m.flags[j] & CPF_CANCELLED) continue;
-      uint32_t di = out->idx.get_or_zero(m.cell[j]);
+      const uint32_t di = ai[j];
+      // map::operator[] on the pair's cell (which inserts a zero entry
+      // when the pair is absent, as the reference's lookup does)
       if (m.flags[j] & CPF_BCANC) out->pair[di] = out->idx.get_or_zero(m.cell[m.pair[j]]);
       bool cmn = (i == 1) && ixn.contains(cx_of(m.cell[j]), cy_of(m.cell[j]));
       for (int k = 0; k < 2; ++k) {
         const ConnSet &src = k == 0 ? m.asc[j] : m.des[j];
         ConnSet &dst = k == 0 ? out->asc[di] : out->des[di];
         for (uint32_t v : src) {
-          uint32_t vc = m.cell[v];
-          if (cmn && ixn.contains(cx_of(vc), cy_of(vc))) continue;
+          if (cmn) {
+            const uint32_t vc = m.cell[v];
+            if (ixn.contains(cx_of(vc), cy_of(vc))) continue;
+          }
           if (m.flags[v] & CPF_CANCELLED) continue;
-          dst.insert(out->idx.get_or_zero(vc));
+          dst.insert(ai[v]);
         }
       }
     }
@@ -398,26 +414,31 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
     }
   }
   Complex *arr[2] = {&m1, &m2};
+  std::vector<uint32_t> at;
   for (int i = 0; i < 2; ++i) {
     Complex &m = *arr[i];
+    // at[j]: m's index of P's cell j (UINT32_MAX when absent), kept equal to
+    // m.idx.find(P.cell[j]) as nodes are replicated into m
+    at.resize(P.size());
+    for (uint32_t j = 0; j < P.size(); ++j) at[j] = m.idx.find(P.cell[j]);
+    auto rep = [&](uint32_t x) {
+      const uint32_t r = m.replicate(P, x);
+      m.fn.push_back(P.fn[x]);
+      at[x] = r;
+      return r;
+    };
     for (uint32_t j = 0; j < P.size(); ++j) {
       if (P.flags[j] & CPF_CANCELLED) throw std::logic_error("all cps ought to be uncancelled by now");
       if (!(P.flags[j] & CPF_BCANC)) continue;
       uint32_t sp = P.pair[j];
-      bool src_in = m.idx.contains(P.cell[j]);
-      bool sp_in = m.idx.contains(P.cell[sp]);
+      bool src_in = at[j] != UINT32_MAX;
+      bool sp_in = at[sp] != UINT32_MAX;
       if (!src_in && !sp_in) continue;
-      if (!src_in) {
-        m.replicate(P, j);
-        m.fn.push_back(P.fn[j]);
-      }
-      if (!sp_in) {
-        m.replicate(P, sp);
-        m.fn.push_back(P.fn[sp]);
-      }
-      uint32_t d = m.idx.find(P.cell[j]);
+      if (!src_in) rep(j);
+      if (!sp_in) rep(sp);
+      uint32_t d = at[j];
       if (!src_in || !sp_in || !(m.flags[d] & CPF_BCANC)) {
-        uint32_t dp = m.idx.find(P.cell[sp]);
+        uint32_t dp = at[sp];
         m.flags[d] |= CPF_BCANC;
         m.flags[dp] |= CPF_BCANC;
         m.pair[d] = dp;
@@ -429,25 +450,22 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
         (k == 0 ? m.asc[d] : m.des[d]).clear();
         for (uint32_t v : src) {
           if (P.flags[v] & CPF_BCANC) throw std::logic_error("only non cancellable cps must be remaining");
-          uint32_t mv = m.idx.find(P.cell[v]);
-          if (mv == UINT32_MAX) {
-            mv = m.replicate(P, v);
-            m.fn.push_back(P.fn[v]);
-          }
+          uint32_t mv = at[v];
+          if (mv == UINT32_MAX) mv = rep(v);
           (k == 0 ? m.asc[d] : m.des[d]).insert(mv);
         }
       }
     }
     for (uint32_t j = 0; j < P.size(); ++j) {
       if (P.flags[j] & CPF_BCANC) continue;
-      uint32_t d = m.idx.find(P.cell[j]);
+      uint32_t d = at[j];
       if (d == UINT32_MAX) continue;
       for (int k = 0; k < 2; ++k) {
         const ConnSet &src = k == 0 ? P.asc[j] : P.des[j];
         ConnSet &dst = k == 0 ? m.asc[d] : m.des[d];
         dst.clear();
         for (uint32_t v : src) {
-          uint32_t mv = m.idx.find(P.cell[v]);
+          uint32_t mv = at[v];
           if (mv == UINT32_MAX) continue;
           dst.insert(mv);
         }

@@ -831,6 +831,57 @@ void launch_ring_resolve(const TileDesc *d_tiles, const int *d_prefix, int ntile
 void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_owner, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);
 }
+hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, size_t up_bytes, void *dl_dst,
+                             const void *dl_src, size_t dl_bytes, const TileDesc *d_tiles, const int *d_prefix,
+                             int ntiles, const int *nblocks, uint32_t *d_err) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipGraphCreate(&g, 0);
+  if (e != hipSuccess) return e;
+  hipGraphNode_t prev = nullptr;
+  auto chain = [&](hipGraphNode_t n) { prev = n; };
+  auto memcpy_node = [&](void *dst, const void *src, size_t bytes, hipMemcpyKind kind) -> hipError_t {
+    hipGraphNode_t n;
+    hipError_t r = hipGraphAddMemcpyNode1D(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, dst, src, bytes, kind);
+    if (r == hipSuccess) chain(n);
+    return r;
+  };
+  auto kernel_node = [&](const void *fn, int grid, void **args) -> hipError_t {
+    if (grid <= 0) return hipSuccess;
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(fn);
+    p.gridDim = dim3(grid);
+    p.blockDim = dim3(256);
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t n;
+    hipError_t r = hipGraphAddKernelNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    if (r == hipSuccess) chain(n);
+    return r;
+  };
+  const int *pg = d_prefix + KB_GRADIENT * (ntiles + 1), *ps = d_prefix + KB_SCAN * (ntiles + 1);
+  const int *pr = d_prefix + KB_RING * (ntiles + 1);
+  void *a_grad[] = {(void *)&d_tiles, (void *)&pg, (void *)&ntiles, (void *)&d_err};
+  void *a_scan[] = {(void *)&d_tiles, (void *)&ps, (void *)&ntiles};
+  void *a_parts[] = {(void *)&d_tiles};
+  void *a_ring[] = {(void *)&d_tiles, (void *)&pr, (void *)&ntiles, (void *)&d_err};
+  void *a_own[] = {(void *)&d_tiles, (void *)&pg, (void *)&ntiles};
+  const bool scan = nblocks[KB_SCAN] > 0;
+  if ((e = memcpy_node(up_dst, up_src, up_bytes, hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = kernel_node((const void *)k_grad_local, nblocks[KB_GRADIENT], a_grad)) == hipSuccess &&
+      (e = kernel_node((const void *)k_scan_reduce, nblocks[KB_SCAN], a_scan)) == hipSuccess &&
+      (e = kernel_node((const void *)k_scan_parts, scan ? ntiles : 0, a_parts)) == hipSuccess &&
+      (e = kernel_node((const void *)k_scan_down, nblocks[KB_SCAN], a_scan)) == hipSuccess &&
+      (e = kernel_node((const void *)k_ring_resolve, nblocks[KB_RING], a_ring)) == hipSuccess &&
+      (e = kernel_node((const void *)k_owner, nblocks[KB_GRADIENT], a_own)) == hipSuccess &&
+      (e = memcpy_node(dl_dst, dl_src, dl_bytes, hipMemcpyDeviceToHost)) == hipSuccess) {
+    *out = g;
+    return hipSuccess;
+  }
+  (void)hipGraphDestroy(g);
+  return e;
+}
+
 void launch_scan(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s) {
   if (!nblocks) return;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nblocks), dim3(256), 0, s, d_tiles, d_prefix, ntiles);

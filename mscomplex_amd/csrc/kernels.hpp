@@ -28,5 +28,12 @@ void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, 
 // Manifold labels for ndisc leaves; prefix has ndisc+1 entries of 256-item blocks.
 int plan_disc_blocks(const DiscDesc *d, int n, int *prefix);
 void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s);
+// The stage chain of run_tiles up to the critical counts (upload of the
+// descriptor block, gradient, scan, ring resolution, owner images, download
+// of the counts) as a HIP graph built node by node, so that no stream is
+// ever in capture mode.  d_prefix holds KB_COUNT prefix tables of ntiles+1.
+hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, size_t up_bytes, void *dl_dst,
+                             const void *dl_src, size_t dl_bytes, const TileDesc *d_tiles, const int *d_prefix,
+                             int ntiles, const int *nblocks, uint32_t *d_err);
 
 }  // namespace mscx

@@ -201,11 +201,11 @@ int mscx_set_graphs(int on, int *previous) {
   });
 }
 
-int mscx_graph_stats(uint64_t *captures, uint64_t *replays) {
+int mscx_graph_stats(uint64_t *builds, uint64_t *replays) {
   return mscx::guarded([&] {
     mscx::Runtime &rt = mscx::runtime();
     std::lock_guard<std::mutex> lk(rt.mu);
-    if (captures) *captures = rt.graph_captures;
+    if (builds) *builds = rt.graph_builds;
     if (replays) *replays = rt.graph_replays;
   });
 }

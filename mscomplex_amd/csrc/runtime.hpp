@@ -65,8 +65,8 @@ struct Runtime {
   // stage graphs (run_tiles), guarded by mu
   bool graphs = true;
   hipGraphExec_t graph_exec = nullptr;
-  std::vector<uintptr_t> graph_key;  // launch configuration graph_exec was captured with
-  uint64_t graph_captures = 0, graph_replays = 0;
+  std::vector<uintptr_t> graph_key;  // launch configuration graph_exec was built for
+  uint64_t graph_builds = 0, graph_replays = 0;
 };
 Runtime &runtime();
 void ensure_init();

@@ -138,10 +138,12 @@ void run_tiles(mscx_tile *const *tiles, int n) {
 
   // descriptor upload, gradient + block-local trace, scan of the critical
   // counts, ring resolution, owner images and the count download need no
-  // host decision in between: one HIP graph, captured once per launch
+  // host decision in between: one HIP graph, built once per launch
   // configuration (staging addresses, sizes, grid sizes) and replayed; the
   // tile descriptors themselves are data that the upload node reads from the
-  // pinned staging buffer at every replay
+  // pinned staging buffer at every replay.  The graph is built node by node
+  // rather than captured from the stream, so other threads' use of the
+  // device (including the legacy stream) is never affected.
   StageTimer &tm = stage_timer();
   auto chain = [&] {
     MSCX_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
@@ -168,22 +170,13 @@ void run_tiles(mscx_tile *const *tiles, int n) {
         MSCX_HIP(hipGraphExecDestroy(rt.graph_exec));
         rt.graph_exec = nullptr;
       }
-      MSCX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      try {
-        chain();
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(s, &g);  // leave capture mode before reporting
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
       hipGraph_t g = nullptr;
-      MSCX_HIP(hipStreamEndCapture(s, &g));
+      MSCX_HIP(build_front_graph(&g, dev, host, total, hk, dk, o_err + 4 - o_k, dd, dp, n, nb, derr));
       const hipError_t e = hipGraphInstantiate(&rt.graph_exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       MSCX_HIP(e);
       rt.graph_key = key;
-      ++rt.graph_captures;
+      ++rt.graph_builds;
     } else {
       ++rt.graph_replays;
     }

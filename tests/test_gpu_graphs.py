@@ -1,10 +1,10 @@
 """The tile stages that need no host decision (descriptor upload, gradient,
 scan, ring resolution, owner images, count download) submitted as one HIP
-graph, captured once per launch configuration and replayed.  Every replay,
+graph, built once per launch configuration and replayed.  Every replay,
 on new data written in place into the bound device buffer, must equal the
 directly launched run of the same data word for word; direct runs and host
 synchronisations sit between the replays, and a second configuration is
-captured and replayed in turn."""
+built and replayed in turn."""
 import numpy as np
 import pytest
 
@@ -56,7 +56,7 @@ def test_replays_equal_direct_runs(shape):
         c0, r0 = mx.graph_stats()
         for k, f in enumerate(fields):
             gbuf.upload(f)
-            mx.work_tiles(gt)                 # captured on the first pass, replayed after
+            mx.work_tiles(gt)                 # built on the first pass, replayed after
             got = _snapshot(gt)
             mx.set_graphs(False)
             ebuf.upload(f)
