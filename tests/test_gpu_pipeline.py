@@ -1,6 +1,8 @@
 """GPU parity of the whole grid pipeline (mscx_grid_run: HIP stages for
 every leaf tile, merge up, simplification, merge down, manifold discs)
 against the oracle's GridDataManager order, word for word for every piece."""
+import os
+
 import numpy as np
 import pytest
 
@@ -60,3 +62,13 @@ def test_dense_cancellation_fails_cleanly():
     with pytest.raises(mx.MscxError) as e:
         mx.GridRun(rng_field("white", 150, 130, 5), levels=0, threshold=0.1)
     assert e.value.code == -4
+
+
+@pytest.mark.timeout(600)
+def test_bench_workload_4096_4x4():
+    """The benchmark's workload itself: 4096^2 value noise (8 octaves,
+    lacunarity 2, gain 0.5, period 1024, the bench seed), 4x4 tiles, 1%
+    simplification; every one of the 31 pieces word for word after the
+    discs (the oracle runs its --num-parallel form on up to 16 threads)."""
+    g = mx.gen_value_noise(4096, 4096, 8, 2.0, 0.5, 1024.0, 20240601)
+    _compare(g, "vnoise4096-bench", lx=2, ly=2, threshold=0.01, threads=min(16, os.cpu_count() or 1))
