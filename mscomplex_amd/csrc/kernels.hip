@@ -240,7 +240,9 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   const int nbx = T.nseg;
   const int bx = lb % nbx, by = lb / nbx;
   const int i0 = bx * TX, j0 = by * TY;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the wave index is uniform: keep it (and the row indices derived from it)
+  // in scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
   // A block is interior when every cell of it and of its one-vertex halo
   // lies strictly inside the interior rect: no boundary rule applies and
@@ -255,10 +257,12 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   __syncthreads();
 
   // 2. codes, critical flags per row segment, trace links
+  // (the row walk schedules better with its first row in a vector register)
+  const int lj0 = (tid >> 6) * RW;
   if (interior)
-    grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, wv * RW);
+    grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, lj0);
   else
-    grad_rows<false>(T, g, sk, sl, i0, j0, bx, lane, wv * RW);
+    grad_rows<false>(T, g, sk, sl, i0, j0, bx, lane, lj0);
   // rows past the ext edge hold no entries
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
@@ -390,7 +394,9 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   const int nbx = T.nseg;
   const int bx = lb % nbx, by = lb / nbx;
   const int i0 = bx * TX, j0 = by * TY;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the wave index is uniform: keep it (and the row indices derived from it)
+  // in scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
   const int64_t pitch = T.pitch, own_pitch = T.own_pitch, loc_pitch = T.loc_pitch;
   const GLB uint8_t *const codes = glb((const uint8_t *)T.codes);
