@@ -145,7 +145,8 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   const uint32_t ex_col = (li == TX - 1 ? 1u : 0u) | (li == 0 ? 2u : 0u);
   // interior blocks lie inside the ext: every row exists, and the walk is
   // unrolled so that the window slides by register renaming
-#pragma unroll(INTERIOR ? RW : 1)
+  constexpr int UNROLL = INTERIOR ? RW : 1;
+#pragma unroll UNROLL
   for (int rr = 0; rr < RW; ++rr) {
     const int lj = lj0 + rr, j = K.j;
     if (!INTERIOR && j >= g.vy) break;  // uniform across the wave
