@@ -336,6 +336,12 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   out->idx.reserve(total);
   out->cell.reserve(total);
   out->fn.reserve(total);
+  out->pair.reserve(total);
+  out->flags.reserve(total);
+  out->asc.reserve(total);
+  out->des.reserve(total);
+  out->asc_disc.reserve(total);
+  out->des_disc.reserve(total);
 
   // at[i][j]: the index in `out` of the cell of m_i's node j (every
   // non-cancelled node's cell is in `out` after this loop), so that the arcs
