@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_grad_local launch of the main workload, read by rocprofv3
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB device copy
 # gives; profiles/tile_stages_4096_vnoise_4x4_counters.md)
-GRAD_TRAFFIC_BYTES = 144.4e6 + 192.0e6
+GRAD_TRAFFIC_BYTES = 144.2e6 + 192.0e6
 
 SIDE = 4096
 LX = LY = 2          # 4x4 tiles
