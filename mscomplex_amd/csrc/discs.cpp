@@ -183,19 +183,41 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   MSCX_HIP(hipMemcpyAsync(dev, stage, o_lab, hipMemcpyHostToDevice, s));
   launch_disc_labels((const DiscDesc *)dev, (const int *)(dev + o_pref), n, nblocks, s);
   MSCX_HIP(hipGetLastError());
+  // the labels come back leaf by leaf, each followed by an event, so that a
+  // leaf's grouping starts as soon as its own labels have arrived
   char *labels = host + codes_bytes;
-  MSCX_HIP(hipMemcpyAsync(labels, dev + o_lab, lo, hipMemcpyDeviceToHost, s));
+  struct Events {
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    }
+  } ev;
+  ev.e.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) MSCX_HIP(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming));
+  for (int i = 0; i < n; ++i) {
+    MSCX_HIP(hipMemcpyAsync(labels + lab_off[i], dev + o_lab + lab_off[i], (nv[i] + nq[i]) * 4,
+                            hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipEventRecord(ev.e[i], s));
+  }
   uint32_t *herr = (uint32_t *)(stage + o_err);
   MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
+
+  // 4. saddle tracing (needs no labels) while the labels arrive, then the
+  //    extremum discs (grouping), per leaf in parallel; a label the kernel
+  //    flagged as bad is the none label, which grouping skips
+  try {
+    par_for(n, [&](int i) {
+      disc_paths(*cs[i], dp[i], rect[i], ext[i], cv[i]);
+      MSCX_HIP(hipEventSynchronize(ev.e[i]));
+      const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);
+      disc_group(*cs[i], rect[i], lv, lv + nv[i]);
+    });
+  } catch (...) {
+    (void)hipStreamSynchronize(s);  // no copy may still be landing in the staging buffers
+    throw;
+  }
   MSCX_HIP(hipStreamSynchronize(s));
   if (*herr) throw std::logic_error("owner extremum not in surviving map");
-
-  // 4. extremum discs (grouping) and saddle tracing, per leaf in parallel
-  par_for(n, [&](int i) {
-    const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);
-    disc_group(*cs[i], rect[i], lv, lv + nv[i]);
-    disc_paths(*cs[i], dp[i], rect[i], ext[i], cv[i]);
-  });
 }
 
 }  // namespace mscx
