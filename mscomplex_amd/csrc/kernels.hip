@@ -50,7 +50,8 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 // a lane owns one vertex column and, per vertex (i, j), the four cells
 // (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1).
 //  1. keys of the tile's vertices and a 1/2-vertex halo go to LDS (all
-//     global loads of a lane are issued before the first LDS store)
+//     global loads of a lane are issued before the first LDS store; interior
+//     blocks load without bounds tests)
 //  2. each wave walks GRAD_TY/4 rows with a 4x4 window of keys in registers
 //     (rows j-1..j+2, columns i-1..i+2; one new LDS row per step) and
 //     evaluates the pairing candidates it needs there (vertex_cand /
@@ -59,22 +60,21 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 //     segment (two ballots) and the trace links: a vertex points across its
 //     paired edge to the next vertex, a quad across its paired edge to the
 //     next quad (grid_dataset.cpp:1376-1427).  Blocks whose cells all lie
-//     strictly inside the interior rect take a path without boundary tests.
+//     strictly inside the interior rect take a path without boundary tests,
+//     with the walk unrolled.
 //  3. pointer jumping in LDS takes every entry to its terminal in the tile:
 //     its critical root, "none", or the exit through the tile's side
 //  4. terminals go to `loc` (16 bits per vertex and per quad); the
 //     terminals of the tile's perimeter entries go to its ring words, in the
 //     form the ring resolution chases.
 
-// trace link of a vertex (q = 0) or quad (q = 1) entry from its code; a
-// critical entry becomes its own root terminal, recorded as its cell row in
-// the block (2 lj + q) and its rank among the critical cells of that row's
-// segment
 // Trace link of a vertex (q = 0) or quad (q = 1) entry (li, lj) from its
-// code: its root terminal when critical, none when unpaired, else the entry
-// across its pair (DIR_PX, NX, PY, NY step by +1, -1, +TX, -TX), as the byte
-// offset of that entry in the block's 16-bit entry array, or, when the step
-// leaves the tile through side s = dir - 1, the exit terminal.
+// code: when critical, its own root terminal (its cell row in the block,
+// 2 lj + q, and its rank among the critical cells of that row's segment);
+// none when unpaired; else the entry across its pair (DIR_PX, NX, PY, NY
+// step by +1, -1, +TX, -TX), as the byte offset of that entry in the block's
+// 16-bit entry array, or, when the step leaves the tile through side
+// s = dir - 1, the exit terminal.
 // exits: bit s set when this entry's step in direction s + 1 leaves the
 // tile.  Every value is computed before it is selected (no divergent
 // branches).

@@ -46,7 +46,8 @@ MSCX_HD void ring_at(int p, int &li, int &lj) {
 }
 
 // Local terminal of a vertex or quad entry after the block-local trace
-// (16 bits; in LDS an entry with bit 15 clear is a link to another entry):
+// (16 bits; in LDS an entry with bit 15 clear is a link: the byte offset of
+// another entry of the block):
 //   0x8000 | r<<7 | k   root: the critical cell in cell row r (0..63) of the
 //                       block, k-th critical cell of that row's segment
 //   0xA000 | s<<6 | p   exit: the path leaves through side s (0 +x, 1 -x,
