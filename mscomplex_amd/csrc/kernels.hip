@@ -386,6 +386,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY, RW = TY / 4;
   __shared__ uint32_t so[2 * TY];
   __shared__ uint32_t dv[NV], dq[NV];
+  __shared__ uint32_t exit_base[4];
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
@@ -414,6 +415,15 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     const int cy = 2 * j0 + r;
     if (cy < g.ch) so[r] = glb((const uint32_t *)T.seg_offs)[(int64_t)cy * nbx + bx];
   }
+  // an exit through side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y) at position p
+  // continues at ring word exit_base[s] + q * RING_Q + p of the neighbour
+  // (UINT32_MAX: no neighbour there; a valid gradient's paths stay inside
+  // ext, so this only guards bad input)
+  if (tid < 4) {
+    const int tbx = bx + (tid == 0) - (tid == 1), tby = by + (tid == 2) - (tid == 3);
+    const bool ok = tbx >= 0 && tby >= 0 && tbx < nbx && tby * TY < g.vy;
+    exit_base[tid] = ok ? (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)ring_entry_base(tid) : 0xFFFFFFFFu;
+  }
   __syncthreads();
   // ring word at offset off of forest q in block (tbx, tby), or UINT32_MAX
   // when that block does not exist (a valid gradient's paths stay inside
@@ -426,28 +436,16 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     const uint32_t x = ring_index(tbx, tby, q, off);
     return x == 0xFFFFFFFFu ? OWN_NONE : ringidx[x];
   };
-  // the neighbours across sides 0 (+x), 1 (-x), 2 (+y), 3 (-y): whether each
-  // exists, and its block offset
-  const uint32_t side_ok = (uint32_t)(bx + 1 < nbx) | (uint32_t)(bx > 0) << 1 | (uint32_t)((by + 1) * TY < g.vy) << 2 |
-                           (uint32_t)(by > 0) << 3;
-
   // 1. terminals -> critical indices in LDS.  A lane issues all its
   //    terminal loads, then all its ring loads, before it uses any of them.
-  // an exit through side s at position p continues at the neighbour's ring
-  // word ring_entry_base(s) + p (branch-free decode)
-  constexpr uint32_t ENTRY_BASE = (uint32_t)(2 * TX) | (uint32_t)(2 * TX + TY) << 8 | (uint32_t)TX << 24;
-  static_assert(2 * TX + TY < 256, "entry bases fit a byte");
   uint32_t rx[2 * RW];
 #pragma unroll
   for (int k = 0; k < 2 * RW; ++k) {
     const int q = k & 1;
     const uint32_t w = (wl[k >> 1] >> (16 * q)) & 0xFFFFu;
-    const uint32_t s = (w >> 6) & 3u, pos = w & 63u;
-    const int step = (s & 2u) ? nbx : 1, dblk = (s & 1u) ? -step : step;
-    const uint32_t x = (uint32_t)(lb + dblk) * RING_W + (uint32_t)(q * RING_Q) +
-                       __builtin_amdgcn_ubfe(ENTRY_BASE, 8 * s, 8) + pos;
-    const bool ok = term_kind(w) == 1 && __builtin_amdgcn_ubfe(side_ok, s, 1);
-    rx[k] = ok ? x : 0xFFFFFFFFu;
+    const uint32_t base = exit_base[(w >> 6) & 3u];
+    const bool ok = term_kind(w) == 1 && base != 0xFFFFFFFFu;
+    rx[k] = ok ? base + (uint32_t)(q * RING_Q) + (w & 63u) : 0xFFFFFFFFu;
   }
   uint32_t rv[2 * RW];
 #pragma unroll
