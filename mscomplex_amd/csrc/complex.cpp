@@ -11,6 +11,7 @@
 #include <queue>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "runtime.hpp"
 #include "tile.hpp"
@@ -419,10 +420,10 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
       uncancel_pairs(P, si, pi);
     }
   }
-  Complex *arr[2] = {&m1, &m2};
-  std::vector<uint32_t> at;
-  for (int i = 0; i < 2; ++i) {
-    Complex &m = *arr[i];
+  // The two children are updated from P (which is only read from here on)
+  // independently of each other: large ones on two threads.
+  auto down = [&](Complex &m) {
+    std::vector<uint32_t> at;
     // at[j]: m's index of P's cell j (UINT32_MAX when absent), kept equal to
     // m.idx.find(P.cell[j]) as nodes are replicated into m
     at.resize(P.size());
@@ -477,6 +478,27 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
         }
       }
     }
+  };
+  if (P.size() >= 4096) {
+    std::exception_ptr err;
+    std::thread other([&] {
+      try {
+        down(m2);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    try {
+      down(m1);
+    } catch (...) {
+      other.join();
+      throw;
+    }
+    other.join();
+    if (err) std::rethrow_exception(err);
+  } else {
+    down(m1);
+    down(m2);
   }
 }
 
