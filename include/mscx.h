@@ -28,6 +28,7 @@ extern "C" {
 #define MSCX_ELOGIC -3    /* invariant violated (reference: std::logic_error) */
 #define MSCX_ENOMEM -4    /* allocation failure or HBM budget exceeded */
 #define MSCX_ESTATE -5    /* call out of order (e.g. download before work) */
+#define MSCX_EIO -6       /* input file cannot be opened or is too short (reference: runtime_error "unable to read stream") */
 
 typedef struct mscx_tile mscx_tile;       /* a GridDataset: one rect + ext window on the GPU */
 typedef struct mscx_complex mscx_complex; /* a GridMSComplex (host resident) */
@@ -174,6 +175,27 @@ int mscx_complex_destroy(mscx_complex *c);
 #define MSCX_RUN_NO_DISCS 2
 int mscx_grid_run(const float *data, int32_t size_x, int32_t size_y, int32_t levels, int32_t lx, int32_t ly,
                   double threshold, uint32_t flags, mscx_grid **out);
+/* The whole reference driver over a raw grid file: GridDataManager(filename,
+ * size_x, size_y, num_levels, single_threaded, use_ocl, simp_tresh,
+ * out_of_core, num_parallel) (grid_datamanager.cpp:596-662) with its reader
+ * readDataAndInit / computeSubdomainMsgraphs (:493-594) and, when
+ * disc_prefix is not NULL, collectManifold's disc files
+ * (<disc_prefix>asc_(x,y) / des_(x,y), appended leaf by leaf in piece order;
+ * the reference driver's prefix is "dp__disc_", :397-420).  The file holds
+ * at least size_x * size_y little-endian floats, x fastest (the reference's
+ * format, main.cpp -gf); trailing bytes are ignored as the reference's
+ * reader ignores them.  It is streamed once into HBM in row slabs through
+ * two pinned host buffers, each slab's copy overlapping the next read
+ * (MSCX_READ_SLAB_BYTES in the environment sets the slab size, default
+ * 8 MiB); the pieces bind to the resident grid, so the reference's
+ * three-row rewind between buffers has no counterpart.  num_parallel caps
+ * the host threads of the host phases (0 = default, 1 = the reference's
+ * -st).  levels / lx / ly / threshold / flags as for mscx_grid_run
+ * (MSCX_RUN_DEVICE_INPUT is invalid here).  A missing, unreadable or short
+ * file fails with MSCX_EIO before the device is touched. */
+int mscx_grid_run_file(const char *path, int32_t size_x, int32_t size_y, int32_t levels, int32_t lx, int32_t ly,
+                       double threshold, int32_t num_parallel, uint32_t flags, const char *disc_prefix,
+                       mscx_grid **out);
 int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n);
 /* Borrowed pointer to piece i's complex (leaves first, root last). */
 int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out);

@@ -6,6 +6,7 @@ GridMSComplex and the GridDataManager compute order, backed by libmscx.so
 CPU fallback: every compute call goes through the HIP library.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -440,6 +441,23 @@ class GridRun:
             p = ptr(data)
         check(lib().mscx_grid_run(p, sx, sy, levels, lx, ly, float(threshold), flags, ctypes.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_file(cls, path, size, levels=1, lx=0, ly=0, threshold=0.0, num_parallel=0, discs=True,
+                  disc_prefix=None):
+        """The reference driver over a raw float32 grid file (x fastest) of
+        size (sx, sy) (mscx_grid_run_file): GridDataManager(filename, sx, sy,
+        levels, ..., simp_tresh, ..., num_parallel).  disc_prefix, when
+        given, receives every leaf's disc files (the reference driver uses
+        "dp__disc_" in its working directory)."""
+        sx, sy = size
+        h = ctypes.c_void_p()
+        pre = None if disc_prefix is None else os.fsencode(disc_prefix)
+        check(lib().mscx_grid_run_file(os.fsencode(path), sx, sy, levels, lx, ly, float(threshold), int(num_parallel),
+                                       0 if discs else 2, pre, ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        return self
 
     def __del__(self):
         h = getattr(self, "_h", None)

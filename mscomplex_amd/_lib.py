@@ -54,6 +54,8 @@ _SIGS = {
     "mscx_complex_destroy": (c_int, [P]),
     "mscx_grid_run": (c_int, [P, c_int32, c_int32, c_int32, c_int32, c_int32, c_double, c_uint32,
                               ctypes.POINTER(P)]),
+    "mscx_grid_run_file": (c_int, [ctypes.c_char_p, c_int32, c_int32, c_int32, c_int32, c_int32, c_double, c_int32,
+                                   c_uint32, ctypes.c_char_p, ctypes.POINTER(P)]),
     "mscx_grid_num_pieces": (c_int, [P, ctypes.POINTER(c_int32)]),
     "mscx_grid_piece": (c_int, [P, c_int32, ctypes.POINTER(P)]),
     "mscx_grid_times": (c_int, [P, P]),

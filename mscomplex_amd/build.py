@@ -24,7 +24,11 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
               "-Wno-unused-command-line-argument"]
 
 DEVICE_SOURCES = ["kernels.hip"]
-HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline.cpp", "server.cpp", "gen.cpp"]
+HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline.cpp", "reader.cpp", "server.cpp",
+                "gen.cpp"]
+# the reference's dmsc command line over the C ABI (links libmscx.so)
+CLI_SOURCE = "dmsc_main.cpp"
+CLI = os.path.join(HERE, "mscx_dmsc")
 
 
 def _hipcc():
@@ -63,13 +67,21 @@ def build(verbose=False):
     with _cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"]
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread",
+           "-Wl,-soname,libmscx.so"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    cmd = [_host_cxx(), "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, CLI_SOURCE), "-o", tmp, LIB,
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 

@@ -479,7 +479,7 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
       }
     }
   };
-  if (P.size() >= 4096) {
+  if (P.size() >= 4096 && host_threads() > 1) {
     std::exception_ptr err;
     std::thread other([&] {
       try {

@@ -19,15 +19,17 @@ static Scratch g_stage;  // host staging of descriptors and tables
 
 template <class F>
 static void par_for(int n, F f) {
-  int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+  const int nt = host_threads();
   if (n <= 1 || nt <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
   std::vector<std::thread> th;
   std::vector<std::exception_ptr> errs(std::min(n, nt));
+  const int cap = thread_cap();
   for (int t = 0; t < (int)errs.size(); ++t)
     th.emplace_back([&, t] {
+      ThreadCap keep(cap);
       try {
         for (int i = t; i < n; i += (int)errs.size()) f(i);
       } catch (...) {

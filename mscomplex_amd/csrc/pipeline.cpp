@@ -21,8 +21,6 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-int host_threads() { return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u); }
-
 // Dynamic parallel loop over [0, n) on up to `nt` threads; the first
 // exception is rethrown after all threads have joined.
 template <class F>
@@ -36,8 +34,10 @@ static void parallel_for(int n, F f, int nt = 0) {
   std::atomic<int> next(0);
   std::vector<std::thread> th;
   std::vector<std::exception_ptr> errs(nt);
+  const int cap = thread_cap();
   for (int t = 0; t < nt; ++t)
     th.emplace_back([&, t] {
+      ThreadCap keep(cap);
       try {
         for (int i = next++; i < n; i = next++) f(i);
       } catch (...) {
@@ -386,6 +386,42 @@ extern "C" const char *mscx_batch_error(const mscx_batch *b, int32_t i) {
 
 extern "C" int mscx_batch_destroy(mscx_batch *b) {
   return guarded([&] { delete b; });
+}
+
+// GridDataManager's constructor over a file (grid_datamanager.cpp:596-662):
+// read, compute, merge up, simplify (threshold > 0), merge down, collect the
+// manifolds and write each leaf's discs (collectManifold, :397-420).  The
+// file is checked before the device is touched.
+extern "C" int mscx_grid_run_file(const char *path, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
+                                  double threshold, int32_t num_parallel, uint32_t flags, const char *disc_prefix,
+                                  mscx_grid **out) {
+  return guarded([&] {
+    MSCX_REQUIRE(path && out, MSCX_EINVAL, "null argument");
+    MSCX_REQUIRE(!(flags & MSCX_RUN_DEVICE_INPUT), MSCX_EINVAL, "a file run reads host input");
+    MSCX_REQUIRE(!(disc_prefix && (flags & MSCX_RUN_NO_DISCS)), MSCX_EINVAL, "disc files need the discs");
+    MSCX_REQUIRE(num_parallel >= 0 && num_parallel <= 1024, MSCX_EINVAL, "num_parallel must be in [0, 1024]");
+    MSCX_REQUIRE(threshold >= 0.0 && !std::isnan(threshold), MSCX_EINVAL, "threshold must be >= 0");
+    check_grid_args(sx, sy, levels, lx, ly);
+    GridFile f;
+    open_grid_file(path, sx, sy, f);
+    ensure_init();
+    ThreadCap cap(num_parallel);
+    std::unique_ptr<mscx_grid> G(new mscx_grid);
+    const double t0 = now_s();
+    G->dev_input = device_pool().alloc(f.bytes);
+    stream_grid_file(f, sx, (float *)G->dev_input);
+    prepare_grid(G.get(), (const float *)G->dev_input, sx, sy, levels, lx, ly, threshold,
+                 flags | MSCX_RUN_DEVICE_INPUT);
+    G->t[0] += now_s() - t0;
+    std::vector<mscx_grid *> gs{G.get()};
+    run_grids(gs, nullptr, nullptr);
+    if (disc_prefix) {
+      const double t5 = now_s();
+      for (size_t i = 0; i < G->leaves.size(); ++i) write_discs(G->pieces[i]->c, disc_prefix);
+      G->t[4] += now_s() - t5;
+    }
+    *out = G.release();
+  });
 }
 
 extern "C" int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n) {

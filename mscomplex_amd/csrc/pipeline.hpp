@@ -40,7 +40,6 @@ struct mscx_grid {
 namespace mscx {
 
 double now_s();
-int host_threads();
 
 // Validates arguments, uploads host input (unless MSCX_RUN_DEVICE_INPUT),
 // creates and binds the leaf tiles.
@@ -57,6 +56,21 @@ void grids_discs(std::vector<mscx_grid *> &grids);
 // HBM a grid's tiles and input take (for admission against a budget);
 // the critical-point block is estimated at one critical cell in four.
 size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, bool with_input);
+
+// A raw grid file opened for the file-driven run (reader.cpp).
+struct GridFile {
+  int fd = -1;
+  size_t bytes = 0;  // size_x * size_y * 4, the part that is read
+  GridFile() = default;
+  GridFile(const GridFile &) = delete;
+  GridFile &operator=(const GridFile &) = delete;
+  ~GridFile();
+};
+// Opens path and checks that it holds size_x * size_y floats (MSCX_EIO).
+void open_grid_file(const char *path, int32_t sx, int32_t sy, GridFile &f);
+// Streams the grid into dev (size_x * size_y floats) through two pinned
+// slabs, each copy overlapping the next read; rejects NaN (MSCX_EINVAL).
+void stream_grid_file(GridFile &f, int32_t sx, float *dev);
 
 // Deep copy of a grid's pieces (the complexes); tiles are not copied.
 void copy_pieces(const mscx_grid *src, mscx_grid *dst);

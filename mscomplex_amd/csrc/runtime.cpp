@@ -2,11 +2,24 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <thread>
+
 namespace mscx {
 
 static thread_local std::string g_last_error;
 
 void set_last_error(const std::string &m) { g_last_error = m; }
+
+static thread_local int t_thread_cap = 0;
+
+int thread_cap() { return t_thread_cap; }
+void set_thread_cap(int n) { t_thread_cap = n; }
+
+int host_threads() {
+  if (t_thread_cap > 0) return t_thread_cap;
+  return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+}
 
 Runtime &runtime() {
   static Runtime rt;

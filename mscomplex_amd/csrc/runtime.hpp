@@ -71,6 +71,23 @@ struct Runtime {
 Runtime &runtime();
 void ensure_init();
 
+// Host threads the parallel host phases use: min(hardware threads, 16), or
+// the calling thread's cap when a ThreadCap is live on it (the file run's
+// num_parallel; 1 runs every host phase on the caller).  parallel loops pass
+// their caller's cap on to their workers.
+int host_threads();
+int thread_cap();
+void set_thread_cap(int n);
+struct ThreadCap {
+  int prev;
+  explicit ThreadCap(int n) : prev(thread_cap()) {
+    if (n > 0) set_thread_cap(n);
+  }
+  ~ThreadCap() { set_thread_cap(prev); }
+  ThreadCap(const ThreadCap &) = delete;
+  ThreadCap &operator=(const ThreadCap &) = delete;
+};
+
 // Growable device scratch and pinned host staging used by batched launches.
 struct Scratch {
   void *dev = nullptr;
