@@ -158,6 +158,18 @@ int mscx_complex_write_discs(const mscx_complex *c, const char *prefix);
  *   des_disc...
  * *len receives the word count; the words are written when cap suffices. */
 int mscx_complex_dump(const mscx_complex *c, uint32_t *buf, size_t cap, size_t *len);
+/* Host archive of a complex: the counterpart of write_msgraph_to_archive /
+ * read_msgraph_from_archive (grid_datamanager.cpp:91-119), which store a
+ * GridMSComplex's rect, ext, values, id map and critical points
+ * (grid_mscomplex.cpp:604-644) as Boost binary archives.  The format is
+ * this library's own and portable across little-endian hosts: magic
+ * "MSCXCPX1", version 1, a reserved word, the word count (uint64), the dump
+ * words above (discs included) and an FNV-1a 64 checksum of them.  A loaded
+ * complex dumps word for word as the saved one and takes part in merges,
+ * simplification and disc writing like any other.  Load fails with
+ * MSCX_EIO on a missing, truncated, corrupt or foreign file. */
+int mscx_complex_save(const mscx_complex *c, const char *path);
+int mscx_complex_load(const char *path, mscx_complex **out);
 int mscx_complex_num_cps(const mscx_complex *c, uint32_t *n);
 int mscx_complex_destroy(mscx_complex *c);
 

@@ -280,6 +280,17 @@ class GridMSComplex:
     def write_discs(self, prefix):
         check(lib().mscx_complex_write_discs(self._h, prefix.encode()))
 
+    def save(self, path):
+        """Host archive of the complex (write_msgraph_to_archive counterpart)."""
+        check(lib().mscx_complex_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """A complex from an archive written by save() (read_msgraph_from_archive)."""
+        h = ctypes.c_void_p()
+        check(lib().mscx_complex_load(os.fsencode(path), ctypes.byref(h)))
+        return cls(h)
+
     def num_cps(self):
         n = ctypes.c_uint32()
         check(lib().mscx_complex_num_cps(self._h, ctypes.byref(n)))

@@ -49,6 +49,8 @@ _SIGS = {
     "mscx_complex_simplify_un_simplify": (c_int, [P, c_double]),
     "mscx_tile_fill_discs": (c_int, [P, P]),
     "mscx_complex_write_discs": (c_int, [P, ctypes.c_char_p]),
+    "mscx_complex_save": (c_int, [P, ctypes.c_char_p]),
+    "mscx_complex_load": (c_int, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "mscx_complex_dump": (c_int, [P, P, c_size_t, ctypes.POINTER(c_size_t)]),
     "mscx_complex_num_cps": (c_int, [P, ctypes.POINTER(c_uint32)]),
     "mscx_complex_destroy": (c_int, [P]),

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <limits>
 #include <memory>
@@ -813,6 +814,118 @@ void write_discs(const Complex &m, const std::string &prefix) {
   }
 }
 
+// A host archive of one complex, the counterpart of the reference's
+// Boost binary archives of a GridMSComplex (write_msgraph_to_archive /
+// read_msgraph_from_archive, grid_datamanager.cpp:91-119; the fields of
+// grid_mscomplex.cpp:604-644).  Boost archives are not portable, so the
+// format is this library's own: an 8-byte magic, a version word, a reserved
+// word, the word count, the complex's dump words (rect, ext, and per
+// critical point its cell, value bits, pair, flags, arcs and discs) and an
+// FNV-1a checksum of those words, all little-endian.  Discs are kept too,
+// so a loaded complex dumps word for word as the saved one.
+static const char kArchiveMagic[8] = {'M', 'S', 'C', 'X', 'C', 'P', 'X', '1'};
+static const uint32_t kArchiveVersion = 1;
+
+static uint64_t fnv1a(const uint32_t *w, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; ++i)
+    for (int b = 0; b < 4; ++b) {
+      h ^= (w[i] >> (8 * b)) & 0xFFu;
+      h *= 1099511628211ull;
+    }
+  return h;
+}
+
+void save_complex(const Complex &m, const std::string &path) {
+  std::vector<uint32_t> w;
+  m.dump(w);
+  const uint32_t hdr[2] = {kArchiveVersion, 0};
+  const uint64_t n = w.size(), sum = fnv1a(w.data(), w.size());
+  std::ofstream os(path, std::ios::out | std::ios::trunc | std::ios::binary);
+  if (!os.is_open()) throw Error(MSCX_EIO, "cannot write complex archive: " + path);
+  os.write(kArchiveMagic, 8);
+  os.write((const char *)hdr, sizeof hdr);
+  os.write((const char *)&n, 8);
+  os.write((const char *)w.data(), (std::streamsize)(w.size() * 4));
+  os.write((const char *)&sum, 8);
+  os.flush();
+  if (!os.good()) throw Error(MSCX_EIO, "cannot write complex archive: " + path);
+}
+
+void load_complex(const std::string &path, Complex &m) {
+  std::ifstream is(path, std::ios::in | std::ios::binary);
+  if (!is.is_open()) throw Error(MSCX_EIO, "cannot read complex archive: " + path);
+  char magic[8];
+  uint32_t hdr[2];
+  uint64_t n = 0;
+  is.read(magic, 8);
+  is.read((char *)hdr, sizeof hdr);
+  is.read((char *)&n, 8);
+  if (!is.good() || memcmp(magic, kArchiveMagic, 8) != 0) throw Error(MSCX_EIO, "not a complex archive: " + path);
+  if (hdr[0] != kArchiveVersion) throw Error(MSCX_EIO, "unsupported complex archive version: " + path);
+  is.seekg(0, std::ios::end);
+  const uint64_t bytes = (uint64_t)is.tellg();
+  if (n < 9 || bytes != 24 + 4 * n + 8) throw Error(MSCX_EIO, "truncated complex archive: " + path);
+  is.seekg(24);
+  std::vector<uint32_t> w((size_t)n);
+  uint64_t sum = 0;
+  is.read((char *)w.data(), (std::streamsize)(4 * n));
+  is.read((char *)&sum, 8);
+  if (!is.good() || sum != fnv1a(w.data(), w.size())) throw Error(MSCX_EIO, "corrupt complex archive: " + path);
+
+  // parse the dump words (the checksum passed, so a malformed body means a
+  // writer bug; still, every length is checked against the words left)
+  auto bad = [&]() { return Error(MSCX_EIO, "corrupt complex archive: " + path); };
+  size_t at = 0;
+  auto take = [&](size_t k) {
+    if (w.size() - at < k) throw bad();
+    const uint32_t *p = w.data() + at;
+    at += k;
+    return p;
+  };
+  const uint32_t *h = take(9);
+  const uint32_t ncp = h[0];
+  Complex c;
+  c.rect = Rect{(int)h[1], (int)h[2], (int)h[3], (int)h[4]};
+  c.ext = Rect{(int)h[5], (int)h[6], (int)h[7], (int)h[8]};
+  if ((uint64_t)ncp * 8 > w.size()) throw bad();
+  c.cell.resize(ncp);
+  c.fn.resize(ncp);
+  c.pair.resize(ncp);
+  c.flags.resize(ncp);
+  c.asc.resize(ncp);
+  c.des.resize(ncp);
+  c.asc_disc.resize(ncp);
+  c.des_disc.resize(ncp);
+  c.idx.reserve(ncp);
+  for (uint32_t i = 0; i < ncp; ++i) {
+    const uint32_t *r = take(8);
+    c.cell[i] = r[0];
+    memcpy(&c.fn[i], &r[1], 4);
+    c.pair[i] = r[2];
+    if (r[3] > 0xFFu || (r[2] != UINT32_MAX && r[2] >= ncp)) throw bad();
+    c.flags[i] = (uint8_t)r[3];
+    const uint32_t na = r[4], nd = r[5], nad = r[6], ndd = r[7];
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t cnt = k == 0 ? na : nd;
+      const uint32_t *v = take(cnt);
+      ConnSet &dst = k == 0 ? c.asc[i] : c.des[i];
+      for (uint32_t e = 0; e < cnt; ++e) {
+        if (v[e] >= ncp || (e && v[e] < v[e - 1])) throw bad();
+        dst.insert(v[e]);
+      }
+    }
+    const uint32_t *ad = take(nad);
+    c.asc_disc[i].assign(ad, ad + nad);
+    const uint32_t *dd = take(ndd);
+    c.des_disc[i].assign(dd, dd + ndd);
+    if (c.idx.contains(c.cell[i])) throw bad();
+    c.idx.set(c.cell[i], i);
+  }
+  if (at != w.size()) throw bad();
+  m = std::move(c);
+}
+
 }  // namespace mscx
 
 using namespace mscx;
@@ -897,6 +1010,22 @@ int mscx_complex_write_discs(const mscx_complex *c, const char *prefix) {
   return guarded([&] {
     MSCX_REQUIRE(c && prefix, MSCX_EINVAL, "null argument");
     write_discs(c->c, prefix);
+  });
+}
+
+int mscx_complex_save(const mscx_complex *c, const char *path) {
+  return guarded([&] {
+    MSCX_REQUIRE(c && path, MSCX_EINVAL, "null argument");
+    save_complex(c->c, path);
+  });
+}
+
+int mscx_complex_load(const char *path, mscx_complex **out) {
+  return guarded([&] {
+    MSCX_REQUIRE(path && out, MSCX_EINVAL, "null argument");
+    std::unique_ptr<mscx_complex> h(new mscx_complex);
+    load_complex(path, h->c);
+    *out = h.release();
   });
 }
 

@@ -238,6 +238,10 @@ void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ex
 void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n);
 
 void write_discs(const Complex &m, const std::string &prefix);
+// Host archive of a complex (dump words + checksum); load throws MSCX_EIO
+// on a missing, truncated or corrupt file.
+void save_complex(const Complex &m, const std::string &path);
+void load_complex(const std::string &path, Complex &m);
 
 }  // namespace mscx
 

@@ -143,3 +143,14 @@ def test_dmsc_command_line(tmp_path):
         got = _read_dir(d)
         assert sorted(got) == sorted(exp)
         assert all(got[n] == exp[n] for n in exp), sub
+
+
+def test_archives_of_device_run(tmp_path):
+    """Every piece of a device-computed run survives an archive round trip
+    (mscx_complex_save / mscx_complex_load) word for word."""
+    data = _field("vnoise", 256, 200)
+    run = mx.GridRun(data, lx=1, ly=1, threshold=0.01)
+    for i in range(run.num_pieces()):
+        p = tmp_path / f"dp_msgraph_{i}"
+        run.piece(i).save(p)
+        assert np.array_equal(mx.GridMSComplex.load(p).dump(), run.piece(i).dump()), f"piece {i}"
