@@ -485,11 +485,15 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
 
   // 2. owners of the four cells of each vertex: vertices and quads from
   //    the terminals, critical edges by rank, paired edges from the vertex or
-  //    quad they are paired with (every value is computed, then selected)
+  //    quad they are paired with (every value is computed, then selected).
+  //    Unrolled, so that the per-row values held in register arrays (codes,
+  //    side owners) are read without selects; rows past the ext edge are
+  //    skipped by a branch that is uniform across the wave.
+#pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r, li = lane;
     const int i = i0 + li, j = j0 + lj;
-    if (j >= g.vy) break;  // uniform across the wave
+    if (j >= g.vy) continue;
     const bool ein = j + 1 < g.vy;
     const uint16_t w0 = cw0[r], w1 = cw1[r];
     const uint8_t cv = (uint8_t)w0, chh = (uint8_t)(w0 >> 8), ce = (uint8_t)w1, cq = (uint8_t)(w1 >> 8);
