@@ -7,6 +7,7 @@
 // also driven by the serving loop (server.cpp).
 #include "pipeline.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -21,8 +22,11 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Dynamic parallel loop over [0, n) on up to `nt` threads; the first
-// exception is rethrown after all threads have joined.
+// Dynamic parallel loop over [0, n) on up to `nt` threads.  When items
+// fail, the exception of the lowest failing index is rethrown after all
+// threads have joined: the failure a sequential loop would have met first,
+// whatever the threads' timing (items after a failure still run; callers
+// discard their results).
 template <class F>
 static void parallel_for(int n, F f, int nt = 0) {
   if (nt <= 0) nt = host_threads();
@@ -34,19 +38,27 @@ static void parallel_for(int n, F f, int nt = 0) {
   std::atomic<int> next(0);
   std::vector<std::thread> th;
   std::vector<std::exception_ptr> errs(nt);
+  std::vector<int> err_at(nt, n);
   const int cap = thread_cap();
   for (int t = 0; t < nt; ++t)
     th.emplace_back([&, t] {
       ThreadCap keep(cap);
-      try {
-        for (int i = next++; i < n; i = next++) f(i);
-      } catch (...) {
-        errs[t] = std::current_exception();
+      for (int i = next++; i < n; i = next++) {
+        try {
+          f(i);
+        } catch (...) {
+          if (i < err_at[t]) {
+            err_at[t] = i;
+            errs[t] = std::current_exception();
+          }
+        }
       }
     });
   for (auto &x : th) x.join();
-  for (auto &e : errs)
-    if (e) std::rethrow_exception(e);
+  int first = -1;
+  for (int t = 0; t < nt; ++t)
+    if (errs[t] && (first < 0 || err_at[t] < err_at[first])) first = t;
+  if (first >= 0) std::rethrow_exception(errs[first]);
 }
 
 // createPieces_quadtree (grid_datamanager.cpp:53-89): split along y into
@@ -79,15 +91,26 @@ void create_pieces(Rect r, Rect e, int level, int lx, int ly, bool two_d, std::v
   create_pieces(r2, e2, level - 1, nlx, nly, two_d, out);
 }
 
-static void check_grid_args(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly) {
-  MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
-  MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
-}
-
+// Leaf rects of a tiling.  The halo rule grows a child's ext one vertex past
+// its split line even when the split falls on the grid's edge (a slab
+// thinner than four cells splits off an empty piece), so a tiling finer
+// than the grid can give ext rects that leave it; the reference would read
+// past its input buffer for those (readDataAndInit copies the ext window),
+// so they are rejected here, before any data is read.
 static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, std::vector<LeafGeom> &out) {
   const bool two_d = lx > 0 || ly > 0;
   const Rect whole{0, 0, 2 * (sx - 1), 2 * (sy - 1)};
   create_pieces(whole, whole, two_d ? lx + ly : levels, lx, ly, two_d, out);
+  for (const LeafGeom &lg : out)
+    MSCX_REQUIRE(lg.ext.x0 >= 0 && lg.ext.y0 >= 0 && lg.ext.x1 <= whole.x1 && lg.ext.y1 <= whole.y1, MSCX_EINVAL,
+                 "tiling is finer than the grid: a piece's ext rect leaves the grid");
+}
+
+static void check_grid_args(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly) {
+  MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
+  MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
+  std::vector<LeafGeom> leaves;
+  leaves_of(sx, sy, levels, lx, ly, leaves);
 }
 
 void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
@@ -200,7 +223,11 @@ void grid_merge_up(mscx_grid *G, bool inner_parallel) {
 
 // simplify_un_simplify at the root, then mergePiecesDown +
 // finalMergeDownPiecesInRange (:308-395): parents before children, one tree
-// level at a time
+// level at a time.  Each level lists its parents in the reference's
+// single-threaded order (mergePiecesDown walks an internal level from the
+// highest piece index down, finalMergeDownPiecesInRange the leaves'
+// parents upwards), so that of several failing merges the one reported is
+// the one the reference meets first.
 void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
   const int nleaf = (int)G->leaves.size();
   const int nt = inner_parallel ? 0 : 1;
@@ -229,6 +256,9 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
         if (c >= nleaf) next.push_back(c);
         if (c + 1 >= nleaf) next.push_back(c + 1);
       }
+      // an internal level whose children are internal runs downwards
+      if (!next.empty() && 2 * (next[0] - nleaf) >= nleaf) std::sort(next.rbegin(), next.rend());
+      else std::sort(next.begin(), next.end());
       level.swap(next);
     }
   }
@@ -317,7 +347,8 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
 extern "C" int mscx_grid_run(const float *data, int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly,
                              double threshold, uint32_t flags, mscx_grid **out) {
   return guarded([&] {
-    MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
+    MSCX_REQUIRE(out && data, MSCX_EINVAL, "null argument");
+    check_grid_args(sx, sy, levels, lx, ly);  // before the device is touched
     ensure_init();
     std::unique_ptr<mscx_grid> G(new mscx_grid);
     const double t0 = now_s();

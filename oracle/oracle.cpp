@@ -840,6 +840,23 @@ DataManager::DataManager(const fn_t *data, int sx, int sy, int levels, int lx, i
   rect_t r(cellid_t(0, 0), cellid_t(2 * (sx - 1), 2 * (sy - 1)));
   rect_t e = r;
   createPieces(r, e, m_num_levels, lx, ly);
+  // A tiling finer than the grid gives pieces whose ext rect (the split
+  // line plus one vertex of halo) leaves the grid; readDataAndInit would
+  // copy past the end of the buffer for them (undefined behaviour in the
+  // reference), so the restatement refuses such tilings up front.
+  for (Piece *p : m_pieces) {
+    const rect_t &pe = p->dataset->m_ext_rect;
+    if (pe.bl[0] < 0 || pe.bl[1] < 0 || pe.tr[0] > r.tr[0] || pe.tr[1] > r.tr[1]) {
+      // the destructor does not run for a throwing constructor
+      for (Piece *q : m_pieces) {
+        delete q->dataset;
+        delete q->msgraph;
+        delete q;
+      }
+      m_pieces.clear();
+      throw std::invalid_argument("tiling is finer than the grid: a piece's ext rect leaves the grid");
+    }
+  }
 }
 
 DataManager::~DataManager() {

@@ -64,10 +64,12 @@ def product_pipeline(data, levels=0, lx=0, ly=0, threshold=0.0):
         pieces.append(mx.GridMSComplex.merge_up(pieces[j], pieces[j + 1]))
     if threshold > 0:
         pieces[-1].simplify_un_simplify(threshold)
-    if n > 1:
-        for p in range(2 * n - 2, n - 1, -1):   # parents before children
-            c = 2 * (p - n)
-            pieces[p].merge_down(pieces[c], pieces[c + 1])
+    # the reference's single-threaded merge-down order: the internal levels
+    # from the highest piece down (mergePiecesDown), then the leaves'
+    # parents upwards (finalMergeDownPiecesInRange)
+    for p in list(range(2 * n - 2, n + n // 2 - 1, -1)) + list(range(n, n + n // 2)):
+        c = 2 * (p - n)
+        pieces[p].merge_down(pieces[c], pieces[c + 1])
     for i, (rect, ext) in enumerate(leaves):
         codes, own = inputs[i]
         pieces[i].fill_discs_host(rect, ext, codes, own)
@@ -129,3 +131,44 @@ def test_dense_cancellation_fails_cleanly():
     with pytest.raises(mx.MscxError) as e:
         c.simplify_un_simplify(0.1)
     assert e.value.code == -4
+
+
+# Tilings finer than the grid: some pieces degenerate and the reference's
+# own invariant checks (logic_error) fire.  The product must fail where the
+# reference's single-threaded order fails first, with the same message, and
+# succeed (word for word) where it succeeds.
+DEGENERATE = [(5, 5, dict(levels=3)), (2, 2, dict(levels=1)), (2, 9, dict(levels=2)), (9, 2, dict(levels=1)),
+              (3, 3, dict(levels=4)), (2, 2, dict(levels=0)), (3, 3, dict(lx=1, ly=1)), (2, 5, dict(lx=1, ly=1)),
+              (6, 6, dict(lx=2, ly=2)), (4, 17, dict(levels=3)), (3, 40, dict(levels=4)), (40, 3, dict(lx=3, ly=0))]
+
+
+def oracle_outcome(data, **kw):
+    try:
+        return oracle.Pipeline(data, keep=True, **kw), None
+    except RuntimeError as e:
+        return None, str(e)
+
+
+TOO_FINE = "tiling is finer than the grid"
+
+
+@pytest.mark.parametrize("case", DEGENERATE, ids=[f"{a}x{b}-{kw}" for a, b, kw in DEGENERATE])
+def test_degenerate_tilings(case):
+    sx, sy, kw = case
+    data = rng_field("white", sx, sy, 1)
+    orc, msg = oracle_outcome(data, threshold=0.0, **kw)
+    if msg is not None and msg.startswith(TOO_FINE):
+        # a piece's ext rect would leave the grid: refused before any data
+        # is read (mscx_grid_run checks the tiling before the device)
+        with pytest.raises(mx.MscxError) as e:
+            mx.GridRun(data, threshold=0.0, **kw)
+        assert e.value.code == -1 and TOO_FINE in str(e.value)
+    elif msg is not None:
+        with pytest.raises(mx.MscxError) as e:
+            product_pipeline(data, threshold=0.0, **kw)
+        assert e.value.code == -3 and str(e.value).endswith(msg), (str(e.value), msg)
+    else:
+        mine = product_pipeline(data, threshold=0.0, **kw)
+        assert len(mine) == orc.num_pieces()
+        for i, c in enumerate(mine):
+            assert np.array_equal(orc.dump(4, i), c.dump()), f"piece {i}"

@@ -94,3 +94,15 @@ def test_cli_missing_file(tmp_path):
     r = _cli("-gf", str(tmp_path / "none.raw"), "-d", "8", "8", cwd=tmp_path)
     assert r.returncode == 1 and "unable to read stream" in r.stderr
     assert os.listdir(tmp_path) == []
+
+
+def test_tiling_finer_than_the_grid_is_einval(tmp_path):
+    # 2 x 2 vertices split into 2 slabs (the reference's default -l 1): the
+    # upper piece's ext rect would reach past the grid
+    p = tmp_path / "g.raw"
+    np.zeros((2, 2), np.float32).tofile(p)
+    with pytest.raises(mx.MscxError) as e:
+        mx.GridRun.from_file(str(p), (2, 2))
+    assert e.value.code == -1 and "finer than the grid" in str(e.value)
+    r = _cli("-gf", str(p), "-d", "2", "2", cwd=tmp_path)
+    assert r.returncode == 1 and "finer than the grid" in r.stderr

@@ -83,3 +83,27 @@ def test_worst_case_fields_1024_8x8(levels):
     0.5% of the range; every piece word for word after the discs."""
     g = mx.gen_white(1024, 1024, levels=levels, seed=77)
     _compare(g, f"white{levels}-1024-8x8", lx=3, ly=3, threshold=0.005, threads=min(16, os.cpu_count() or 1))
+
+
+from tests.test_complex_cpu import DEGENERATE, TOO_FINE, oracle_outcome  # noqa: E402
+
+
+@pytest.mark.parametrize("case", DEGENERATE, ids=[f"{a}x{b}-{kw}" for a, b, kw in DEGENERATE])
+def test_tilings_near_the_grid_size(case):
+    """Tilings as fine as the grid allows, and finer: where a piece's ext
+    rect would leave the grid the run is refused (MSCX_EINVAL) before the
+    device reads anything; otherwise every piece matches the oracle."""
+    sx, sy, kw = case
+    data = rng_field("white", sx, sy, 1)
+    print(f"[pipeline] tiling {sx}x{sy} {kw}", flush=True)
+    orc, msg = oracle_outcome(data, threshold=0.0, **kw)
+    if msg is not None:
+        assert msg.startswith(TOO_FINE), msg
+        with pytest.raises(mx.MscxError) as e:
+            mx.GridRun(data, threshold=0.0, **kw)
+        assert e.value.code == -1 and TOO_FINE in str(e.value)
+        return
+    run = mx.GridRun(data, threshold=0.0, **kw)
+    assert run.num_pieces() == orc.num_pieces()
+    for i in range(run.num_pieces()):
+        assert np.array_equal(orc.dump(4, i), run.piece(i).dump()), f"piece {i}"
