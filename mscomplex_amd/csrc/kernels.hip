@@ -68,26 +68,28 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 //     terminals of the tile's perimeter entries go to its ring words, in the
 //     form the ring resolution chases.
 
+// The block's 16-bit trace entries in LDS: per forest (q = 0 vertices, 1
+// quads) a (TX + 2) x (TY + 2) array whose interior holds entry (li, lj) at
+// (li + 1, lj + 1) and whose one-entry border holds the exit terminals: the
+// border slot a step from an edge entry lands on is T_EXIT with that side
+// and position, so a link never needs to test whether it leaves the tile.
+constexpr int ENT_W = GRAD_TX + 2, ENT_H = GRAD_TY + 2, ENT_N = ENT_W * ENT_H;
+__device__ __forceinline__ int ent_at(int q, int li, int lj) { return q * ENT_N + (lj + 1) * ENT_W + li + 1; }
+
 // Trace link of a vertex (q = 0) or quad (q = 1) entry (li, lj) from its
 // code: when critical, its own root terminal (its cell row in the block,
 // 2 lj + q, and its rank among the critical cells of that row's segment);
-// none when unpaired; else the entry across its pair (DIR_PX, NX, PY, NY
-// step by +1, -1, +TX, -TX), as the byte offset of that entry in the block's
-// 16-bit entry array, or, when the step leaves the tile through side
-// s = dir - 1, the exit terminal.
-// exits: bit s set when this entry's step in direction s + 1 leaves the
-// tile.  Every value is computed before it is selected (no divergent
-// branches).
-__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank, uint32_t exits) {
-  constexpr int TX = GRAD_TX, NV = TX * GRAD_TY;
-  static_assert(TX == 64, "step table holds +-TX in a signed byte");
-  constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)TX << 16) | ((uint32_t)(256 - TX) << 24);
+// none when unpaired; else the slot across its pair (DIR_PX, NX, PY, NY
+// step by +1, -1, +ENT_W, -ENT_W), as a byte offset into the entry array (a
+// border slot when the step leaves the tile).  Every value is computed
+// before it is selected (no divergent branches).
+__device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank) {
+  static_assert(ENT_W < 128, "step table holds +-ENT_W in a signed byte");
+  constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)ENT_W << 16) | ((uint32_t)(256 - ENT_W) << 24);
   const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
   const uint32_t s = (uint32_t)code_dir(c) - 1u;  // side 0..3 when paired
   const int step = __builtin_amdgcn_sbfe((int)STEP, (int)(8 * s), 8);
-  const uint32_t in_link = 2u * (uint32_t)(q * NV + lj * TX + li + step);  // LDS byte offset
-  const uint32_t exit_link = T_EXIT | (s << 6) | (s < 2 ? (uint32_t)lj : (uint32_t)li);
-  const uint32_t link = __builtin_amdgcn_ubfe(exits, s, 1) ? exit_link : in_link;
+  const uint32_t link = 2u * (uint32_t)(ent_at(q, li, lj) + step);  // LDS byte offset
   const uint32_t paired = (c & CF_PAIRED) ? link : T_NONE;
   return (c & CF_CRITICAL) ? root : paired;
 }
@@ -101,7 +103,7 @@ __device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, ui
 template <bool INTERIOR>
 __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, const uint32_t *__restrict__ sk,
                                           uint16_t *__restrict__ sl, int i0, int j0, int bx, int lane, int lj0) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, NV = TX * TY, RW = TY / 4;
+  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, RW = TY / 4;
   const int li = lane, i = i0 + li;
   // column i0 + TX: vertex candidate at row j0 + lj0 + L and vertical-edge
   // candidate at the same row, lane L in 0..RW (keys from LDS rows
@@ -141,8 +143,6 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   int64_t rs0 = (int64_t)(2 * K.j) * nseg + bx;
   GLB uint64_t *const critbits = glb(T.critbits);
   GLB uint32_t *const seg_counts = glb(T.seg_counts);
-  // sides a step from this lane's column leaves the tile through (link_of)
-  const uint32_t ex_col = (li == TX - 1 ? 1u : 0u) | (li == 0 ? 2u : 0u);
   // interior blocks lie inside the ext: every row exists, and the walk is
   // unrolled so that the window slides by register renaming
   constexpr int UNROLL = INTERIOR ? RW : 1;
@@ -179,9 +179,8 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
     crow += 2 * pitch;
     rs0 += 2 * nseg;
     // codes outside the ext are 0, so those entries get T_NONE
-    const uint32_t ex = ex_col | (lj == TY - 1 ? 4u : 0u) | (lj == 0 ? 8u : 0u);
-    sl[lj * TX + li] = (uint16_t)link_of(cv, li, lj, 0, rank_v, ex);
-    sl[NV + lj * TX + li] = (uint16_t)link_of(cq, li, lj, 1, rank_q, ex);
+    sl[ent_at(0, li, lj)] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
+    sl[ent_at(1, li, lj)] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
     vc00 = vc01;
     vc10 = vc11;
     hc00 = hc01;
@@ -231,7 +230,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   constexpr int NV = TX * TY;              // entries per forest
   constexpr int RW = TY / 4;               // vertex rows per wave
   __shared__ uint32_t sk[KH * KW];
-  __shared__ uint16_t sl[2 * NV];
+  __shared__ uint16_t sl[2 * ENT_N];
 
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
@@ -268,8 +267,20 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
     if (j0 + lj < g.vy) continue;
-    sl[lj * TX + lane] = (uint16_t)T_NONE;
-    sl[NV + lj * TX + lane] = (uint16_t)T_NONE;
+    sl[ent_at(0, lane, lj)] = (uint16_t)T_NONE;
+    sl[ent_at(1, lane, lj)] = (uint16_t)T_NONE;
+  }
+  // the border: exit terminals of side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y) at
+  // the position along that side (the entry's row for x sides, column for y)
+  for (int p = tid; p < 2 * RING_Q; p += 256) {
+    const int q = p >= RING_Q ? 1 : 0, pp = p - q * RING_Q;
+    int s, pos, bi, bj;  // side, position, border slot (column, row) in the padded array
+    if (pp < 2 * TX) {
+      s = pp < TX ? 2 : 3, pos = pp % TX, bi = pos + 1, bj = s == 2 ? TY + 1 : 0;
+    } else {
+      s = pp < 2 * TX + TY ? 0 : 1, pos = (pp - 2 * TX) % TY, bi = s == 0 ? TX + 1 : 0, bj = pos + 1;
+    }
+    sl[q * ENT_N + bj * ENT_W + bi] = (uint16_t)(T_EXIT | ((uint32_t)s << 6) | (uint32_t)pos);
   }
   __syncthreads();
 
@@ -285,24 +296,33 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   //    guards bad input).
   {
     constexpr int PER = 2 * NV / 256;
-    // links are byte offsets into sl, below the terminal bit
-    static_assert((4 * NV & (4 * NV - 1)) == 0 && 4 * NV <= 0x8000, "entry offsets below the terminal bit");
+    // links are byte offsets into sl, below the terminal bit; a terminal
+    // masked by ENT_MASK is still an offset inside sl (T_ROOT's row and rank
+    // bits stay below 0x2000, T_EXIT's side and position below 0x2100,
+    // T_NONE's are 0), so the read for a terminal entry is harmless
+    constexpr uint32_t ENT_MASK = 0x3FFEu;
+    static_assert(4 * ENT_N <= 0x4000 && 0x2000 + 0x100 < 2 * 2 * ENT_N, "entry offsets and masked terminals");
     const char *slb = (const char *)sl;
+    // this thread's entries: (li, lj) = (tid & 63, (tid >> 6) + 4 m) of
+    // forest q, for k = 8 q + m; their slots are a fixed stride apart
+    const int e0 = ent_at(0, tid & 63, tid >> 6);
+    auto slot = [&](int k) { return e0 + (k >> 3) * ENT_N + (k & 7) * 4 * ENT_W; };
+    static_assert(PER == 16 && TY == 32 && TX == 64, "entry walk layout");
     uint32_t cur[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) cur[k] = sl[tid + 256 * k];
+    for (int k = 0; k < PER; ++k) cur[k] = sl[slot(k)];
     uint32_t least = 0xFFFFu;  // all terminal: every entry >= 0x8000
 #pragma unroll
     for (int k = 0; k < PER; ++k) least = min(least, cur[k]);
     for (int round = 0; least < 0x8000u && round < 2 * NV; ++round) {
       uint32_t nxt[PER];
 #pragma unroll
-      for (int k = 0; k < PER; ++k) nxt[k] = *(const uint16_t *)(slb + (cur[k] & (4 * NV - 2)));
+      for (int k = 0; k < PER; ++k) nxt[k] = *(const uint16_t *)(slb + (cur[k] & ENT_MASK));
       least = 0xFFFFu;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         cur[k] = cur[k] < 0x8000u ? nxt[k] : cur[k];
-        sl[tid + 256 * k] = (uint16_t)cur[k];
+        sl[slot(k)] = (uint16_t)cur[k];
         least = min(least, cur[k]);
       }
     }
@@ -311,7 +331,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
       atomicOr(err, 2u);
 #pragma unroll
       for (int k = 0; k < PER; ++k)
-        if (!(cur[k] & 0x8000u)) sl[tid + 256 * k] = (uint16_t)T_NONE;
+        if (!(cur[k] & 0x8000u)) sl[slot(k)] = (uint16_t)T_NONE;
     }
   }
   __syncthreads();
@@ -321,7 +341,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     const int lj = wv * RW + r;
     const int j = j0 + lj;
     if (j >= g.vy) break;
-    T.loc[(int64_t)j * T.loc_pitch + i0 + lane] = (uint32_t)sl[lj * TX + lane] | ((uint32_t)sl[NV + lj * TX + lane] << 16);
+    T.loc[(int64_t)j * T.loc_pitch + i0 + lane] = (uint32_t)sl[ent_at(0, lane, lj)] | ((uint32_t)sl[ent_at(1, lane, lj)] << 16);
   }
   uint32_t *ring = T.ring + (int64_t)lb * RING_W;
   for (int p = tid; p < RING_W; p += 256) {
@@ -331,7 +351,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     ring_at(pp, li, lj);
     uint32_t out = OWN_NONE;
     if (i0 + li + q < g.vx && j0 + lj + q < g.vy) {
-      const uint32_t w = sl[q * NV + lj * TX + li];
+      const uint32_t w = sl[ent_at(q, li, lj)];
       const uint32_t kind = term_kind(w);
       if (kind == 0) {
         // root: ext-local cell row, segment (= block column), rank in the segment
