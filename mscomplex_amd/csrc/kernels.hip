@@ -139,7 +139,10 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   // this lane's two code bytes in cell row 2j, and this wave's critical
   // words / counts of cell row 2j; both advance by two cell rows per step
   const int64_t pitch = T.pitch, nseg = T.nseg;
-  GLB uint8_t *crow = glb(T.codes) + (int64_t)(2 * K.j) * pitch + 2 * i;
+  // (a tile's code array is below 4 GiB: a 32-bit offset from the uniform
+  // base lets the stores address through a scalar base register)
+  GLB uint8_t *const codes = glb(T.codes);
+  uint32_t crow = (uint32_t)((int64_t)(2 * K.j) * pitch + 2 * i);
   int64_t rs0 = (int64_t)(2 * K.j) * nseg + bx;
   GLB uint64_t *const critbits = glb(T.critbits);
   GLB uint32_t *const seg_counts = glb(T.seg_counts);
@@ -160,11 +163,11 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
     uint8_t cv, chh, ce, cq;
     row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
     const bool ein = INTERIOR || j + 1 < g.vy;  // uniform across the wave
-    *(GLB uint16_t *)crow = (uint16_t)(cv | (chh << 8));
+    *(GLB uint16_t *)(codes + crow) = (uint16_t)(cv | (chh << 8));
     const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
     // with no odd row below (ext edge) its words are not written
     const uint64_t b2 = __ballot(ein && (ce & CF_CRITICAL)), b3 = __ballot(ein && (cq & CF_CRITICAL));
-    if (ein) *(GLB uint16_t *)(crow + pitch) = (uint16_t)(ce | (cq << 8));
+    if (ein) *(GLB uint16_t *)(codes + (crow + (uint32_t)pitch)) = (uint16_t)(ce | (cq << 8));
     if (lane == 0) {
       *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
       seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
@@ -176,7 +179,7 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
     // ranks among the row segment's critical cells before this lane's
     const uint32_t rank_v = rank_below(b1, rank_below(b0, 0));
     const uint32_t rank_q = rank_below(b3, rank_below(b2, 0)) + (uint32_t)((b2 >> lane) & 1u);
-    crow += 2 * pitch;
+    crow += 2 * (uint32_t)pitch;
     rs0 += 2 * nseg;
     // codes outside the ext are 0, so those entries get T_NONE
     sl[ent_at(0, li, lj)] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
@@ -271,16 +274,19 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     sl[ent_at(1, lane, lj)] = (uint16_t)T_NONE;
   }
   // the border: exit terminals of side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y) at
-  // the position along that side (the entry's row for x sides, column for y)
-  for (int p = tid; p < 2 * RING_Q; p += 256) {
-    const int q = p >= RING_Q ? 1 : 0, pp = p - q * RING_Q;
-    int s, pos, bi, bj;  // side, position, border slot (column, row) in the padded array
-    if (pp < 2 * TX) {
-      s = pp < TX ? 2 : 3, pos = pp % TX, bi = pos + 1, bj = s == 2 ? TY + 1 : 0;
-    } else {
-      s = pp < 2 * TX + TY ? 0 : 1, pos = (pp - 2 * TX) % TY, bi = s == 0 ? TX + 1 : 0, bj = pos + 1;
+  // the position along that side (the entry's row for x sides, column for
+  // y).  Waves 0 / 1 fill the y sides of forest 0 / 1 (lane = column),
+  // waves 2 / 3 the x sides (lane = row).
+  {
+    uint16_t *f = sl + (wv & 1) * ENT_N;
+    const uint32_t pos = (uint32_t)lane;
+    if (wv < 2) {
+      f[(TY + 1) * ENT_W + lane + 1] = (uint16_t)(T_EXIT | (2u << 6) | pos);
+      f[lane + 1] = (uint16_t)(T_EXIT | (3u << 6) | pos);
+    } else if (lane < TY) {
+      f[(lane + 1) * ENT_W + TX + 1] = (uint16_t)(T_EXIT | (0u << 6) | pos);
+      f[(lane + 1) * ENT_W] = (uint16_t)(T_EXIT | (1u << 6) | pos);
     }
-    sl[q * ENT_N + bj * ENT_W + bi] = (uint16_t)(T_EXIT | ((uint32_t)s << 6) | (uint32_t)pos);
   }
   __syncthreads();
 
