@@ -33,56 +33,53 @@ static inline uint32_t hash32(uint32_t k) {
 void CellMap::reserve(size_t n) {
   size_t want = 16;
   while (want < n * 2) want <<= 1;
-  if (want <= keys_.size()) return;
-  std::vector<uint32_t> ok, ov;
-  ok.swap(keys_);
-  ov.swap(vals_);
-  keys_.assign(want, UINT32_MAX);
-  vals_.assign(want, 0);
+  if (want <= slots_.size()) return;
+  std::vector<Slot> old;
+  old.swap(slots_);
+  slots_.assign(want, Slot{UINT32_MAX, 0});
   mask_ = want - 1;
   n_ = 0;
-  for (size_t i = 0; i < ok.size(); ++i)
-    if (ok[i] != UINT32_MAX) set(ok[i], ov[i]);
+  for (const Slot &e : old)
+    if (e.key != UINT32_MAX) set(e.key, e.val);
 }
 
-void CellMap::grow() { reserve(keys_.empty() ? 16 : keys_.size()); }
+void CellMap::grow() { reserve(slots_.empty() ? 16 : slots_.size()); }
 
 size_t CellMap::slot(uint32_t key) const {
   size_t i = hash32(key) & mask_;
-  while (keys_[i] != UINT32_MAX && keys_[i] != key) i = (i + 1) & mask_;
+  const Slot *t = slots_.data();
+  while (t[i].key != UINT32_MAX && t[i].key != key) i = (i + 1) & mask_;
   return i;
 }
 
 uint32_t CellMap::find(uint32_t key) const {
-  if (keys_.empty()) return UINT32_MAX;
-  size_t i = slot(key);
-  return keys_[i] == key ? vals_[i] : UINT32_MAX;
+  if (slots_.empty()) return UINT32_MAX;
+  const Slot &e = slots_[slot(key)];
+  return e.key == key ? e.val : UINT32_MAX;
 }
 
-void CellMap::insert(uint32_t key, uint32_t val) {
-  if ((n_ + 1) * 2 > keys_.size()) grow();
-  size_t i = slot(key);
-  if (keys_[i] == key) return;
-  keys_[i] = key;
-  vals_[i] = val;
+uint32_t CellMap::find_or_insert(uint32_t key, uint32_t val) {
+  if ((n_ + 1) * 2 > slots_.size()) grow();
+  Slot &e = slots_[slot(key)];
+  if (e.key == key) return e.val;
+  e = Slot{key, val};
   ++n_;
+  return UINT32_MAX;
 }
 
 void CellMap::set(uint32_t key, uint32_t val) {
-  if ((n_ + 1) * 2 > keys_.size()) grow();
-  size_t i = slot(key);
-  if (keys_[i] != key) {
-    keys_[i] = key;
+  if ((n_ + 1) * 2 > slots_.size()) grow();
+  Slot &e = slots_[slot(key)];
+  if (e.key != key) {
+    e.key = key;
     ++n_;
   }
-  vals_[i] = val;
+  e.val = val;
 }
 
 uint32_t CellMap::get_or_zero(uint32_t key) {
-  uint32_t v = find(key);
-  if (v != UINT32_MAX) return v;
-  set(key, 0);
-  return 0;
+  const uint32_t v = find_or_insert(key, 0);
+  return v == UINT32_MAX ? 0 : v;
 }
 
 // ------------------------------------------------------------------ Complex
@@ -97,8 +94,8 @@ DiscSpan Complex::disc(uint32_t i, int j) const {
 
 uint32_t Complex::replicate(const Complex &src, uint32_t j) {
   uint32_t c = src.cell[j];
-  if (idx.contains(c)) throw std::logic_error("this cp is present in msc");
   uint32_t n = size();
+  if (idx.find_or_insert(c, n) != UINT32_MAX) throw std::logic_error("this cp is present in msc");
   cell.push_back(c);
   pair.push_back(UINT32_MAX);
   flags.push_back(src.flags[j]);
@@ -106,7 +103,6 @@ uint32_t Complex::replicate(const Complex &src, uint32_t j) {
   des.emplace_back();
   asc_disc.emplace_back();
   des_disc.emplace_back();
-  idx.set(c, n);
   return n;
 }
 
@@ -334,47 +330,72 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   pmax(m1.ext.x1, m1.ext.y1, m2.ext.x1, m2.ext.y1, tx, ty);
   out->ext = rect_of_points(bx, by, tx, ty);
   const Complex *arr[2] = {&m1, &m2};
-  size_t total = m1.size() + m2.size();
-  out->idx.reserve(total);
-  out->cell.reserve(total);
-  out->fn.reserve(total);
-  out->pair.reserve(total);
-  out->flags.reserve(total);
-  out->asc.reserve(total);
-  out->des.reserve(total);
-  out->asc_disc.reserve(total);
-  out->des_disc.reserve(total);
 
-  // at[i][j]: the index in `out` of the cell of m_i's node j (every
-  // non-cancelled node's cell is in `out` after this loop), so that the arcs
-  // below are translated without hashing their cells
+  // The reference walks msc1's then msc2's critical points, replicating
+  // each one not yet in the union, then copies every arc.  Here the union's
+  // indices are assigned first (at[i][j]: the index in `out` of m_i's node
+  // j, UINT32_MAX for a cancelled node that is not replicated): msc1's
+  // non-cancelled nodes in order, then msc2's nodes that are not msc1's
+  // non-cancelled cells, in order -- the indices the reference's loop
+  // gives.  The union's arrays are then filled by index, its cell map is
+  // built on one thread while two others copy the arcs into disjoint ranges
+  // of destination nodes, and the pair indices are looked up last, in the
+  // reference's order (a multiset's content does not depend on the order of
+  // its inserts, and each destination set is written by one thread).
   std::vector<uint32_t> at[2];
+  uint32_t n = 0;
+  at[0].assign(m1.size(), UINT32_MAX);
+  for (uint32_t j = 0; j < m1.size(); ++j)
+    if (!(m1.flags[j] & CPF_CANCELLED)) at[0][j] = n++;
+  const uint32_t n1 = n;
+  at[1].assign(m2.size(), UINT32_MAX);
+  for (uint32_t j = 0; j < m2.size(); ++j) {
+    // m2's cell among the union's cells so far, i.e. among m1's
+    // non-cancelled ones (m1's map may also hold zero entries for cells it
+    // never held, hence the check of the cell itself)
+    const uint32_t k = m1.idx.find(m2.cell[j]);
+    if (k != UINT32_MAX && k < m1.size() && m1.cell[k] == m2.cell[j] && at[0][k] != UINT32_MAX) {
+      at[1][j] = at[0][k];
+      continue;
+    }
+    if (m2.flags[j] & CPF_CANCELLED) continue;
+    at[1][j] = n++;
+  }
+  out->cell.resize(n);
+  out->fn.resize(n);
+  out->pair.assign(n, UINT32_MAX);
+  out->flags.resize(n);
+  out->asc.resize(n);
+  out->des.resize(n);
+  out->asc_disc.resize(n);
+  out->des_disc.resize(n);
   for (int i = 0; i < 2; ++i) {
     const Complex &m = *arr[i];
-    at[i].assign(m.size(), UINT32_MAX);
+    const uint32_t *ai = at[i].data();
     for (uint32_t j = 0; j < m.size(); ++j) {
-      if (i == 1) {
-        const uint32_t e = out->idx.find(m.cell[j]);
-        if (e != UINT32_MAX) {
-          at[i][j] = e;
-          continue;
-        }
-      }
-      if (m.flags[j] & CPF_CANCELLED) continue;
-      at[i][j] = out->replicate(m, j);
-      out->fn.push_back(m.fn[j]);
+      const uint32_t d = ai[j];
+      // m2's nodes on m1's cells keep m1's data
+      if (d == UINT32_MAX || (i == 1 && d < n1)) continue;
+      out->cell[d] = m.cell[j];
+      out->fn[d] = m.fn[j];
+      out->flags[d] = m.flags[j];
     }
   }
-  for (int i = 0; i < 2; ++i) {
+  auto build_map = [&] {
+    out->idx.reserve((size_t)m1.size() + m2.size());
+    for (uint32_t d = 0; d < n; ++d)
+      if (out->idx.find_or_insert(out->cell[d], d) != UINT32_MAX) throw std::logic_error("this cp is present in msc");
+  };
+  // arcs of m_i's node j into out; `lo_half` selects the destinations below
+  // n1 (m1's nodes) or from n1 on
+  auto copy_arcs = [&](int i, bool lo_half) {
     const Complex &m = *arr[i];
     const uint32_t *ai = at[i].data();
     for (uint32_t j = 0; j < m.size(); ++j) {
       if (m.flags[j] & CPF_CANCELLED) continue;
       const uint32_t di = ai[j];
-      // map::operator[] on the pair's cell (which inserts a zero entry
-      // when the pair is absent, as the reference's lookup does)
-      if (m.flags[j] & CPF_BCANC) out->pair[di] = out->idx.get_or_zero(m.cell[m.pair[j]]);
-      bool cmn = (i == 1) && ixn.contains(cx_of(m.cell[j]), cy_of(m.cell[j]));
+      if ((di < n1) != lo_half) continue;
+      const bool cmn = (i == 1) && ixn.contains(cx_of(m.cell[j]), cy_of(m.cell[j]));
       for (int k = 0; k < 2; ++k) {
         const ConnSet &src = k == 0 ? m.asc[j] : m.des[j];
         ConnSet &dst = k == 0 ? out->asc[di] : out->des[di];
@@ -388,6 +409,53 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
         }
       }
     }
+  };
+  auto lo = [&] {
+    copy_arcs(0, true);
+    copy_arcs(1, true);
+  };
+  auto hi = [&] { copy_arcs(1, false); };
+  if (n >= 16384 && host_threads() > 2) {
+    std::exception_ptr e1, e2;
+    std::thread t1([&] {
+      try {
+        build_map();
+      } catch (...) {
+        e1 = std::current_exception();
+      }
+    });
+    std::thread t2([&] {
+      try {
+        hi();
+      } catch (...) {
+        e2 = std::current_exception();
+      }
+    });
+    std::exception_ptr e0;
+    try {
+      lo();
+    } catch (...) {
+      e0 = std::current_exception();
+    }
+    t1.join();
+    t2.join();
+    if (e1) std::rethrow_exception(e1);
+    if (e0) std::rethrow_exception(e0);
+    if (e2) std::rethrow_exception(e2);
+  } else {
+    build_map();
+    lo();
+    hi();
+  }
+  // map::operator[] on each boundary-cancellable node's pair cell (which
+  // inserts a zero entry when the pair is absent, as the reference's lookup
+  // does), m1's nodes then m2's, so that m2's lookup wins for a shared node
+  for (int i = 0; i < 2; ++i) {
+    const Complex &m = *arr[i];
+    const uint32_t *ai = at[i].data();
+    for (uint32_t j = 0; j < m.size(); ++j)
+      if (!(m.flags[j] & CPF_CANCELLED) && (m.flags[j] & CPF_BCANC))
+        out->pair[ai[j]] = out->idx.get_or_zero(m.cell[m.pair[j]]);
   }
   for (int y = ixn.y0; y <= ixn.y1; ++y) {
     for (int x = ixn.x0; x <= ixn.x1; ++x) {

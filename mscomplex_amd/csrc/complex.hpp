@@ -132,7 +132,9 @@ class ConnSet {
   }
 };
 
-// packed cell id (x | y << 16) -> critical-point index
+// packed cell id (x | y << 16) -> critical-point index.  Open addressing
+// with linear probing; a slot holds its key and value side by side, so a
+// probe touches one cache line.
 class CellMap {
  public:
   void reserve(size_t n);
@@ -140,15 +142,20 @@ class CellMap {
   uint32_t find(uint32_t key) const;
   bool contains(uint32_t key) const { return find(key) != UINT32_MAX; }
   // map::insert semantics: keeps an existing value
-  void insert(uint32_t key, uint32_t val);
+  void insert(uint32_t key, uint32_t val) { find_or_insert(key, val); }
   // map::operator[] = val semantics: overwrites
   void set(uint32_t key, uint32_t val);
   // map::operator[] read semantics: inserts 0 when absent
   uint32_t get_or_zero(uint32_t key);
+  // one probe: the existing value, or UINT32_MAX after inserting val
+  uint32_t find_or_insert(uint32_t key, uint32_t val);
   size_t size() const { return n_; }
 
  private:
-  std::vector<uint32_t> keys_, vals_;
+  struct Slot {
+    uint32_t key, val;
+  };
+  std::vector<Slot> slots_;
   size_t n_ = 0, mask_ = 0;
   size_t slot(uint32_t key) const;
   void grow();

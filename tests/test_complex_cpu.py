@@ -172,3 +172,12 @@ def test_degenerate_tilings(case):
         assert len(mine) == orc.num_pieces()
         for i, c in enumerate(mine):
             assert np.array_equal(orc.dump(4, i), c.dump()), f"piece {i}"
+
+
+def test_large_merges_take_the_threaded_path():
+    # a root of more than 16384 nodes: merge_up builds the union's cell map
+    # on one thread while two others copy the arcs; the result must not
+    # depend on it
+    data = rng_field("white", 300, 260, 4)
+    compare(data, lx=1, ly=1, threshold=0.0)
+    compare(data, levels=2, threshold=0.01)
