@@ -3,6 +3,7 @@
 #include "heap.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,15 +22,6 @@ namespace mscx {
 
 // ------------------------------------------------------------------ CellMap
 
-static inline uint32_t hash32(uint32_t k) {
-  k ^= k >> 16;
-  k *= 0x7FEB352Du;
-  k ^= k >> 15;
-  k *= 0x846CA68Bu;
-  k ^= k >> 16;
-  return k;
-}
-
 void CellMap::reserve(size_t n) {
   size_t want = 16;
   while (want < n * 2) want <<= 1;
@@ -46,7 +38,7 @@ void CellMap::reserve(size_t n) {
 void CellMap::grow() { reserve(slots_.empty() ? 16 : slots_.size()); }
 
 size_t CellMap::slot(uint32_t key) const {
-  size_t i = hash32(key) & mask_;
+  size_t i = hash(key) & mask_;
   const Slot *t = slots_.data();
   while (t[i].key != UINT32_MAX && t[i].key != key) i = (i + 1) & mask_;
   return i;
@@ -90,6 +82,17 @@ DiscSpan Complex::disc(uint32_t i, int j) const {
   if (!xd_off.empty() && i + 1 < xd_off.size() && v.empty() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
     return DiscSpan{xd_cells.data() + xd_off[i], xd_off[i + 1] - xd_off[i]};
   return DiscSpan{v.data(), v.size()};
+}
+
+void Complex::reserve_nodes(size_t n) {
+  cell.reserve(n);
+  fn.reserve(n);
+  pair.reserve(n);
+  flags.reserve(n);
+  asc.reserve(n);
+  des.reserve(n);
+  asc_disc.reserve(n);
+  des_disc.reserve(n);
 }
 
 uint32_t Complex::replicate(const Complex &src, uint32_t j) {
@@ -148,8 +151,9 @@ void complex_build(Complex &c, const Rect &rect, const Rect &ext, const std::vec
   const uint32_t K = (uint32_t)crit.size();
   c.rect = rect;
   c.ext = ext;
-  c.cell = crit;
-  c.fn = fn;
+  c.reserve_nodes(Complex::with_headroom(K));
+  c.cell.assign(crit.begin(), crit.end());
+  c.fn.assign(fn.begin(), fn.end());
   c.pair.assign(K, UINT32_MAX);
   c.flags.assign(K, 0);
   c.asc.assign(K, ConnSet());
@@ -350,6 +354,7 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   const uint32_t n1 = n;
   at[1].assign(m2.size(), UINT32_MAX);
   for (uint32_t j = 0; j < m2.size(); ++j) {
+    if (j + 16 < m2.size()) m1.idx.prefetch(m2.cell[j + 16]);
     // m2's cell among the union's cells so far, i.e. among m1's
     // non-cancelled ones (m1's map may also hold zero entries for cells it
     // never held, hence the check of the cell itself)
@@ -361,6 +366,7 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
     if (m2.flags[j] & CPF_CANCELLED) continue;
     at[1][j] = n++;
   }
+  out->reserve_nodes(Complex::with_headroom(n));
   out->cell.resize(n);
   out->fn.resize(n);
   out->pair.assign(n, UINT32_MAX);
@@ -383,8 +389,10 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   }
   auto build_map = [&] {
     out->idx.reserve((size_t)m1.size() + m2.size());
-    for (uint32_t d = 0; d < n; ++d)
+    for (uint32_t d = 0; d < n; ++d) {
+      if (d + 16 < n) out->idx.prefetch(out->cell[d + 16]);
       if (out->idx.find_or_insert(out->cell[d], d) != UINT32_MAX) throw std::logic_error("this cp is present in msc");
+    }
   };
   // arcs of m_i's node j into out; `lo_half` selects the destinations below
   // n1 (m1's nodes) or from n1 on
@@ -490,19 +498,33 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
     }
   }
   // The two children are updated from P (which is only read from here on)
-  // independently of each other: large ones on two threads.
-  auto down = [&](Complex &m) {
+  // independently of each other: large ones on two threads.  Within a child,
+  // a first pass follows the reference's walk over the boundary-cancellable
+  // nodes without copying arcs: it makes the replicas in the order the copies
+  // would make them, sets the pair fields and raises the reference's checks.
+  // The arc sets of every node present in the child are then rewritten from
+  // the final index map, each by one thread (a set's content does not depend
+  // on when its neighbours were replicated).  Should two of P's nodes map to
+  // one node of the child (possible only through a zero entry of its cell
+  // map), the sets are rewritten in the reference's order on one thread.
+  auto down = [&](Complex &m, int nt) {
     std::vector<uint32_t> at;
     // at[j]: m's index of P's cell j (UINT32_MAX when absent), kept equal to
     // m.idx.find(P.cell[j]) as nodes are replicated into m
     at.resize(P.size());
-    for (uint32_t j = 0; j < P.size(); ++j) at[j] = m.idx.find(P.cell[j]);
+    constexpr uint32_t AHEAD = 16;
+    for (uint32_t j = 0; j < P.size(); ++j) {
+      if (j + AHEAD < P.size()) m.idx.prefetch(P.cell[j + AHEAD]);
+      at[j] = m.idx.find(P.cell[j]);
+    }
     auto rep = [&](uint32_t x) {
       const uint32_t r = m.replicate(P, x);
       m.fn.push_back(P.fn[x]);
       at[x] = r;
       return r;
     };
+    std::vector<uint32_t> fill;  // P's nodes whose sets are rewritten in m, in the reference's order
+    fill.reserve(P.size());
     for (uint32_t j = 0; j < P.size(); ++j) {
       if (P.flags[j] & CPF_CANCELLED) throw std::logic_error("all cps ought to be uncancelled by now");
       if (!(P.flags[j] & CPF_BCANC)) continue;
@@ -520,45 +542,75 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
         m.pair[d] = dp;
         m.pair[dp] = d;
       }
-      for (int k = 0; k < 2; ++k) {
-        const ConnSet &src = k == 0 ? P.asc[j] : P.des[j];
-        // `dst` is re-fetched after replicate() may have grown the arrays
-        (k == 0 ? m.asc[d] : m.des[d]).clear();
-        for (uint32_t v : src) {
+      fill.push_back(j);
+      for (int k = 0; k < 2; ++k)
+        for (uint32_t v : (k == 0 ? P.asc[j] : P.des[j])) {
           if (P.flags[v] & CPF_BCANC) throw std::logic_error("only non cancellable cps must be remaining");
-          uint32_t mv = at[v];
-          if (mv == UINT32_MAX) mv = rep(v);
-          (k == 0 ? m.asc[d] : m.des[d]).insert(mv);
+          if (at[v] == UINT32_MAX) rep(v);
         }
-      }
     }
-    for (uint32_t j = 0; j < P.size(); ++j) {
-      if (P.flags[j] & CPF_BCANC) continue;
-      uint32_t d = at[j];
-      if (d == UINT32_MAX) continue;
+    for (uint32_t j = 0; j < P.size(); ++j)
+      if (!(P.flags[j] & CPF_BCANC) && at[j] != UINT32_MAX) fill.push_back(j);
+    // a boundary-cancellable node's neighbours are all in m (replicated
+    // above); another node's neighbours absent from m are dropped
+    auto rewrite = [&](uint32_t j) {
+      const uint32_t d = at[j];
       for (int k = 0; k < 2; ++k) {
         const ConnSet &src = k == 0 ? P.asc[j] : P.des[j];
         ConnSet &dst = k == 0 ? m.asc[d] : m.des[d];
         dst.clear();
         for (uint32_t v : src) {
-          uint32_t mv = at[v];
-          if (mv == UINT32_MAX) continue;
-          dst.insert(mv);
+          const uint32_t mv = at[v];
+          if (mv != UINT32_MAX) dst.insert(mv);
         }
       }
+    };
+    bool distinct = true;
+    if (nt > 1 && fill.size() >= 8192) {
+      std::vector<uint8_t> seen(m.size(), 0);
+      for (uint32_t j : fill) {
+        if (seen[at[j]]) {
+          distinct = false;
+          break;
+        }
+        seen[at[j]] = 1;
+      }
+    }
+    if (nt > 1 && fill.size() >= 8192 && distinct) {
+      std::atomic<size_t> next(0);
+      const size_t nf = fill.size(), chunk = 512;
+      std::vector<std::exception_ptr> errs(nt);
+      auto work = [&](int t) {
+        try {  // rewrite() throws only when an allocation fails
+          for (size_t b = next.fetch_add(chunk); b < nf; b = next.fetch_add(chunk))
+            for (size_t q = b; q < std::min(nf, b + chunk); ++q) rewrite(fill[q]);
+        } catch (...) {
+          errs[t] = std::current_exception();
+        }
+      };
+      std::vector<std::thread> th;
+      for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+      work(0);
+      for (auto &t : th) t.join();
+      for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+    } else {
+      for (uint32_t j : fill) rewrite(j);
     }
   };
-  if (P.size() >= 4096 && host_threads() > 1) {
+  const int nt = host_threads();
+  if (P.size() >= 4096 && nt > 1) {
+    const int per = std::max(1, nt / 2);
     std::exception_ptr err;
     std::thread other([&] {
       try {
-        down(m2);
+        down(m2, per);
       } catch (...) {
         err = std::current_exception();
       }
     });
     try {
-      down(m1);
+      down(m1, per);
     } catch (...) {
       other.join();
       throw;
@@ -566,8 +618,8 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
     other.join();
     if (err) std::rethrow_exception(err);
   } else {
-    down(m1);
-    down(m2);
+    down(m1, 1);
+    down(m2, 1);
   }
 }
 

@@ -149,6 +149,19 @@ class CellMap {
   uint32_t get_or_zero(uint32_t key);
   // one probe: the existing value, or UINT32_MAX after inserting val
   uint32_t find_or_insert(uint32_t key, uint32_t val);
+  // starts loading key's home slot (loops over many keys call it a few
+  // keys ahead, so that the probes' cache misses overlap)
+  void prefetch(uint32_t key) const {
+    if (!slots_.empty()) __builtin_prefetch(&slots_[hash(key) & mask_]);
+  }
+  static uint32_t hash(uint32_t k) {
+    k ^= k >> 16;
+    k *= 0x7FEB352Du;
+    k ^= k >> 15;
+    k *= 0x846CA68Bu;
+    k ^= k >> 16;
+    return k;
+  }
   size_t size() const { return n_; }
 
  private:
@@ -196,6 +209,11 @@ struct Complex {
   int64_t arc_budget = INT64_MAX;
 
   uint32_t size() const { return (uint32_t)cell.size(); }
+  // capacity for n nodes in every per-node array
+  void reserve_nodes(size_t n);
+  // capacity a complex is built with: room for the few nodes merge_down
+  // replicates into it, so that those do not move every array
+  static size_t with_headroom(size_t n) { return n + n / 8 + 64; }
   // shallow_replicate_cp (grid_mscomplex.cpp:7-22)
   uint32_t replicate(const Complex &src, uint32_t j);
 
