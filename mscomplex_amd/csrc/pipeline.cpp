@@ -40,20 +40,29 @@ static void parallel_for(int n, F f, int nt = 0) {
   std::vector<std::exception_ptr> errs(nt);
   std::vector<int> err_at(nt, n);
   const int cap = thread_cap();
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back([&, t] {
-      ThreadCap keep(cap);
-      for (int i = next++; i < n; i = next++) {
-        try {
-          f(i);
-        } catch (...) {
-          if (i < err_at[t]) {
-            err_at[t] = i;
-            errs[t] = std::current_exception();
-          }
+  auto body = [&](int t) {
+    ThreadCap keep(cap);
+    for (int i = next++; i < n; i = next++) {
+      try {
+        f(i);
+      } catch (...) {
+        if (i < err_at[t]) {
+          err_at[t] = i;
+          errs[t] = std::current_exception();
         }
       }
-    });
+    }
+  };
+  // worker 0 is the calling thread; a worker that cannot be started leaves
+  // its share to the others
+  for (int t = 1; t < nt; ++t) {
+    try {
+      th.emplace_back(body, t);
+    } catch (...) {
+      break;
+    }
+  }
+  body(0);
   for (auto &x : th) x.join();
   int first = -1;
   for (int t = 0; t < nt; ++t)
@@ -294,7 +303,20 @@ void copy_pieces(const mscx_grid *src, mscx_grid *dst) {
 using namespace mscx;
 
 mscx_grid::~mscx_grid() {
-  for (mscx_complex *c : pieces) delete c;
+  // releasing a large grid's complexes (millions of small blocks) is spread
+  // over the host threads; it completes before the destructor returns
+  if (pieces.size() >= 8) {
+    try {
+      parallel_for((int)pieces.size(), [&](int i) {
+        delete pieces[i];
+        pieces[i] = nullptr;
+      });
+    } catch (...) {  // thread creation failed: release the rest here
+      for (mscx_complex *&c : pieces) delete c, c = nullptr;
+    }
+  } else {
+    for (mscx_complex *c : pieces) delete c;
+  }
   if (owns_tiles)
     for (mscx_tile *t : tiles) mscx_tile_destroy(t);
   if (dev_input) mscx_dev_free(dev_input);
