@@ -5,6 +5,8 @@
 // launch, reading the owner images that are already resident in HBM.
 #include <algorithm>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -14,8 +16,35 @@
 
 namespace mscx {
 
-static Scratch g_disc;   // host: codes + labels; device: descriptors, tables, labels
-static Scratch g_stage;  // host staging of descriptors and tables
+// Staging of one call: host codes + labels and device descriptors, tables
+// and labels (disc), host staging of descriptors and tables (stage).  Calls
+// from different threads (grids finishing together in a batch or on the
+// serving loop's workers) each lease their own set, so their host work runs
+// concurrently; a set is kept for reuse once returned, so steady-state calls
+// allocate nothing.
+struct DiscScratch {
+  Scratch disc, stage;
+};
+static std::mutex g_pool_mu;
+static std::vector<std::unique_ptr<DiscScratch>> g_pool;
+struct DiscLease {
+  std::unique_ptr<DiscScratch> s;
+  DiscLease() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (g_pool.empty()) {
+      s.reset(new DiscScratch);
+    } else {
+      s = std::move(g_pool.back());
+      g_pool.pop_back();
+    }
+  }
+  ~DiscLease() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(std::move(s));
+  }
+  DiscLease(const DiscLease &) = delete;
+  DiscLease &operator=(const DiscLease &) = delete;
+};
 
 template <class F>
 static void par_for(int n, F f) {
@@ -100,11 +129,13 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
 void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   if (n <= 0) return;
   ensure_init();
-  // The label pass runs on the auxiliary stream under its own lock, so it
-  // overlaps the tile stages of other grids on the work stream; the owner
-  // images and codes it reads were completed (and synchronised) by run_tiles.
-  static std::mutex disc_mu;  // guards g_disc / g_stage
-  std::lock_guard<std::mutex> lk(disc_mu);
+  // The label pass runs on the auxiliary stream, so it overlaps the tile
+  // stages of other grids on the work stream; the owner images and codes it
+  // reads were completed (and synchronised) by run_tiles.  Concurrent calls
+  // share the stream (a synchronisation may also wait for another call's
+  // copies) but not their staging.
+  DiscLease lease;
+  Scratch &g_disc = lease.s->disc, &g_stage = lease.s->stage;
   hipStream_t s = runtime().aux;
   for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i]->worked, MSCX_ESTATE, "tile has not been worked");
 

@@ -294,8 +294,15 @@ void grids_discs(std::vector<mscx_grid *> &grids) {
 
 void copy_pieces(const mscx_grid *src, mscx_grid *dst) {
   for (mscx_complex *c : dst->pieces) delete c;
-  dst->pieces.clear();
-  for (const mscx_complex *c : src->pieces) dst->pieces.push_back(new mscx_complex(*c));
+  dst->pieces.assign(src->pieces.size(), nullptr);
+  // the pieces are independent: copied on the host threads
+  try {
+    parallel_for((int)src->pieces.size(), [&](int i) { dst->pieces[i] = new mscx_complex(*src->pieces[i]); });
+  } catch (...) {
+    for (mscx_complex *&c : dst->pieces) delete c, c = nullptr;
+    dst->pieces.clear();
+    throw;
+  }
 }
 
 }  // namespace mscx

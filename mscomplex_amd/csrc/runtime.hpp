@@ -60,7 +60,7 @@ int guarded(F &&f) {
 struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;  // tile stages (run_tiles), serialised by mu
-  hipStream_t aux = nullptr;     // manifold label pass, serialised by its own lock
+  hipStream_t aux = nullptr;     // manifold passes (label kernel and their copies)
   std::mutex mu;
   // stage graphs (run_tiles), guarded by mu
   bool graphs = true;

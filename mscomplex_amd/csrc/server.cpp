@@ -27,6 +27,12 @@
 
 namespace mscx {
 
+// Whether a grid's merge levels run their merges on several host threads.
+// Small grids keep one thread each (many of them run at once on the
+// workers); large ones, whose merges take hundreds of milliseconds, use the
+// level parallelism the plain run uses.
+static bool large_grid(const mscx_grid *G) { return (int64_t)G->sx * G->sy >= ((int64_t)1 << 22); }
+
 namespace {
 
 // A computed grid kept for repeat requests.
@@ -126,7 +132,7 @@ struct mscx_server {
       G->threshold = R.threshold;
       G->flags = R.flags;
       copy_pieces(&B, G.get());
-      grid_simplify_down(G.get(), false);
+      grid_simplify_down(G.get(), large_grid(G.get()));
       std::vector<mscx_grid *> one{G.get()};
       grids_discs(one);
       R.result = std::move(G);
@@ -323,7 +329,7 @@ struct mscx_server {
   void first_run(Request &R) {
     mscx_grid *G = R.result.get();
     try {
-      grid_merge_up(G, false);
+      grid_merge_up(G, large_grid(G));
       if (R.res) {
         std::unique_ptr<mscx_grid> base(new mscx_grid);
         base->sx = G->sx;
@@ -353,7 +359,7 @@ struct mscx_server {
           post([this, W, res] { resimplify(*W, res); });
         }
       }
-      grid_simplify_down(G, false);
+      grid_simplify_down(G, large_grid(G));
       std::vector<mscx_grid *> one{G};
       grids_discs(one);
       if (R.res) release(R.res);
