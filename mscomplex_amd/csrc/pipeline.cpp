@@ -361,10 +361,18 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
       (*error)[gi] = e.what();
     }
   };
-  if (ng == 1)
+  if (ng == 1) {
     tree(0, true);
-  else
-    parallel_for(ng, [&](int gi) { tree(gi, false); });
+  } else {
+    // largest grids first: the batch ends when its slowest tree does, so
+    // the long ones start at once and the small ones fill the gaps
+    std::vector<int> order(ng);
+    for (int gi = 0; gi < ng; ++gi) order[gi] = gi;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (int64_t)grids[a]->sx * grids[a]->sy > (int64_t)grids[b]->sx * grids[b]->sy;
+    });
+    parallel_for(ng, [&](int k) { tree(order[k], false); });
+  }
   std::vector<mscx_grid *> done;
   for (int gi = 0; gi < ng; ++gi)
     if (ok[gi]) done.push_back(grids[gi]);
