@@ -250,8 +250,8 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   // A block is interior when every cell of it and of its one-vertex halo
   // lies strictly inside the interior rect: no boundary rule applies and
   // every key of the window is inside the ext.
-  const bool interior = (2 * i0 - 1 > g.rx0) && (2 * (i0 + TX) + 1 < g.rx1) && (2 * j0 - 1 > g.ry0) &&
-                        (2 * (j0 + TY) + 1 < g.ry1);
+  const bool cols_in = (2 * i0 - 1 > g.rx0) && (2 * (i0 + TX) + 1 < g.rx1);
+  const bool interior = cols_in && (2 * j0 - 1 > g.ry0) && (2 * (j0 + TY) + 1 < g.ry1);
   // 1. keys (0 outside the ext window; the rules never read those)
   if (interior)
     load_keys<true>(T, g, sk, i0, j0, lane, wv);
@@ -261,8 +261,14 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
 
   // 2. codes, critical flags per row segment, trace links
   // (the row walk schedules better with its first row in a vector register)
+  // A wave takes the interior walk when its own rows (with their one-vertex
+  // halo, and the next wave's first row its walk also evaluates) are
+  // interior, so boundary block rows run most of their waves without
+  // boundary tests; the keys it reads are then all inside the ext.
   const int lj0 = (tid >> 6) * RW;
-  if (interior)
+  const int wj0 = j0 + wv * RW;
+  const bool wave_in = cols_in && (2 * wj0 - 1 > g.ry0) && (2 * (wj0 + RW) + 1 < g.ry1);
+  if (wave_in)
     grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, lj0);
   else
     grad_rows<false>(T, g, sk, sl, i0, j0, bx, lane, lj0);
