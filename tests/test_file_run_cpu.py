@@ -15,6 +15,13 @@ from mscomplex_amd import build as mbuild
 CLI = mbuild.CLI
 
 
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    # the command line is built with the library; build both if it is missing
+    if not os.path.exists(CLI) or not os.path.exists(mbuild.LIB):
+        mbuild.build()
+
+
 def _cli(*args, cwd=None):
     return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=60)
 
