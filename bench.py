@@ -435,7 +435,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_grad_local", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": GRAD_TRAFFIC_BYTES,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
-                     "algorithmic_bytes": g_bytes, "avg_launch_ms": round(g_ms, 4)},
+                     "algorithmic_bytes": g_bytes, "avg_launch_ms": round(g_ms, 4),
+                     "limiter": "VALU issue: the gradient rules' integer compares and selects "
+                                "(~180 VALU per vertex row step of a lane), not HBM bandwidth"},
         "gradient_trace": {"kernels": ["k_grad_local", "k_ring_resolve", "k_owner"],
                            "algorithmic_bytes": g_bytes + t_bytes, "ms": round(gt_ms, 4),
                            "achieved_GBs": round(gt_ach, 1), "frac": round(gt_ach / HBM_PEAK_GBS, 4)},
