@@ -515,14 +515,28 @@ def uneven_mix(n, side_lo, side_hi, seed, aspect_max=4.0, thresholds=None):
     return out
 
 
+def _pick_tiling(res, margin):
+    """From [(seconds, tiling)] in candidate order (coarsest first): the
+    first, replaced by a later one only when faster by more than margin."""
+    choice = None
+    for el, tl in res:
+        if choice is None or el < choice[0] * (1.0 - margin):
+            choice = (el, tl)
+    return choice[1] if choice else (0, 0)
+
+
 def autotune_tiling(grids, thresholds, candidates=((0, 0), (1, 1), (2, 2)), classes=(256, 1024, 1 << 30),
-                    reps=1):
+                    reps=1, margin=0.05):
     """Leaf tiling (lx, ly) per size class for batched runs, chosen by timing.
 
     For each size class (grids whose larger side is below the class bound),
-    every candidate tiling runs the class's grids as one batch and the
-    fastest wins.  Returns (policy, table): policy(sx, sy) -> (lx, ly), and
-    the measured seconds per (class bound, candidate)."""
+    every candidate tiling runs the class's grids as one batch.  Candidates
+    are taken in order (coarsest first) and a later one replaces the current
+    choice only when it is faster by more than `margin` (a fraction): timing
+    noise on a small sample does not pick a finer tiling, whose merge tree
+    costs host time in the full batch.  Returns (policy, table):
+    policy(sx, sy) -> (lx, ly), and the measured seconds per (class bound,
+    candidate)."""
     import time as _time
     table = {}
     best = {}
@@ -542,7 +556,7 @@ def autotune_tiling(grids, thresholds, candidates=((0, 0), (1, 1), (2, 2)), clas
                 el = min(el, _time.perf_counter() - t0)
             table[(bound, (lx, ly))] = el
             res.append((el, (lx, ly)))
-        best[bound] = min(res)[1] if res else (0, 0)
+        best[bound] = _pick_tiling(res, margin)
 
     def policy(sx, sy):
         for bound in classes:

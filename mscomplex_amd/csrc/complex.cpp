@@ -863,7 +863,15 @@ void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t
 }
 
 void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv) {
-  for (uint32_t si : dp.surv_saddles) {
+  disc_paths_range(m, dp, rect, ext, cv, 0, dp.surv_saddles.size());
+}
+
+// The saddles [begin, end) of dp.surv_saddles; each saddle's paths are its
+// own, so disjoint ranges may run on different threads.
+void disc_paths_range(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv,
+                      size_t begin, size_t end) {
+  for (size_t q0 = begin; q0 < end; ++q0) {
+    const uint32_t si = dp.surv_saddles[q0];
     for (int j = 0; j < 2; ++j) {
       std::vector<uint32_t> &disc = j == 0 ? m.asc_disc[si] : m.des_disc[si];
       for (size_t k = 0; k != disc.size(); ++k) {

@@ -257,6 +257,8 @@ void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, con
                       std::vector<uint32_t> &lv, std::vector<uint32_t> &lq);
 void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq);
 void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv);
+void disc_paths_range(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv,
+                      size_t begin, size_t end);
 
 // postMergeFillDiscs for n worked tiles and their complexes, label pass on
 // the GPU (discs.cpp).

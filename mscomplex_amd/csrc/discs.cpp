@@ -4,6 +4,7 @@
 // map search per interior cell) runs on the GPU over every leaf in one
 // launch, reading the owner images that are already resident in HBM.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -46,28 +47,63 @@ struct DiscLease {
   DiscLease &operator=(const DiscLease &) = delete;
 };
 
+// Dynamic parallel loop over [0, n) on the host threads; of several failing
+// items the lowest index's exception is rethrown after all have joined.
 template <class F>
 static void par_for(int n, F f) {
-  const int nt = host_threads();
+  const int nt = std::min(n, host_threads());
   if (n <= 1 || nt <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
-  std::vector<std::thread> th;
-  std::vector<std::exception_ptr> errs(std::min(n, nt));
+  std::atomic<int> next(0);
+  std::vector<std::exception_ptr> errs(nt);
+  std::vector<int> err_at(nt, n);
   const int cap = thread_cap();
-  for (int t = 0; t < (int)errs.size(); ++t)
-    th.emplace_back([&, t] {
-      ThreadCap keep(cap);
+  auto body = [&](int t) {
+    ThreadCap keep(cap);
+    for (int i = next++; i < n; i = next++) {
       try {
-        for (int i = t; i < n; i += (int)errs.size()) f(i);
+        f(i);
       } catch (...) {
-        errs[t] = std::current_exception();
+        if (i < err_at[t]) {
+          err_at[t] = i;
+          errs[t] = std::current_exception();
+        }
       }
-    });
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) {
+    try {
+      th.emplace_back(body, t);
+    } catch (...) {
+      break;
+    }
+  }
+  body(0);
   for (auto &x : th) x.join();
-  for (auto &e : errs)
-    if (e) std::rethrow_exception(e);
+  int first = -1;
+  for (int t = 0; t < nt; ++t)
+    if (errs[t] && (first < 0 || err_at[t] < err_at[first])) first = t;
+  if (first >= 0) std::rethrow_exception(errs[first]);
+}
+
+// Work items of a per-leaf loop split into chunks of at most `chunk`
+// elements: (leaf, begin, end), the largest leaves' chunks first, so that a
+// few large leaves spread over the threads and small ones fill the gaps.
+struct Chunk {
+  int leaf;
+  size_t begin, end;
+};
+static std::vector<Chunk> chunks_of(const std::vector<size_t> &count, size_t chunk) {
+  std::vector<int> order(count.size());
+  for (size_t i = 0; i < count.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return count[a] > count[b]; });
+  std::vector<Chunk> out;
+  for (int i : order)
+    for (size_t b = 0; b < std::max<size_t>(count[i], 1); b += chunk) out.push_back({i, b, std::min(count[i], b + chunk)});
+  return out;
 }
 
 // The label of disc_labels_host (grid_dataset.cpp:1142-1187) depends on a
@@ -79,8 +115,10 @@ static void par_for(int n, F f) {
 // and self[k] (the critical cell k itself).  A cell owning another cell is
 // always critical in the leaf, except possibly a redirect target; for such a
 // target the owner is read from the device image.
+// Entries [kb, ke) of the tables (the redirect check runs with the chunk
+// that starts at 0).
 static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, const DiscPrep &dp,
-                              const mscx_tile *t, uint32_t *owned, uint32_t *self) {
+                              const mscx_tile *t, uint32_t *owned, uint32_t *self, uint32_t kb, uint32_t ke) {
   auto find = [](const std::vector<std::pair<uint32_t, uint32_t>> &v, uint32_t key) {
     auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(key, (uint32_t)0));
     return (it != v.end() && it->first == key) ? it->second : UINT32_MAX;
@@ -107,12 +145,13 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
   // a redirect at a vertex or quad that the leaf does not hold as critical
   // would change the label of a non-critical cell, which the tables cannot
   // express
-  for (const auto &r : dp.redirect) {
-    if (cell_dim(r.first) == 1 || !in_r(r.first)) continue;
-    uint32_t i = m.idx.find(r.first);
-    if (i == UINT32_MAX || i >= K) throw std::logic_error("owner redirect at a cell that is not critical in the leaf");
-  }
-  for (uint32_t k = 0; k < K; ++k) {
+  if (kb == 0)
+    for (const auto &r : dp.redirect) {
+      if (cell_dim(r.first) == 1 || !in_r(r.first)) continue;
+      uint32_t i = m.idx.find(r.first);
+      if (i == UINT32_MAX || i >= K) throw std::logic_error("owner redirect at a cell that is not critical in the leaf");
+    }
+  for (uint32_t k = kb; k < ke; ++k) {
     const uint32_t c = m.cell[k];
     const uint32_t dim = cell_dim(c);
     if (dim == 1) {  // saddles own no vertex or quad
@@ -169,7 +208,15 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     cv[i] = CodesView{(const uint8_t *)host + code_off[i], tiles[i]->d.pitch, ext[i].x0, ext[i].y0};
 
   // 2. critical-point pass (seeds, redirects, surviving maps)
-  par_for(n, [&](int i) { disc_prepare(*cs[i], rect[i], cv[i], dp[i]); });
+  std::vector<size_t> nk(n);
+  for (int i = 0; i < n; ++i) nk[i] = tiles[i]->K;
+  {
+    const std::vector<Chunk> whole = chunks_of(nk, ~(size_t)0);  // one item per leaf, largest first
+    par_for((int)whole.size(), [&](int q) {
+      const int i = whole[q].leaf;
+      disc_prepare(*cs[i], rect[i], cv[i], dp[i]);
+    });
+  }
 
   // 3. label tables per critical index (host) + label pass (device)
   std::vector<size_t> tab_off(n + 1, 0);
@@ -182,11 +229,17 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   char *dev = (char *)g_disc.reserve_dev(dev_bytes);
   char *stage = (char *)g_stage.reserve_host(o_lab);
   DiscDesc *hd = (DiscDesc *)(stage + o_desc);
-  par_for(n, [&](int i) {
-    const uint32_t K = tiles[i]->K;
-    uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
-    disc_label_tables(*cs[i], K, rect[i], dp[i], tiles[i], owned, owned + K);
-  });
+  {
+    // entries of a leaf's tables are independent: large leaves are split
+    const std::vector<Chunk> items = chunks_of(nk, 1 << 16);
+    par_for((int)items.size(), [&](int q) {
+      const int i = items[q].leaf;
+      const uint32_t K = tiles[i]->K;
+      uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
+      disc_label_tables(*cs[i], K, rect[i], dp[i], tiles[i], owned, owned + K, (uint32_t)items[q].begin,
+                        (uint32_t)items[q].end);
+    });
+  }
   size_t lo = 0;
   std::vector<size_t> lab_off(n);
   for (int i = 0; i < n; ++i) {
@@ -235,12 +288,24 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   uint32_t *herr = (uint32_t *)(stage + o_err);
   MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
 
-  // 4. saddle tracing (needs no labels) while the labels arrive, then the
-  //    extremum discs (grouping), per leaf in parallel; a label the kernel
-  //    flagged as bad is the none label, which grouping skips
+  // 4. saddle tracing (needs no labels; each saddle's paths are its own, so
+  //    a large leaf's saddles are split) while the labels arrive, then the
+  //    extremum discs (grouping: one item per leaf, after every tracing
+  //    item; it writes other fields of the complex than the tracing does);
+  //    a label the kernel flagged as bad is the none label, which grouping
+  //    skips
+  std::vector<size_t> ns(n);
+  for (int i = 0; i < n; ++i) ns[i] = dp[i].surv_saddles.size();
+  std::vector<Chunk> items = chunks_of(ns, 1 << 13);
+  const int ntrace = (int)items.size();
+  for (int i = 0; i < n; ++i) items.push_back({i, 0, 0});
   try {
-    par_for(n, [&](int i) {
-      disc_paths(*cs[i], dp[i], rect[i], ext[i], cv[i]);
+    par_for((int)items.size(), [&](int q) {
+      const int i = items[q].leaf;
+      if (q < ntrace) {
+        disc_paths_range(*cs[i], dp[i], rect[i], ext[i], cv[i], items[q].begin, items[q].end);
+        return;
+      }
       MSCX_HIP(hipEventSynchronize(ev.e[i]));
       const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);
       disc_group(*cs[i], rect[i], lv, lv + nv[i]);

@@ -54,3 +54,11 @@ def test_write_discs_format(tmp_path):
     assert c.num_cps() == 1
     c.write_discs(str(tmp_path) + "/dp_")
     assert os.listdir(tmp_path) == []  # empty discs write nothing
+
+
+def test_autotune_prefers_the_coarser_tiling_within_the_margin():
+    pick = mx._pick_tiling
+    assert pick([], 0.05) == (0, 0)
+    assert pick([(0.5709, (0, 0)), (0.5702, (1, 1)), (0.6376, (2, 2))], 0.05) == (0, 0)
+    assert pick([(1.0, (0, 0)), (0.90, (1, 1)), (0.80, (2, 2))], 0.05) == (2, 2)
+    assert pick([(1.0, (0, 0)), (0.90, (1, 1)), (0.88, (2, 2))], 0.05) == (1, 1)
