@@ -45,8 +45,30 @@ def lib():
         L.orc_pipeline_rect.argtypes = [P, ctypes.c_int, P]
         L.orc_pipeline_times.argtypes = [P, P]
         L.orc_pipeline_free.argtypes = [P]
+        L.orc_set_plant.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
+
+
+class plant:
+    """Context manager that runs the oracle with one planted mistake (the
+    values are listed at g_plant in oracle.cpp); the pin tests use it to show
+    that each check fails on a plausible error.  stop_after < 4 ends every
+    pipeline after that phase (0 compute .. 4 discs), before later phases'
+    own invariant checks can throw.  Not thread-safe: pipelines
+    run inside it must use threads=1."""
+
+    def __init__(self, which, stop_after=4):
+        self.which = int(which)
+        self.stop_after = int(stop_after)
+
+    def __enter__(self):
+        lib().orc_set_plant(self.which, self.stop_after)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0, 4)
+        return False
 
 
 def _p(a):

@@ -21,6 +21,20 @@
 
 namespace orc {
 
+// Planted mistakes for the pin tests (tests/test_oracle_pins_cpu.py): 0 in
+// every normal run.  Each value makes one plausible error in one function,
+// so that a test can show that its check fails on that error.
+//   1 merge_up keeps the shared line's duplicate arcs (no dedup)
+//   2 cancelPairs adds new arcs with set semantics (drops multiplicity)
+//   3 uncancel_pairs keeps only non-cancellable neighbours (no expansion
+//     through a cancelled neighbour's pair)
+//   4 simplify starts max_val at the lowest float instead of FLT_MIN
+int g_plant = 0;
+// Last phase DataManager::run performs (0 compute .. 4 discs); the pin tests
+// lower it to look at a planted mistake before a later phase's own
+// invariant checks throw.
+int g_stop_after = 4;
+
 static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -506,6 +520,7 @@ void cancelPairs(GridMSComplex *msc, uint cp0, uint cp1, std::vector<uint> *new_
   for (auto a = p0->asc.begin(); a != p0->asc.end(); ++a) {
     for (auto d = p1->des.begin(); d != p1->des.end(); ++d) {
       if (*a == cp1 || *d == cp0) continue;
+      if (g_plant == 2 && msc->m_cps[*a]->des.count(*d)) continue;
       msc->m_cps[*a]->des.insert(*d);
       msc->m_cps[*d]->asc.insert(*a);
       if (new_edges) {
@@ -546,7 +561,7 @@ void uncancel_pairs(GridMSComplex *msc, uint cp1, uint cp2) {
     for (auto it = acdc[i]->begin(); it != acdc[i]->end(); ++it) {
       if (*it == other) continue;
       critpt_t *conn = msc->m_cps[*it];
-      if (!conn->isBoundryCancelable) {
+      if (!conn->isBoundryCancelable || g_plant == 3) {
         fresh.insert(*it);
         continue;
       }
@@ -622,7 +637,7 @@ GridMSComplex *GridMSComplex::merge_up(const GridMSComplex &msc1, const GridMSCo
       for (uint k = 0; k < 2; ++k) {
         for (auto it = s_acdc[k]->begin(); it != s_acdc[k]->end(); ++it) {
           const critpt_t *conn = msc->m_cps[*it];
-          if (cmn && ixn.contains(conn->cellid)) continue;
+          if (cmn && ixn.contains(conn->cellid) && g_plant != 1) continue;
           if (conn->isCancelled) continue;
           d_acdc[k]->insert(out->m_id_cp_map[conn->cellid]);
         }
@@ -762,6 +777,7 @@ void GridMSComplex::simplify(crit_idx_pair_list_t &canc_pairs_list, double thres
   std::priority_queue<crit_idx_pair_t, crit_idx_pair_list_t, persistence_comparator_t> pq(comp);
 
   fn_t max_val = std::numeric_limits<fn_t>::min();  // FLT_MIN, as at :399
+  if (g_plant == 4) max_val = std::numeric_limits<fn_t>::lowest();
   fn_t min_val = std::numeric_limits<fn_t>::max();
 
   for (uint i = 0; i < m_cps.size(); ++i) {
@@ -986,12 +1002,14 @@ void DataManager::run(PipelineSnapshots *snaps, int nthreads) {
   double t2 = now_s();
   S.t_merge_up = t2 - t1;
   snapshot(S.after_merge_up);
+  if (g_stop_after < 2) return;
 
   // root simplification (:651-652)
   if (m_simp_tresh > 0.0) m_pieces.back()->msgraph->simplify_un_simplify(m_simp_tresh);
   double t3 = now_s();
   S.t_simplify = t3 - t2;
   snapshot(S.after_simplify);
+  if (g_stop_after < 3) return;
 
   // mergePiecesDown (:308-357), single-threaded order.  With zero levels the
   // reference's loop bound shifts by -1; no iteration runs.
@@ -1022,6 +1040,7 @@ void DataManager::run(PipelineSnapshots *snaps, int nthreads) {
   double t4 = now_s();
   S.t_merge_down = t4 - t3;
   snapshot(S.after_merge_down);
+  if (g_stop_after < 4) return;
 
   // collectManifold (:397-422): re-init (flags reset, critical list kept)
   // and re-run work(), then postMergeFillDiscs.

@@ -22,6 +22,10 @@
 
 namespace orc {
 
+extern int g_plant;  // planted mistake for the pin tests (0: none)
+extern int g_stop_after;  // last pipeline phase run (4: all)
+
+
 typedef int16_t coord_t;
 typedef float fn_t;
 typedef unsigned int uint;

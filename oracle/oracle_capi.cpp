@@ -16,6 +16,14 @@ extern "C" {
 
 const char *orc_last_error() { return g_err.c_str(); }
 
+// Selects a planted mistake (0 = none; the values are listed at g_plant in
+// oracle.cpp).  Only the pin tests call it, to show their checks fail on
+// each mistake.
+void orc_set_plant(int v, int stop_after) {
+  orc::g_plant = v;
+  orc::g_stop_after = stop_after;
+}
+
 // GridDataset(rect, ext).init(data).work() for one tile.  crit receives the
 // critical list (packed cellids, x | y<<16) when crit_cap suffices; *K gets
 // its length either way.
