@@ -58,8 +58,10 @@ int mscx_profile_read(double *ms, uint32_t *counts, int n);
  * download) is built into a HIP graph (node by node: no stream is put in
  * capture mode) once per launch configuration and replayed by later batches
  * with the same configuration; with 0 every stage
- * is launched directly.  While mscx_profile is enabled stages are launched
- * directly.  *previous (may be NULL) receives the old setting;
+ * is launched directly.  While mscx_profile is enabled the graph also
+ * records the stage timer's HIP events around its kernels, so the timed
+ * submission path is the default one.  *previous (may be NULL) receives the
+ * old setting;
  * mscx_graph_stats counts graph builds and replays since mscx_init. */
 int mscx_set_graphs(int on, int *previous);
 int mscx_graph_stats(uint64_t *builds, uint64_t *replays);

@@ -874,7 +874,7 @@ void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int 
 }
 hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, size_t up_bytes, void *dl_dst,
                              const void *dl_src, size_t dl_bytes, const TileDesc *d_tiles, const int *d_prefix,
-                             int ntiles, const int *nblocks, uint32_t *d_err) {
+                             int ntiles, const int *nblocks, uint32_t *d_err, const hipEvent_t *ev) {
   hipGraph_t g = nullptr;
   hipError_t e = hipGraphCreate(&g, 0);
   if (e != hipSuccess) return e;
@@ -900,6 +900,13 @@ hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, 
     if (r == hipSuccess) chain(n);
     return r;
   };
+  auto event_node = [&](int k) -> hipError_t {
+    if (!ev) return hipSuccess;
+    hipGraphNode_t n;
+    hipError_t r = hipGraphAddEventRecordNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, ev[k]);
+    if (r == hipSuccess) chain(n);
+    return r;
+  };
   const int *pg = d_prefix + KB_GRADIENT * (ntiles + 1), *ps = d_prefix + KB_SCAN * (ntiles + 1);
   const int *pr = d_prefix + KB_RING * (ntiles + 1);
   void *a_grad[] = {(void *)&d_tiles, (void *)&pg, (void *)&ntiles, (void *)&d_err};
@@ -909,12 +916,17 @@ hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, 
   void *a_own[] = {(void *)&d_tiles, (void *)&pg, (void *)&ntiles};
   const bool scan = nblocks[KB_SCAN] > 0;
   if ((e = memcpy_node(up_dst, up_src, up_bytes, hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = event_node(0)) == hipSuccess &&
       (e = kernel_node((const void *)k_grad_local, nblocks[KB_GRADIENT], a_grad)) == hipSuccess &&
+      (e = event_node(1)) == hipSuccess && (e = event_node(2)) == hipSuccess &&
       (e = kernel_node((const void *)k_scan_reduce, nblocks[KB_SCAN], a_scan)) == hipSuccess &&
       (e = kernel_node((const void *)k_scan_parts, scan ? ntiles : 0, a_parts)) == hipSuccess &&
       (e = kernel_node((const void *)k_scan_down, nblocks[KB_SCAN], a_scan)) == hipSuccess &&
+      (e = event_node(3)) == hipSuccess && (e = event_node(4)) == hipSuccess &&
       (e = kernel_node((const void *)k_ring_resolve, nblocks[KB_RING], a_ring)) == hipSuccess &&
+      (e = event_node(5)) == hipSuccess && (e = event_node(6)) == hipSuccess &&
       (e = kernel_node((const void *)k_owner, nblocks[KB_GRADIENT], a_own)) == hipSuccess &&
+      (e = event_node(7)) == hipSuccess &&
       (e = memcpy_node(dl_dst, dl_src, dl_bytes, hipMemcpyDeviceToHost)) == hipSuccess) {
     *out = g;
     return hipSuccess;

@@ -32,8 +32,12 @@ void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int 
 // descriptor block, gradient, scan, ring resolution, owner images, download
 // of the counts) as a HIP graph built node by node, so that no stream is
 // ever in capture mode.  d_prefix holds KB_COUNT prefix tables of ntiles+1.
+// With ev != NULL the graph also records ev[0]/ev[1] around the gradient
+// kernel, ev[2]/ev[3] around the scan, ev[4]/ev[5] around the ring
+// resolution and ev[6]/ev[7] around the owner kernel (stage timing of the
+// replayed graph).
 hipError_t build_front_graph(hipGraph_t *out, void *up_dst, const void *up_src, size_t up_bytes, void *dl_dst,
                              const void *dl_src, size_t dl_bytes, const TileDesc *d_tiles, const int *d_prefix,
-                             int ntiles, const int *nblocks, uint32_t *d_err);
+                             int ntiles, const int *nblocks, uint32_t *d_err, const hipEvent_t *ev);
 
 }  // namespace mscx

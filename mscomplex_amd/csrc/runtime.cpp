@@ -137,12 +137,16 @@ StageTimer &stage_timer() {
   return t;
 }
 
-void StageTimer::begin(int k, hipStream_t s) {
-  if (!on) return;
+void StageTimer::ensure(int k) {
   if (!a[k]) {
     MSCX_HIP(hipEventCreate(&a[k]));
     MSCX_HIP(hipEventCreate(&b[k]));
   }
+}
+
+void StageTimer::begin(int k, hipStream_t s) {
+  if (!on) return;
+  ensure(k);
   MSCX_HIP(hipEventRecord(a[k], s));
 }
 

@@ -131,6 +131,7 @@ struct StageTimer {
   bool pending[ST_COUNT] = {};
   double ms[ST_COUNT] = {};
   uint32_t count[ST_COUNT] = {};
+  void ensure(int k);  // creates stage k's event pair
   void begin(int k, hipStream_t s);
   void end(int k, hipStream_t s);
   void collect();  // call after a stream synchronisation

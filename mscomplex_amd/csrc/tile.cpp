@@ -10,7 +10,7 @@ namespace mscx {
 
 static Scratch g_batch;  // descriptors, block prefixes, K array, flags
 
-static void check_rects(const int32_t *rect, const int32_t *ext) {
+void check_rects(const int32_t *rect, const int32_t *ext) {
   for (int k = 0; k < 4; ++k) {
     MSCX_REQUIRE(rect[k] >= 0 && ext[k] >= 0 && rect[k] <= 32766 && ext[k] <= 32766, MSCX_EINVAL,
                  "rect/ext coordinates must lie in [0, 32766]");
@@ -27,7 +27,7 @@ static void check_rects(const int32_t *rect, const int32_t *ext) {
 // Geometry of a tile over ext and the layout of its one device allocation:
 // codes, counts, offsets, scan partials, critical flags, owners, local
 // terminals, ring words.  Returns the allocation size; offsets in o[8].
-static size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[8]) {
+size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[8]) {
   TileGeom &g = d.g;
   g.vx = (ext[2] - ext[0]) / 2 + 1;
   g.vy = (ext[3] - ext[1]) / 2 + 1;
@@ -162,8 +162,18 @@ void run_tiles(mscx_tile *const *tiles, int n) {
     MSCX_HIP(hipGetLastError());
     MSCX_HIP(hipMemcpyAsync(hk, dk, o_err + 4 - o_k, hipMemcpyDeviceToHost, s));
   };
-  if (rt.graphs && !tm.on) {
-    std::vector<uintptr_t> key{(uintptr_t)n, (uintptr_t)dev, (uintptr_t)host, total};
+  if (rt.graphs) {
+    // with stage timing on, the graph records the timer's events around its
+    // kernels (a graph of its own: the timing flag is part of the key)
+    hipEvent_t ev[8] = {};
+    const int evst[4] = {ST_GRADIENT, ST_SCAN, ST_TRACE, ST_OWNER};
+    if (tm.on)
+      for (int k = 0; k < 4; ++k) {
+        tm.ensure(evst[k]);
+        ev[2 * k] = tm.a[evst[k]];
+        ev[2 * k + 1] = tm.b[evst[k]];
+      }
+    std::vector<uintptr_t> key{(uintptr_t)n, (uintptr_t)dev, (uintptr_t)host, total, (uintptr_t)tm.on};
     for (int k = 0; k < KB_CRIT; ++k) key.push_back((uintptr_t)nb[k]);
     if (!rt.graph_exec || rt.graph_key != key) {
       if (rt.graph_exec) {
@@ -171,7 +181,8 @@ void run_tiles(mscx_tile *const *tiles, int n) {
         rt.graph_exec = nullptr;
       }
       hipGraph_t g = nullptr;
-      MSCX_HIP(build_front_graph(&g, dev, host, total, hk, dk, o_err + 4 - o_k, dd, dp, n, nb, derr));
+      MSCX_HIP(build_front_graph(&g, dev, host, total, hk, dk, o_err + 4 - o_k, dd, dp, n, nb, derr,
+                                 tm.on ? ev : nullptr));
       const hipError_t e = hipGraphInstantiate(&rt.graph_exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       MSCX_HIP(e);
@@ -181,6 +192,9 @@ void run_tiles(mscx_tile *const *tiles, int n) {
       ++rt.graph_replays;
     }
     MSCX_HIP(hipGraphLaunch(rt.graph_exec, s));
+    if (tm.on)
+      for (int k = 0; k < 4; ++k) tm.pending[evst[k]] = nb[k == 0 ? KB_GRADIENT : k == 1 ? KB_SCAN
+                                                              : k == 2 ? KB_RING : KB_GRADIENT] > 0;
   } else {
     chain();
   }

@@ -44,6 +44,14 @@ def test_config_gaussians_512_l0():
     _compare(mx.gen_gaussians(512, 512, 32, 1e-3, seed=1), "gauss512", levels=0, threshold=0.0)
 
 
+@pytest.mark.timeout(300)
+def test_white_512_single_leaf():
+    """One leaf with 222,235 critical cells: the manifold pass splits a
+    large leaf's label tables into chunks of 65,536 critical points, so this
+    case reaches the later chunks (the 4096^2 bench leaves hold ~12k)."""
+    _compare(mx.gen_white(512, 512, levels=0, seed=5), "white512-l0", levels=0, threshold=0.0)
+
+
 def test_gaussians_512_slabs():
     _compare(mx.gen_gaussians(512, 512, 32, 1e-3, seed=1), "gauss512-l2", levels=2, threshold=0.01)
 
