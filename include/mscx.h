@@ -99,6 +99,25 @@ int mscx_tile_work(mscx_tile *t);
 /* Batched work(): one launch per stage over all n tiles. */
 int mscx_tiles_work(mscx_tile *const *tiles, int n);
 
+/* mscx_tile_work without any device allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It runs the same kernels as mscx_tile_work on one tile
+ * over (rect, ext) whose input window is already in HBM (dev_verts, row_pitch
+ * floats per row) and writes the tile's critical cells into the caller's
+ * device arrays of kcap entries: dev_crit (packed ids, raster order),
+ * dev_fn (get_cell_fn values), dev_conn4 (4 per cell, the layout of
+ * mscx_tile_download_critical) and dev_pair; *K (host) receives K.  The
+ * tile's intermediates live in a static 32 MiB device array per GPU, so the
+ * tile may have at most about 1026 x 1026 vertices (larger tiles fail with
+ * MSCX_EINVAL); K > kcap fails with MSCX_EINVAL after writing K to *K.  The
+ * call allocates no device or pinned host memory; calls on one GPU share its
+ * array and are serialised by the library (the array keeps the last call's
+ * contents, and callers that bypass the library's lock on one GPU race). */
+int mscx_tile_work_noalloc(const int32_t rect[4], const int32_t ext[4], const float *dev_verts, int64_t row_pitch,
+                           uint32_t kcap, uint32_t *dev_crit, float *dev_fn, uint32_t *dev_conn4,
+                           uint32_t *dev_pair, uint32_t *K);
+
 /* Number of critical cells K (m_critical_cells.size()). */
 int mscx_tile_num_critical(const mscx_tile *t, uint32_t *K);
 

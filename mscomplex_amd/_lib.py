@@ -37,6 +37,7 @@ _SIGS = {
     "mscx_tile_bind_device": (c_int, [P, P, c_int64]),
     "mscx_tile_work": (c_int, [P]),
     "mscx_tiles_work": (c_int, [P, c_int]),
+    "mscx_tile_work_noalloc": (c_int, [P, P, P, c_int64, c_uint32, P, P, P, P, ctypes.POINTER(c_uint32)]),
     "mscx_tile_num_critical": (c_int, [P, ctypes.POINTER(c_uint32)]),
     "mscx_tile_download": (c_int, [P, P, P, P, P]),
     "mscx_tile_download_codes": (c_int, [P, P]),
