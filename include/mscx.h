@@ -29,6 +29,7 @@ extern "C" {
 #define MSCX_ENOMEM -4    /* allocation failure or HBM budget exceeded */
 #define MSCX_ESTATE -5    /* call out of order (e.g. download before work) */
 #define MSCX_EIO -6       /* input file cannot be opened or is too short (reference: runtime_error "unable to read stream") */
+#define MSCX_ELIMIT -7    /* a served request reached its work limit (no reference counterpart) */
 
 typedef struct mscx_tile mscx_tile;       /* a GridDataset: one rect + ext window on the GPU */
 typedef struct mscx_complex mscx_complex; /* a GridMSComplex (host resident) */
@@ -269,6 +270,17 @@ int mscx_batch_destroy(mscx_batch *b);
  * with MSCX_ENOMEM. */
 typedef struct mscx_server mscx_server;
 int mscx_server_create(size_t hbm_budget, int32_t host_threads, mscx_server **out);
+/* Work limit of the requests' simplification (no reference counterpart:
+ * the reference runs until done, which on noisy fields at large thresholds
+ * takes minutes to hours as strangulated pairs are resubmitted after every
+ * cancellation, and grows arc multisets without bound).  A request whose
+ * simplification needs more than max(2^24, factor x its initial heap
+ * entries) heap pops fails with MSCX_ELIMIT, one that would insert more than
+ * max(2^24, 2 x factor x its arc entries) arc entries with MSCX_ENOMEM;
+ * both in seconds, so the worker moves on.  Default 128 (the 4096^2 value
+ * noise root at 1% takes 35x); 0 removes the limit.  Applies to requests
+ * that start after the call. */
+int mscx_server_set_work_limit(mscx_server *s, double factor);
 int mscx_server_submit(mscx_server *s, const float *data, int32_t size_x, int32_t size_y, int32_t lx, int32_t ly,
                        double threshold, uint64_t key, uint32_t flags, uint64_t *ticket);
 /* Waits up to timeout_s seconds (< 0: no limit) for a request; *out is

@@ -216,13 +216,17 @@ def work_tiles(tiles):
 class GridMSComplex:
     """Host-resident MS complex; mirrors dmsc GridMSComplex."""
 
-    def __init__(self, handle, owned=True):
+    def __init__(self, handle, owned=True, owner=None):
         self._h = handle
         self._owned = owned
+        # a borrowed complex keeps the object that owns it alive
+        self._owner = owner
 
     def __del__(self):
+        if not getattr(self, "_owned", False):
+            return
         h = getattr(self, "_h", None)
-        if getattr(self, "_owned", False) and h and h.value:
+        if h and h.value:
             try:
                 lib().mscx_complex_destroy(h)
             except Exception:
@@ -305,26 +309,46 @@ class GridMSComplex:
         return buf
 
 
+class _ViewComplex(GridMSComplex):
+    """A piece of a served request's grid: valid while the request is."""
+
+    @property
+    def _h(self):
+        self._owner._handle()  # raises once the request is released
+        return self.__dict__["_hh"]
+
+    @_h.setter
+    def _h(self, v):
+        self.__dict__["_hh"] = v
+
+
 class _GridView:
-    """Borrowed view of one grid of a batch (same accessors as GridRun)."""
+    """Borrowed view of one grid of a batch or of a served request (same
+    accessors as GridRun).  It keeps its owner alive; a served request's
+    view becomes invalid when the request is released."""
 
     def __init__(self, handle, owner):
         self._h = handle
         self._owner = owner
 
+    def _handle(self):
+        if self._h is None:
+            raise MscxError(-5, "the request of this view has been released")
+        return self._h
+
     def num_pieces(self):
         n = ctypes.c_int32()
-        check(lib().mscx_grid_num_pieces(self._h, ctypes.byref(n)))
+        check(lib().mscx_grid_num_pieces(self._handle(), ctypes.byref(n)))
         return n.value
 
     def piece(self, i):
         h = ctypes.c_void_p()
-        check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
-        return GridMSComplex(h, owned=False)
+        check(lib().mscx_grid_piece(self._handle(), i, ctypes.byref(h)))
+        return _ViewComplex(h, owned=False, owner=self)
 
     def times(self):
         t = np.zeros(5, np.float64)
-        check(lib().mscx_grid_times(self._h, ptr(t)))
+        check(lib().mscx_grid_times(self._handle(), ptr(t)))
         return t
 
 
@@ -389,18 +413,28 @@ class Server:
         h = ctypes.c_void_p()
         check(lib().mscx_server_create(int(hbm_budget), int(host_threads), ctypes.byref(h)))
         self._h = h
+        self._views = {}  # ticket -> views handed out by wait(), invalidated by release()
 
     def close(self):
         h = getattr(self, "_h", None)
         if h and h.value:
             check(lib().mscx_server_destroy(h))
             self._h = None
+            for vs in self._views.values():
+                for v in vs:
+                    v._h = None
+            self._views.clear()
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+    def set_work_limit(self, factor):
+        """Simplification work limit of later requests (x their initial heap
+        entries; 0 = none, the reference's unbounded behaviour)."""
+        check(lib().mscx_server_set_work_limit(self._h, float(factor)))
 
     def submit(self, data, lx=0, ly=0, threshold=0.0, key=0, discs=True, size=None):
         """data: 2-D float32 array, or None for a resident key (then size=(sx, sy))."""
@@ -424,10 +458,16 @@ class Server:
         check(lib().mscx_server_wait(self._h, ticket, float(timeout), ctypes.byref(g), ctypes.byref(st),
                                      ctypes.byref(lat)))
         err = lib().mscx_server_error(self._h, ticket).decode()
-        return st.value, err, lat.value, (_GridView(g, self) if st.value == 0 and g.value else None)
+        view = None
+        if st.value == 0 and g.value:
+            view = _GridView(g, self)
+            self._views.setdefault(ticket, []).append(view)
+        return st.value, err, lat.value, view
 
     def release(self, ticket):
         check(lib().mscx_server_release(self._h, ticket))
+        for v in self._views.pop(ticket, []):
+            v._h = None
 
     def stats(self):
         o = np.zeros(8, np.uint64)
@@ -487,7 +527,7 @@ class GridRun:
     def piece(self, i):
         h = ctypes.c_void_p()
         check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
-        return GridMSComplex(h, owned=False)
+        return GridMSComplex(h, owned=False, owner=self)
 
     def times(self):
         t = np.zeros(5, np.float64)

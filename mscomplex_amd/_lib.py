@@ -69,6 +69,7 @@ _SIGS = {
     "mscx_batch_error": (ctypes.c_char_p, [P, c_int32]),
     "mscx_batch_destroy": (c_int, [P]),
     "mscx_server_create": (c_int, [c_size_t, c_int32, ctypes.POINTER(P)]),
+    "mscx_server_set_work_limit": (c_int, [P, c_double]),
     "mscx_server_submit": (c_int, [P, P, c_int32, c_int32, c_int32, c_int32, c_double, c_uint64, c_uint32,
                                    ctypes.POINTER(c_uint64)]),
     "mscx_server_wait": (c_int, [P, c_uint64, c_double, ctypes.POINTER(P), ctypes.POINTER(c_int32),

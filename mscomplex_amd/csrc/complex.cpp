@@ -209,6 +209,16 @@ static constexpr uint64_t kMaxArcProduct = 1ull << 20;
 // distinct arcs with large multiplicities) after un_simplify.
 static constexpr int64_t kArcEntryBudget = (int64_t)1 << 28;
 
+// Arc-entry budget of one simplify / un_simplify call on m under a work
+// limit (see simplify in complex.hpp).
+static int64_t arc_entry_budget(const Complex &m, double work_limit) {
+  if (!(work_limit > 0)) return kArcEntryBudget;
+  size_t arcs = 0;
+  for (uint32_t i = 0; i < m.size(); ++i) arcs += m.asc[i].size() + m.des[i].size();
+  const double b = std::max((double)(1 << 24), 2.0 * work_limit * (double)arcs);
+  return (int64_t)std::min(b, (double)kArcEntryBudget);
+}
+
 // cancelPairs (discreteMorseAlgorithm.h:348-464 / :466-585): cp0 is made
 // the lower-dimensional cell; every asc(cp0) x des(cp1) pair is connected
 // (duplicates kept, count >= 2 marks a strangulation), cp0/cp1 are removed
@@ -636,7 +646,7 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
 // equal persistences leave in the reference's order.
 
 // simplify (grid_mscomplex.cpp:384-512)
-void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh) {
+void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh, double work_limit) {
   typedef std::pair<uint32_t, uint32_t> P2;
   const float *fn = m.fn.data();
   std::vector<P2> slots;  // (first, second) of every pushed element
@@ -649,7 +659,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   };
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
-  m.arc_budget = kArcEntryBudget;
+  m.arc_budget = arc_entry_budget(m, work_limit);
+  m.pop_budget = work_limit > 0 ? (int64_t)std::max((double)(1 << 24), work_limit * (double)narcs) : INT64_MAX;
   ExactHeap pq;
   pq.reserve(narcs * 2 + 16);
   slots.reserve(narcs * 4 + 16);
@@ -685,6 +696,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
     uint64_t hp = pq.top();
     const P2 cp = slots[ExactHeap::payload(hp)];
     pq.pop();
+    if (--m.pop_budget < 0)
+      throw Error(MSCX_ELIMIT, "simplification work limit reached (strangulation resubmissions at this threshold)");
     // the next element's pair is a random access into a large array
     if (!pq.empty()) __builtin_prefetch(&slots[ExactHeap::payload(pq.top())]);
     // the key is the persistence |fn[v2] - fn[v1]| computed at the push
@@ -722,8 +735,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   }
 }
 
-void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc) {
-  m.arc_budget = kArcEntryBudget;
+void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc, double work_limit) {
+  m.arc_budget = arc_entry_budget(m, work_limit);
   for (auto it = canc.rbegin(); it != canc.rend(); ++it) uncancel_pairs(m, it->first, it->second);
   m.arc_budget = INT64_MAX;
 }

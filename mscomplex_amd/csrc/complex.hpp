@@ -207,6 +207,9 @@ struct Complex {
   // Arc insertions cancel_pairs may still make before failing (set by
   // simplify; see cancel_pairs).
   int64_t arc_budget = INT64_MAX;
+  // Heap pops simplify may still make before failing with MSCX_ELIMIT
+  // (bounded only under a work limit, see simplify).
+  int64_t pop_budget = INT64_MAX;
 
   uint32_t size() const { return (uint32_t)cell.size(); }
   // capacity for n nodes in every per-node array
@@ -233,8 +236,13 @@ void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> 
 void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2);
 void merge_up(const Complex &a, const Complex &b, Complex &out);
 void merge_down(Complex &parent, Complex &a, Complex &b);
-void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh);
-void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc);
+// work_limit > 0 bounds the work of one call in proportion to the complex
+// (no counterpart in the reference, which runs until done): simplify fails
+// with MSCX_ELIMIT after max(2^24, work_limit x initial heap entries) pops,
+// and either call with MSCX_ENOMEM after max(2^24, 2 x work_limit x initial
+// arc entries) inserted arc entries (at most the fixed 2^28 cap).
+void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh, double work_limit = 0);
+void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &canc, double work_limit = 0);
 
 // GridDataset::postMergeFillDiscs from a tile's codes and owner image (host
 // copies over ext, x fastest).

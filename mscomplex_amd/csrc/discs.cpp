@@ -17,8 +17,9 @@
 
 namespace mscx {
 
-// Staging of one call: host codes + labels and device descriptors, tables
-// and labels (disc), host staging of descriptors and tables (stage).  Calls
+// Staging of one call: host codes + labels (disc), host staging of
+// descriptors and tables (stage); the device descriptors, tables and labels
+// are a block of the device pool for the duration of the call.  Calls
 // from different threads (grids finishing together in a batch or on the
 // serving loop's workers) each lease their own set, so their host work runs
 // concurrently; a set is kept for reuse once returned, so steady-state calls
@@ -226,7 +227,18 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   const size_t o_tab = o_err + 256;
   const size_t o_lab = o_tab + tab_off[n];
   const size_t dev_bytes = o_lab + lab_total;
-  char *dev = (char *)g_disc.reserve_dev(dev_bytes);
+  // device tables and labels come from the library's pool (cached between
+  // calls, counted against a server's HBM budget while in use); the block
+  // goes back once the auxiliary stream has finished with it
+  struct PoolBlock {
+    void *p;
+    hipStream_t s;
+    ~PoolBlock() {
+      (void)hipStreamSynchronize(s);
+      device_pool().free(p);
+    }
+  } devblk{device_pool().alloc(dev_bytes), s};
+  char *dev = (char *)devblk.p;
   char *stage = (char *)g_stage.reserve_host(o_lab);
   DiscDesc *hd = (DiscDesc *)(stage + o_desc);
   {

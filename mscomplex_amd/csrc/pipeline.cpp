@@ -243,8 +243,8 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
   const double t2 = now_s();
   if (G->threshold > 0.0) {
     std::vector<std::pair<uint32_t, uint32_t>> canc;
-    simplify(G->pieces.back()->c, canc, G->threshold);
-    un_simplify(G->pieces.back()->c, canc);
+    simplify(G->pieces.back()->c, canc, G->threshold, G->work_limit);
+    un_simplify(G->pieces.back()->c, canc, G->work_limit);
   }
   const double t3 = now_s();
   G->t[2] += t3 - t2;

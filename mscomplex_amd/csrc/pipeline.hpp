@@ -27,6 +27,7 @@ struct mscx_grid {
   int32_t sx = 0, sy = 0;
   double threshold = 0;
   uint32_t flags = 0;
+  double work_limit = 0;  // simplify / un_simplify work limit (0: none; served requests set it)
   std::vector<mscx::LeafGeom> leaves;
   std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
   std::vector<mscx_tile *> tiles;

@@ -72,10 +72,15 @@ using namespace mscx;
 
 struct mscx_server {
   size_t budget = 0;
+  // work limit of every request's simplification (complex.hpp, simplify):
+  // a request whose cancellations would run far beyond its size fails with
+  // MSCX_ELIMIT (or MSCX_ENOMEM) in seconds instead of holding a worker
+  double work_limit = 128.0;
   int nworkers = 1;
   std::mutex mu;
   std::condition_variable cv_sched, cv_work, cv_done;
-  bool stopping = false;
+  bool stopping = false;       // no more admissions (set by destroy)
+  bool workers_stop = false;   // workers leave once the task queue is empty
   uint64_t next_ticket = 1;
   std::map<uint64_t, std::unique_ptr<Request>> reqs;
   std::deque<uint64_t> queue;                        // admission order
@@ -93,18 +98,24 @@ struct mscx_server {
   }
 
   // Drops least recently used idle residents until `need` more bytes fit.
-  // Called with mu held.
+  // First only entries whose tiles nothing else holds (dropping them frees
+  // their HBM now); then, if that is not enough, entries that results not
+  // yet released still hold through their keep pointer (their HBM is freed
+  // when the client releases those results, which a client waiting for this
+  // request may only do once it has been admitted).  Called with mu held.
   void make_room(size_t need) {
-    for (auto it = lru.begin(); it != lru.end() && device_pool().in_use() + reserved + need > budget;) {
-      auto r = resident.find(*it);
-      if (r != resident.end() && r->second->ready && r->second->users == 0) {
-        resident.erase(r);  // tiles are freed once no result references them
-        it = lru.erase(it);
-        ++n_evicted;
-      } else {
-        ++it;
+    for (int pass = 0; pass < 2; ++pass)
+      for (auto it = lru.begin(); it != lru.end() && device_pool().in_use() + reserved + need > budget;) {
+        auto r = resident.find(*it);
+        if (r != resident.end() && r->second->ready && r->second->users == 0 &&
+            (pass == 1 || r->second.use_count() == 1)) {
+          resident.erase(r);
+          it = lru.erase(it);
+          ++n_evicted;
+        } else {
+          ++it;
+        }
       }
-    }
   }
 
   void finish(Request &R, int status, const std::string &err) {
@@ -130,6 +141,7 @@ struct mscx_server {
       G->owns_tiles = false;
       G->keep = res;
       G->threshold = R.threshold;
+      G->work_limit = work_limit;
       G->flags = R.flags;
       copy_pieces(&B, G.get());
       grid_simplify_down(G.get(), large_grid(G.get()));
@@ -165,8 +177,8 @@ struct mscx_server {
       std::function<void()> f;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stopping || !work.empty(); });
-        if (work.empty()) return;
+        cv_work.wait(lk, [&] { return workers_stop || !work.empty(); });
+        if (work.empty()) return;  // workers_stop: every posted task has been taken
         f = std::move(work.front());
         work.pop_front();
       }
@@ -262,6 +274,7 @@ struct mscx_server {
       std::unique_ptr<mscx_grid> G(new mscx_grid);
       try {
         prepare_grid(G.get(), R->data.data(), R->sx, R->sy, 0, R->lx, R->ly, R->threshold, R->flags);
+        G->work_limit = work_limit;
         R->result = std::move(G);
         gs.push_back(R->result.get());
         live.push_back(R);
@@ -402,6 +415,14 @@ int mscx_server_create(size_t hbm_budget, int32_t host_threads_n, mscx_server **
   });
 }
 
+int mscx_server_set_work_limit(mscx_server *s, double factor) {
+  return guarded([&] {
+    MSCX_REQUIRE(s && factor >= 0 && !std::isnan(factor), MSCX_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->work_limit = factor;
+  });
+}
+
 int mscx_server_submit(mscx_server *s, const float *data, int32_t sx, int32_t sy, int32_t lx, int32_t ly,
                        double threshold, uint64_t key, uint32_t flags, uint64_t *ticket) {
   return guarded([&] {
@@ -501,13 +522,30 @@ int mscx_server_destroy(mscx_server *s) {
   return guarded([&] {
     if (!s) return;
     {
+      // requests still queued never run: they fail with MSCX_ESTATE
       std::lock_guard<std::mutex> lk(s->mu);
       s->stopping = true;
+      for (uint64_t t : s->queue) {
+        Request &R = *s->reqs[t];
+        R.state = RQ_DONE;
+        R.status = MSCX_ESTATE;
+        R.error = "the server was destroyed before the request ran";
+        R.t_done = now_s();
+        ++s->n_done;
+      }
+      s->queue.clear();
+      s->cv_done.notify_all();
     }
     s->cv_sched.notify_all();
-    s->cv_work.notify_all();
+    // the scheduler finishes its current round (posting that round's host
+    // phases) and leaves; only then are the workers told to stop, so every
+    // admitted request's tasks -- and the repeats they release -- still run
     if (s->sched.joinable()) s->sched.join();
-    // workers drain the remaining host-phase tasks before leaving
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->workers_stop = true;
+    }
+    s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
     s->reqs.clear();
     s->resident.clear();

@@ -83,14 +83,16 @@ def test_bench_workload_4096_4x4():
 
 
 @pytest.mark.timeout(600)
+@pytest.mark.parametrize("threshold", [0.0, 0.005])
 @pytest.mark.parametrize("levels", [0, 256])
-def test_worst_case_fields_1024_8x8(levels):
-    """The worst-case configuration's fields and tiling at a size the oracle
-    finishes in seconds: white noise U[0,1) (dense critical cells) and its
-    256-level quantisation (ties on plateaus), 8x8 tiles, simplification at
-    0.5% of the range; every piece word for word after the discs."""
+def test_worst_case_fields_1024_8x8(levels, threshold):
+    """The worst-case configuration's fields, tiling and both thresholds at a
+    size the oracle finishes in seconds: white noise U[0,1) (dense critical
+    cells) and its 256-level quantisation (ties on plateaus), 8x8 tiles,
+    simplification at 0 and 0.5% of the range; every piece word for word
+    after the discs."""
     g = mx.gen_white(1024, 1024, levels=levels, seed=77)
-    _compare(g, f"white{levels}-1024-8x8", lx=3, ly=3, threshold=0.005, threads=min(16, os.cpu_count() or 1))
+    _compare(g, f"white{levels}-1024-8x8", lx=3, ly=3, threshold=threshold, threads=min(16, os.cpu_count() or 1))
 
 
 from tests.test_complex_cpu import DEGENERATE, TOO_FINE, oracle_outcome  # noqa: E402

@@ -188,3 +188,65 @@ def test_concurrent_clients():
         _same(view, _plain(g, lx, ly, th), f"client request {k}")
         srv.release(t)
     srv.close()
+
+
+def test_destroy_with_requests_queued():
+    """Destroying the server while requests wait for memory returns: the
+    admitted ones finish, the queued ones are failed, nothing hangs."""
+    grids = [_grid("v", 300, 300, 50 + i) for i in range(6)]
+    one = mx.GridRun(grids[0], levels=0)
+    del one
+    # a budget that admits about one grid at a time
+    srv = mx.Server(24 * MB, host_threads=2)
+    for g in grids:
+        srv.submit(g, 0, 0, 0.01)
+    done = threading.Event()
+
+    def close():
+        srv.close()
+        done.set()
+
+    th = threading.Thread(target=close, daemon=True)
+    th.start()
+    assert done.wait(120), "mscx_server_destroy did not return"
+
+
+def test_borrowed_views_keep_owner_and_expire():
+    g = _grid("v", 120, 100, 3)
+    ref = mx.GridRun(g, levels=0, lx=1, ly=1, threshold=0.01)
+    # a piece of a temporary run stays valid: it keeps the run alive
+    d = mx.GridRun(g, levels=0, lx=1, ly=1, threshold=0.01).piece(0).dump()
+    assert np.array_equal(d, ref.piece(0).dump())
+    srv = mx.Server(1 << 30, host_threads=2)
+    t = srv.submit(g, 1, 1, 0.01)
+    st, err, _, view = srv.wait(t, timeout=120)
+    assert st == 0, err
+    piece = view.piece(0)
+    assert np.array_equal(piece.dump(), ref.piece(0).dump())
+    srv.release(t)
+    with pytest.raises(mx.MscxError):
+        view.piece(0)
+    with pytest.raises(mx.MscxError):
+        piece.dump()
+    srv.close()
+
+
+def test_dense_request_fails_fast():
+    """A request whose simplification would grow arc multisets without bound
+    (white noise at 10%) fails quickly under the server's work limit, and
+    the other requests of the mix are unaffected."""
+    from tests.helpers import rng_field
+    bad = rng_field("white", 150, 130, 5)
+    good = _grid("v", 140, 120, 8)
+    srv = mx.Server(1 << 30, host_threads=2)
+    t_bad = srv.submit(bad, 0, 0, 0.1)
+    t_good = srv.submit(good, 1, 1, 0.02)
+    st, err, lat, _ = srv.wait(t_bad, timeout=120)
+    assert st in (-4, -7), (st, err)
+    assert lat < 5.0, lat
+    st, err, _, view = srv.wait(t_good, timeout=120)
+    assert st == 0, err
+    _same(view, _plain(good, 1, 1, 0.02), "request beside a failing one")
+    srv.release(t_bad)
+    srv.release(t_good)
+    srv.close()
