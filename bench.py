@@ -173,6 +173,43 @@ def worstcase_bench(reps=3):
     return out
 
 
+def worst_e2e_child(side, levels, threshold):
+    """One worst-case grid end to end (child process, so that its peak host
+    memory is its own): prints one JSON line."""
+    import resource
+    import mscomplex_amd as mx
+    data = mx.gen_white(side, side, levels=levels, seed=4242 + levels)
+    buf = mx.DeviceBuffer.from_array(data)
+    del data
+    t0 = time.perf_counter()
+    run = mx.GridRun(None, lx=3, ly=3, threshold=threshold, device_ptr=buf.ptr, size=(side, side), out_of_core=True)
+    el = time.perf_counter() - t0
+    n = run.num_pieces()
+    print(json.dumps({"side": side, "levels": levels, "threshold": threshold, "out_of_core": True,
+                      "end_to_end_s": round(el, 2), "Mcells_per_s": round(side * side / 1e6 / el, 2),
+                      "phase_s": dict(zip(("compute", "merge_up", "simplify", "merge_down", "discs"),
+                                          [round(v, 2) for v in run.times()])),
+                      "root_critical_points": run.piece(n - 1).num_cps(),
+                      "peak_host_rss_GB": round(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6, 1)}),
+          flush=True)
+
+
+def worst_e2e_bench(side=16384, threshold=0.0):
+    """The worst-case configuration end to end at full size: white noise and
+    its 256-level quantisation, 8x8 tiles, out-of-core merge tree, each in a
+    child process."""
+    import subprocess
+    out = {}
+    for levels in (0, 256):
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--worst-e2e-child", str(side), str(levels),
+                            str(threshold)], capture_output=True, text=True, timeout=1500)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        out["white" if levels == 0 else f"quantized{levels}"] = json.loads(line[-1]) if line else \
+            {"error": r.stderr[-2000:], "returncode": r.returncode}
+        print(f"[bench] worst case e2e levels={levels}: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_bench(n=1024, seed=7):
     """The batched configuration: n grids with long sides log-uniform in
     [64, 2048], aspect 1:1-1:4, half Gaussian sums and half value noise,
@@ -341,10 +378,17 @@ def main():
     ap.add_argument("--no-worst", action="store_true")
     ap.add_argument("--served-max-side", type=int, default=4096)
     ap.add_argument("--batched-grids", type=int, default=1024)
+    ap.add_argument("--worst-e2e", action="store_true",
+                    help="also run the 16384^2 worst case end to end (two grids, a few minutes, ~145 GB of host "
+                         "memory each; out-of-core merge tree)")
+    ap.add_argument("--worst-e2e-child", nargs=3, metavar=("SIDE", "LEVELS", "THRESHOLD"), help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the calibration kernel for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.worst_e2e_child:
+        worst_e2e_child(int(args.worst_e2e_child[0]), int(args.worst_e2e_child[1]), float(args.worst_e2e_child[2]))
+        return
     if args.copy_probe:
         copy_probe()
         return
@@ -474,6 +518,8 @@ def main():
     del last
     if not args.no_worst:
         line["worst_case"] = worstcase_bench()
+    if args.worst_e2e:
+        line["worst_case_end_to_end"] = worst_e2e_bench()
     if not args.no_batched:
         line["batched"] = batched_bench(n=args.batched_grids)
     if not args.no_served:

@@ -207,6 +207,15 @@ int mscx_complex_destroy(mscx_complex *c);
  * MSCX_RUN_DEVICE_INPUT is set, then it is a device pointer. */
 #define MSCX_RUN_DEVICE_INPUT 1
 #define MSCX_RUN_NO_DISCS 2
+/* The reference's --out-of-core (grid_datamanager.cpp:91-119,134-210): a
+ * piece that no phase is using is kept serialised -- as its archive words in
+ * host memory, or as an archive file under the directory named by the
+ * MSCX_SPILL_DIR environment variable -- and brought back when a phase or
+ * mscx_grid_piece needs it, so the merge tree's host memory is bounded by
+ * the pieces one merge level works on rather than by every level at once
+ * (a 16384^2 white-noise grid's tree holds ~228 M critical points per
+ * level).  Results are the same as without the flag. */
+#define MSCX_RUN_OUT_OF_CORE 4
 int mscx_grid_run(const float *data, int32_t size_x, int32_t size_y, int32_t levels, int32_t lx, int32_t ly,
                   double threshold, uint32_t flags, mscx_grid **out);
 /* The whole reference driver over a raw grid file: GridDataManager(filename,

@@ -479,9 +479,10 @@ class Server:
 class GridRun:
     """A whole GridDataManager compute run over one grid (mscx_grid_run)."""
 
-    def __init__(self, data, levels=1, lx=0, ly=0, threshold=0.0, discs=True, device_ptr=None, size=None):
+    def __init__(self, data, levels=1, lx=0, ly=0, threshold=0.0, discs=True, device_ptr=None, size=None,
+                 out_of_core=False):
         h = ctypes.c_void_p()
-        flags = 0 if discs else 2
+        flags = (0 if discs else 2) | (4 if out_of_core else 0)
         if device_ptr is not None:
             sx, sy = size
             flags |= 1
@@ -495,17 +496,18 @@ class GridRun:
 
     @classmethod
     def from_file(cls, path, size, levels=1, lx=0, ly=0, threshold=0.0, num_parallel=0, discs=True,
-                  disc_prefix=None):
+                  disc_prefix=None, out_of_core=False):
         """The reference driver over a raw float32 grid file (x fastest) of
         size (sx, sy) (mscx_grid_run_file): GridDataManager(filename, sx, sy,
         levels, ..., simp_tresh, ..., num_parallel).  disc_prefix, when
         given, receives every leaf's disc files (the reference driver uses
-        "dp__disc_" in its working directory)."""
+        "dp__disc_" in its working directory).  out_of_core keeps idle pieces
+        serialised (the reference's --out-of-core)."""
         sx, sy = size
         h = ctypes.c_void_p()
         pre = None if disc_prefix is None else os.fsencode(disc_prefix)
         check(lib().mscx_grid_run_file(os.fsencode(path), sx, sy, levels, lx, ly, float(threshold), int(num_parallel),
-                                       0 if discs else 2, pre, ctypes.byref(h)))
+                                       (0 if discs else 2) | (4 if out_of_core else 0), pre, ctypes.byref(h)))
         self = cls.__new__(cls)
         self._h = h
         return self

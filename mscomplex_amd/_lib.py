@@ -8,7 +8,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmscx.so")
+# MSCX_LIB_PATH selects another build of the same library (the CPU
+# sanitizer builds of build.py --sanitize)
+LIB_PATH = os.environ.get("MSCX_LIB_PATH") or os.path.join(_HERE, "libmscx.so")
 
 _lib = None
 

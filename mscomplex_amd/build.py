@@ -85,6 +85,59 @@ def build(verbose=False):
     return LIB
 
 
+SANITIZERS = {
+    # host code instrumented; device code is built as usual (GPU sanitizers
+    # are not part of this build)
+    "address": ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"],
+    "thread": ["-fsanitize=thread"],
+}
+
+
+def build_sanitized(kind, verbose=False):
+    """A CPU-sanitizer build of the library in mscomplex_amd/_san_<kind>/
+    (host sources through g++ with AddressSanitizer + UndefinedBehavior-
+    Sanitizer, or ThreadSanitizer).  Load it with MSCX_LIB_PATH set to the
+    returned path and the sanitizer runtime preloaded, e.g.
+
+      MSCX_LIB_PATH=$(python -m mscomplex_amd.build --sanitize address) \
+      LD_PRELOAD=$(gcc -print-file-name=libasan.so):$(gcc -print-file-name=libubsan.so) \
+      ASAN_OPTIONS=detect_leaks=0 python -m pytest tests/test_complex_cpu.py
+    """
+    flags = SANITIZERS[kind]
+    out_dir = os.path.join(HERE, f"_san_{kind}")
+    os.makedirs(out_dir, exist_ok=True)
+
+    def comp(src, device):
+        out = os.path.join(out_dir, os.path.basename(src) + ".o")
+        if device:
+            cmd = [_hipcc(), *HIP_FLAGS, "-c", src, "-o", out]
+        else:
+            cmd = ["g++", "-O1", "-g", "-fno-omit-frame-pointer", "-fPIC", "-std=c++17", "-Wall",
+                   "-Wno-unused-result", *flags, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return out
+
+    jobs = [(os.path.join(CSRC, x), True) for x in DEVICE_SOURCES]
+    jobs += [(os.path.join(CSRC, x), False) for x in HOST_SOURCES]
+    with _cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(lambda j: comp(*j), jobs))
+    lib = os.path.join(out_dir, "libmscx.so")
+    cmd = ["g++", "-shared", *flags, "-o", lib + ".tmp", *objs, "-L/opt/rocm/lib", "-lamdhip64", "-lpthread",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(lib + ".tmp", lib)
+    if verbose:
+        print(lib)
+    return lib
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if len(sys.argv) == 3 and sys.argv[1] == "--sanitize":
+        build_sanitized(sys.argv[2], verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)

@@ -993,34 +993,14 @@ void save_complex(const Complex &m, const std::string &path) {
   if (!os.good()) throw Error(MSCX_EIO, "cannot write complex archive: " + path);
 }
 
-void load_complex(const std::string &path, Complex &m) {
-  std::ifstream is(path, std::ios::in | std::ios::binary);
-  if (!is.is_open()) throw Error(MSCX_EIO, "cannot read complex archive: " + path);
-  char magic[8];
-  uint32_t hdr[2];
-  uint64_t n = 0;
-  is.read(magic, 8);
-  is.read((char *)hdr, sizeof hdr);
-  is.read((char *)&n, 8);
-  if (!is.good() || memcmp(magic, kArchiveMagic, 8) != 0) throw Error(MSCX_EIO, "not a complex archive: " + path);
-  if (hdr[0] != kArchiveVersion) throw Error(MSCX_EIO, "unsupported complex archive version: " + path);
-  is.seekg(0, std::ios::end);
-  const uint64_t bytes = (uint64_t)is.tellg();
-  if (n < 9 || bytes != 24 + 4 * n + 8) throw Error(MSCX_EIO, "truncated complex archive: " + path);
-  is.seekg(24);
-  std::vector<uint32_t> w((size_t)n);
-  uint64_t sum = 0;
-  is.read((char *)w.data(), (std::streamsize)(4 * n));
-  is.read((char *)&sum, 8);
-  if (!is.good() || sum != fnv1a(w.data(), w.size())) throw Error(MSCX_EIO, "corrupt complex archive: " + path);
-
-  // parse the dump words (the checksum passed, so a malformed body means a
-  // writer bug; still, every length is checked against the words left)
-  auto bad = [&]() { return Error(MSCX_EIO, "corrupt complex archive: " + path); };
+// A complex from its dump words (Complex::dump); every length is checked
+// against the words left.  `what` names the source in the error.
+void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::string &what) {
+  auto bad = [&]() { return Error(MSCX_EIO, "corrupt complex archive: " + what); };
   size_t at = 0;
   auto take = [&](size_t k) {
-    if (w.size() - at < k) throw bad();
-    const uint32_t *p = w.data() + at;
+    if (n - at < k) throw bad();
+    const uint32_t *p = w + at;
     at += k;
     return p;
   };
@@ -1029,7 +1009,8 @@ void load_complex(const std::string &path, Complex &m) {
   Complex c;
   c.rect = Rect{(int)h[1], (int)h[2], (int)h[3], (int)h[4]};
   c.ext = Rect{(int)h[5], (int)h[6], (int)h[7], (int)h[8]};
-  if ((uint64_t)ncp * 8 > w.size()) throw bad();
+  if ((uint64_t)ncp * 8 > n) throw bad();
+  c.reserve_nodes(Complex::with_headroom(ncp));
   c.cell.resize(ncp);
   c.fn.resize(ncp);
   c.pair.resize(ncp);
@@ -1053,18 +1034,41 @@ void load_complex(const std::string &path, Complex &m) {
       ConnSet &dst = k == 0 ? c.asc[i] : c.des[i];
       for (uint32_t e = 0; e < cnt; ++e) {
         if (v[e] >= ncp || (e && v[e] < v[e - 1])) throw bad();
-        dst.insert(v[e]);
+        dst.push_sorted(v[e]);
       }
     }
     const uint32_t *ad = take(nad);
-    c.asc_disc[i].assign(ad, ad + nad);
+    if (nad) c.asc_disc[i].assign(ad, ad + nad);
     const uint32_t *dd = take(ndd);
-    c.des_disc[i].assign(dd, dd + ndd);
-    if (c.idx.contains(c.cell[i])) throw bad();
-    c.idx.set(c.cell[i], i);
+    if (ndd) c.des_disc[i].assign(dd, dd + ndd);
+    if (c.idx.find_or_insert(c.cell[i], i) != UINT32_MAX) throw bad();
   }
-  if (at != w.size()) throw bad();
+  if (at != n) throw bad();
   m = std::move(c);
+}
+
+void load_complex(const std::string &path, Complex &m) {
+  std::ifstream is(path, std::ios::in | std::ios::binary);
+  if (!is.is_open()) throw Error(MSCX_EIO, "cannot read complex archive: " + path);
+  char magic[8];
+  uint32_t hdr[2];
+  uint64_t n = 0;
+  is.read(magic, 8);
+  is.read((char *)hdr, sizeof hdr);
+  is.read((char *)&n, 8);
+  if (!is.good() || memcmp(magic, kArchiveMagic, 8) != 0) throw Error(MSCX_EIO, "not a complex archive: " + path);
+  if (hdr[0] != kArchiveVersion) throw Error(MSCX_EIO, "unsupported complex archive version: " + path);
+  is.seekg(0, std::ios::end);
+  const uint64_t bytes = (uint64_t)is.tellg();
+  if (n < 9 || bytes != 24 + 4 * n + 8) throw Error(MSCX_EIO, "truncated complex archive: " + path);
+  is.seekg(24);
+  std::vector<uint32_t> w((size_t)n);
+  uint64_t sum = 0;
+  is.read((char *)w.data(), (std::streamsize)(4 * n));
+  is.read((char *)&sum, 8);
+  if (!is.good() || sum != fnv1a(w.data(), w.size())) throw Error(MSCX_EIO, "corrupt complex archive: " + path);
+
+  complex_from_words(w.data(), w.size(), m, path);
 }
 
 }  // namespace mscx

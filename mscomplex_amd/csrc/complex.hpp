@@ -277,6 +277,9 @@ void write_discs(const Complex &m, const std::string &prefix);
 // on a missing, truncated or corrupt file.
 void save_complex(const Complex &m, const std::string &path);
 void load_complex(const std::string &path, Complex &m);
+// A complex from its dump words (the body of an archive); MSCX_EIO when they
+// are malformed (`what` names their source in the message).
+void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::string &what);
 
 }  // namespace mscx
 

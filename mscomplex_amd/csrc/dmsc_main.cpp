@@ -11,9 +11,10 @@
 // -st forces one host thread.  Every piece's discs are appended to
 // dp__disc_asc_(x,y) / dp__disc_des_(x,y) in the working directory, as
 // collectManifold writes them (grid_datamanager.cpp:397-420).  The GPU path
-// always runs, so -ocl is accepted and changes nothing; --out-of-core is
-// accepted and changes nothing because the whole grid (at most 1 GiB) stays
-// resident in HBM.  The reference matches its flags with regular
+// always runs, so -ocl is accepted and changes nothing; --out-of-core keeps
+// the pieces of the merge tree that no phase is using serialised
+// (MSCX_RUN_OUT_OF_CORE; archive files under $MSCX_SPILL_DIR when it is
+// set), as the reference spills them to archive files.  The reference matches its flags with regular
 // expressions over the joined command line and ignores what does not
 // match; here each flag takes its value from the next argument(s), a file
 // name is taken whole, and an unknown argument or a malformed value is a
@@ -117,14 +118,16 @@ int main(int argc, char **argv) {
   }
   if (a.print) {
     printf("{\"file\": \"%s\", \"size\": [%ld, %ld], \"levels\": %ld, \"lx\": %ld, \"ly\": %ld, "
-           "\"simp_tresh\": %.17g, \"num_parallel\": %ld, \"single_threaded\": %s, \"write_discs\": %s}\n",
+           "\"simp_tresh\": %.17g, \"num_parallel\": %ld, \"single_threaded\": %s, \"write_discs\": %s, "
+           "\"out_of_core\": %s}\n",
            a.file.c_str(), a.sx, a.sy, a.levels, a.lx, a.ly, a.thresh, a.num_parallel,
-           a.num_parallel == 1 ? "true" : "false", a.write ? "true" : "false");
+           a.num_parallel == 1 ? "true" : "false", a.write ? "true" : "false", a.out_of_core ? "true" : "false");
     return 0;
   }
   mscx_grid *g = nullptr;
   int rc = mscx_grid_run_file(a.file.c_str(), (int32_t)a.sx, (int32_t)a.sy, (int32_t)a.levels, (int32_t)a.lx,
-                              (int32_t)a.ly, a.thresh, (int32_t)a.num_parallel, 0, a.write ? "dp__disc_" : nullptr, &g);
+                              (int32_t)a.ly, a.thresh, (int32_t)a.num_parallel, a.out_of_core ? MSCX_RUN_OUT_OF_CORE : 0u,
+                              a.write ? "dp__disc_" : nullptr, &g);
   if (rc) {
     fprintf(stderr, "mscx_dmsc: error %d: %s\n", rc, mscx_last_error());
     return 1;

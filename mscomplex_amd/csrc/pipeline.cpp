@@ -7,6 +7,11 @@
 // also driven by the serving loop (server.cpp).
 #include "pipeline.hpp"
 
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdlib>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -131,6 +136,7 @@ void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32
   G->sy = sy;
   G->threshold = threshold;
   G->flags = flags;
+  G->ooc = (flags & MSCX_RUN_OUT_OF_CORE) != 0;
   const float *dev = data;
   if (!(flags & MSCX_RUN_DEVICE_INPUT)) {
     size_t n = (size_t)sx * sy;
@@ -212,19 +218,25 @@ void grids_compute(std::vector<mscx_grid *> &grids) {
   for (mscx_grid *G : grids) G->t[0] += t1 - t0;
 }
 
+static void init_spill(mscx_grid *G);
+
 // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
 void grid_merge_up(mscx_grid *G, bool inner_parallel) {
   const int nleaf = (int)G->leaves.size();
   const int nt = inner_parallel ? 0 : 1;
   const double t1 = now_s();
   G->pieces.resize(2 * nleaf - 1, nullptr);
+  if (G->ooc) init_spill(G);
   for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
   for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
     parallel_for(
         width / 2,
         [&](int k) {
           int j = lo + 2 * k;
-          merge_up(G->pieces[j]->c, G->pieces[j + 1]->c, G->pieces[nleaf + j / 2]->c);
+          merge_up(restore_piece(G, j)->c, restore_piece(G, j + 1)->c, G->pieces[nleaf + j / 2]->c);
+          // out of core: the children wait on the side until merge down
+          spill_piece(G, j);
+          spill_piece(G, j + 1);
         },
         nt);
   G->t[1] += now_s() - t1;
@@ -256,7 +268,8 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
           [&](int k) {
             int p = level[k];
             int c = 2 * (p - nleaf);
-            merge_down(G->pieces[p]->c, G->pieces[c]->c, G->pieces[c + 1]->c);
+            merge_down(restore_piece(G, p)->c, restore_piece(G, c)->c, restore_piece(G, c + 1)->c);
+            spill_piece(G, p);  // out of core: the parent is final
           },
           nt);
       std::vector<int> next;
@@ -284,12 +297,67 @@ void grids_discs(std::vector<mscx_grid *> &grids) {
     if (G->flags & MSCX_RUN_NO_DISCS) continue;
     for (size_t i = 0; i < G->leaves.size(); ++i) {
       dt.push_back(G->tiles[i]);
-      dc.push_back(&G->pieces[i]->c);
+      dc.push_back(&restore_piece(G, (int)i)->c);
     }
   }
   fill_discs_device(dt.data(), dc.data(), (int)dt.size());
+  for (mscx_grid *G : grids)
+    if (G->ooc && G->leaves.size() > 1)
+      parallel_for((int)G->leaves.size(), [&](int i) { spill_piece(G, i); });
   const double t5 = now_s();
   for (mscx_grid *G : grids) G->t[4] += t5 - t4;
+}
+
+// ---------------------------------------------------------------- out of core
+
+static std::atomic<uint64_t> g_spill_seq{0};
+
+static std::string spill_path(const mscx_grid *G, int i) { return G->spill_dir + "/dp_msgraph_" + std::to_string(i); }
+
+// Sizes the spill store and, with MSCX_SPILL_DIR, makes this grid's
+// directory of archive files there.
+static void init_spill(mscx_grid *G) {
+  const size_t n = 2 * G->leaves.size() - 1;
+  G->spilled.assign(n, std::vector<uint32_t>());
+  G->spilled_file.assign(n, 0);
+  const char *dir = getenv("MSCX_SPILL_DIR");
+  if (dir && *dir) {
+    G->spill_dir = std::string(dir) + "/mscx_spill_" + std::to_string((long)getpid()) + "_" +
+                   std::to_string(g_spill_seq.fetch_add(1));
+    MSCX_REQUIRE(mkdir(G->spill_dir.c_str(), 0700) == 0, MSCX_EIO, "cannot create spill directory " + G->spill_dir);
+  }
+}
+
+void spill_piece(mscx_grid *G, int i) {
+  if (!G->ooc || i < 0 || (size_t)i >= G->spilled.size() || !G->pieces[i]) return;
+  if (!G->spill_dir.empty()) {
+    save_complex(G->pieces[i]->c, spill_path(G, i));
+    G->spilled_file[i] = 1;
+  } else {
+    std::vector<uint32_t> w;
+    G->pieces[i]->c.dump(w);
+    w.shrink_to_fit();
+    G->spilled[i].swap(w);
+  }
+  delete G->pieces[i];
+  G->pieces[i] = nullptr;
+}
+
+mscx_complex *restore_piece(mscx_grid *G, int i) {
+  if (G->pieces[i]) return G->pieces[i];
+  MSCX_REQUIRE(G->ooc && (size_t)i < G->spilled.size(), MSCX_ESTATE, "piece is neither live nor spilled");
+  std::unique_ptr<mscx_complex> c(new mscx_complex);
+  if (G->spilled_file[i]) {
+    load_complex(spill_path(G, i), c->c);
+    unlink(spill_path(G, i).c_str());
+    G->spilled_file[i] = 0;
+  } else {
+    const std::vector<uint32_t> &w = G->spilled[i];
+    complex_from_words(w.data(), w.size(), c->c, "spilled piece " + std::to_string(i));
+    std::vector<uint32_t>().swap(G->spilled[i]);
+  }
+  G->pieces[i] = c.release();
+  return G->pieces[i];
 }
 
 void copy_pieces(const mscx_grid *src, mscx_grid *dst) {
@@ -327,6 +395,11 @@ mscx_grid::~mscx_grid() {
   if (owns_tiles)
     for (mscx_tile *t : tiles) mscx_tile_destroy(t);
   if (dev_input) mscx_dev_free(dev_input);
+  if (!spill_dir.empty()) {
+    for (size_t i = 0; i < spilled_file.size(); ++i)
+      if (spilled_file[i]) unlink((spill_dir + "/dp_msgraph_" + std::to_string(i)).c_str());
+    rmdir(spill_dir.c_str());
+  }
 }
 
 struct mscx_batch {
@@ -485,7 +558,10 @@ extern "C" int mscx_grid_run_file(const char *path, int32_t sx, int32_t sy, int3
     run_grids(gs, nullptr, nullptr);
     if (disc_prefix) {
       const double t5 = now_s();
-      for (size_t i = 0; i < G->leaves.size(); ++i) write_discs(G->pieces[i]->c, disc_prefix);
+      for (size_t i = 0; i < G->leaves.size(); ++i) {
+        write_discs(restore_piece(G.get(), (int)i)->c, disc_prefix);
+        if (G->leaves.size() > 1) spill_piece(G.get(), (int)i);
+      }
       G->t[4] += now_s() - t5;
     }
     *out = G.release();
@@ -502,6 +578,12 @@ extern "C" int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n) {
 extern "C" int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out) {
   return guarded([&] {
     MSCX_REQUIRE(g && out && i >= 0 && i < (int32_t)g->pieces.size(), MSCX_EINVAL, "bad piece index");
+    if (!g->pieces[i]) {
+      // out of core: the piece comes back into memory and stays there
+      mscx_grid *G = const_cast<mscx_grid *>(g);
+      std::lock_guard<std::mutex> lk(G->spill_mu);
+      restore_piece(G, i);
+    }
     *out = g->pieces[i];
   });
 }

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,15 @@ struct mscx_grid {
   double threshold = 0;
   uint32_t flags = 0;
   double work_limit = 0;  // simplify / un_simplify work limit (0: none; served requests set it)
+  // Out-of-core mode (MSCX_RUN_OUT_OF_CORE): a piece that no phase is using
+  // is kept as its dump words (or, with MSCX_SPILL_DIR set, as an archive
+  // file there) instead of a live complex, and brought back when a phase or
+  // a caller needs it; pieces[i] is then NULL.
+  bool ooc = false;
+  std::vector<std::vector<uint32_t>> spilled;
+  std::string spill_dir;
+  std::vector<char> spilled_file;
+  std::mutex spill_mu;  // lazy restores by the accessors
   std::vector<mscx::LeafGeom> leaves;
   std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
   std::vector<mscx_tile *> tiles;
@@ -75,5 +85,11 @@ void stream_grid_file(GridFile &f, int32_t sx, float *dev);
 
 // Deep copy of a grid's pieces (the complexes); tiles are not copied.
 void copy_pieces(const mscx_grid *src, mscx_grid *dst);
+
+// Out-of-core mode: spill_piece stores piece i (when live) and frees it;
+// restore_piece makes it live again and returns it.  Both are no-ops for a
+// piece in its target state; tasks on different pieces may run together.
+void spill_piece(mscx_grid *G, int i);
+mscx_complex *restore_piece(mscx_grid *G, int i);
 
 }  // namespace mscx

@@ -74,13 +74,14 @@ def test_cli_defaults_follow_the_reference():
     s = _settings("-gf", "a.raw", "-d", "64", "32")
     # parse_grid: levels 1, num_parallel 1 (=> single-threaded), no simplification
     assert s == {"file": "a.raw", "size": [64, 32], "levels": 1, "lx": 0, "ly": 0, "simp_tresh": 0.0,
-                 "num_parallel": 1, "single_threaded": True, "write_discs": True}
+                 "num_parallel": 1, "single_threaded": True, "write_discs": True, "out_of_core": False}
 
 
 def test_cli_flags():
     s = _settings("-gf", "dir/a.raw", "-d", "4096", "4096", "-l", "3", "--simp-tresh", "0.01", "--num-parallel", "8",
                   "-ocl", "--out-of-core")
     assert s["levels"] == 3 and s["simp_tresh"] == 0.01 and s["num_parallel"] == 8 and not s["single_threaded"]
+    assert s["out_of_core"]
     # -st wins over --num-parallel, as in GridDataManager's constructor
     s = _settings("-st", "--num-parallel", "8", "-gf", "a.raw", "-d", "8", "8")
     assert s["num_parallel"] == 1 and s["single_threaded"]
