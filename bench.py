@@ -260,19 +260,20 @@ def batched_bench(n=1024, seed=7):
             "autotune_s": round(t_tune, 2), "generate_s": round(t_gen, 2)}
 
 
-def served_bench(n=24, seed=11, repeat_frac=0.25, loads=((1.0, 5), (2.0, 6), (8.0, 7)), max_side=4096):
-    """The served configuration: square grids with sides log-uniform in
-    [128, max_side] (8192 in the full configuration), alternating fields,
-    value-noise thresholds uniform in [0, 5%] of the range (Gaussian sums at
-    0, as in the batched run), a quarter of the requests repeating an
-    earlier grid at a new threshold.  Plain = mscx_grid_run per request, one at a time (no
-    residency: repeats recompute).  Served = Poisson arrivals at each of
-    `loads` (x the plain throughput, with the arrival seed) into the
-    continuous-batching server (200 GB HBM budget, grids resident under their
-    key for the repeats), each run starting with an empty server.  At 8x
-    the requests arrive faster than any form can serve them, so that run's
-    throughput is the server's capacity on this mix (bounded below by the
-    longest single request's time)."""
+def served_bench(n=128, seed=11, repeat_frac=0.25, loads=((1.0, 5), (4.0, 6)), max_side=8192, work_limit=128):
+    """The served configuration: Poisson arrivals of square grids with sides
+    log-uniform in [128, max_side], alternating Gaussian sums (32 Gaussians,
+    +/-1e-3 noise) and value noise, every request at a threshold uniform in
+    [0, 5%] of its range, a quarter of the requests repeating an earlier grid
+    at a new threshold.  Plain = mscx_grid_run per request, one at a time
+    (repeats recompute).  Served = arrivals at each of `loads` (x the plain
+    throughput) into the continuous-batching server (200 GB HBM budget,
+    grids resident under their key for the repeats), each run starting with
+    an empty server.  Both forms run the simplification under the same work
+    limit (x its initial heap entries): a noisy grid at a large threshold,
+    which the reference's exact simplification would grind through for
+    minutes, fails in seconds in both and counts as a failed request; its
+    latency is included in the percentiles of all requests."""
     import threading
     import mscomplex_amd as mx
     r = np.random.default_rng(seed)
@@ -280,30 +281,42 @@ def served_bench(n=24, seed=11, repeat_frac=0.25, loads=((1.0, 5), (2.0, 6), (8.
     reqs = []  # (grid index, threshold)
     for i in range(n):
         if base and r.random() < repeat_frac:
-            gi = int(r.integers(len(base)))
-            reqs.append((gi, 0.0 if gi % 2 else float(r.uniform(0, 0.05))))
+            reqs.append((int(r.integers(len(base))), float(r.uniform(0, 0.05))))
             continue
         side = int(round(np.exp(r.uniform(np.log(128), np.log(max_side)))))
         s = int(r.integers(1 << 62))
         g = mx.gen_gaussians(side, side, 32, 1e-3, seed=s) if len(base) % 2 else \
             mx.gen_value_noise(side, side, seed=s, period=side / 4)
         base.append(g)
-        reqs.append((len(base) - 1, 0.0 if (len(base) - 1) % 2 else float(r.uniform(0, 0.05))))
+        reqs.append((len(base) - 1, float(r.uniform(0, 0.05))))
     tiling = [(0, 0) if max(g.shape) < 1024 else (1, 1) if max(g.shape) < 4096 else (2, 2) for g in base]
+    pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2) if len(v) else None
 
-    plain_lat = []
-    t0 = time.perf_counter()
-    for gi, th in reqs:
-        a = time.perf_counter()
-        lx, ly = tiling[gi]
-        run = mx.GridRun(base[gi], levels=0, lx=lx, ly=ly, threshold=th)
-        del run
-        plain_lat.append(time.perf_counter() - a)
-    t_plain = time.perf_counter() - t0
+    os.environ["MSCX_WORK_LIMIT"] = str(work_limit)
+    plain_lat, plain_ok = [], []
+    try:
+        t0 = time.perf_counter()
+        for k, (gi, th) in enumerate(reqs):
+            a = time.perf_counter()
+            lx, ly = tiling[gi]
+            try:
+                run = mx.GridRun(base[gi], levels=0, lx=lx, ly=ly, threshold=th)
+                del run
+                plain_ok.append(True)
+            except mx.MscxError:
+                plain_ok.append(False)
+            plain_lat.append(time.perf_counter() - a)
+            if k % 16 == 15:
+                print(f"[bench] served: plain {k + 1}/{n} requests, {time.perf_counter() - t0:.1f}s",
+                      file=sys.stderr, flush=True)
+        t_plain = time.perf_counter() - t0
+    finally:
+        del os.environ["MSCX_WORK_LIMIT"]
     print(f"[bench] served: plain {t_plain:.2f}s for {n} requests", file=sys.stderr, flush=True)
-    srv = mx.Server(200 << 30)
 
     def run_at(load, gap_seed):
+        srv = mx.Server(200 << 30)
+        srv.set_work_limit(work_limit)
         rate = load * n / t_plain
         gaps = np.random.default_rng(gap_seed).exponential(1.0 / rate, size=n)
         tickets = [None] * n
@@ -323,35 +336,39 @@ def served_bench(n=24, seed=11, repeat_frac=0.25, loads=((1.0, 5), (2.0, 6), (8.
         th_ = threading.Thread(target=client)
         th_.start()
         th_.join()
-        lat = []
-        failed = 0
-        for t in tickets:
-            st, err, l, _ = srv.wait(t, timeout=600)
-            failed += st != 0
+        lat, ok_lat, codes = [], [], {}
+        for k, t in enumerate(tickets):
+            st, err, l, _ = srv.wait(t, timeout=900)
             lat.append(l)
+            if st == 0:
+                ok_lat.append(l)
+            else:
+                codes[str(st)] = codes.get(str(st), 0) + 1
             srv.release(t)
+            if k % 32 == 31:
+                print(f"[bench] served: load {load}x, {k + 1}/{n} collected", file=sys.stderr, flush=True)
         t_served = time.perf_counter() - t0
         s1 = srv.stats()
+        srv.close()
         print(f"[bench] served: load {load}x plain, {t_served:.2f}s", file=sys.stderr, flush=True)
         return {"load_x_plain": load, "offered_rate_per_s": round(rate, 3),
-                "throughput_per_s": round(n / t_served, 3),
-                "latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
-                "rounds": s1["rounds"] - s0["rounds"], "repeats_served_resident": s1["repeats"] - s0["repeats"],
-                "failed": int(failed)}
+                "throughput_per_s": round(n / t_served, 3), "x_plain": round(t_plain / t_served, 2),
+                "latency_ms_all": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
+                "latency_ms_completed": {"p50": pct(ok_lat, 50), "p95": pct(ok_lat, 95), "p99": pct(ok_lat, 99)},
+                "completed": len(ok_lat), "failed_by_status": codes,
+                "rounds": s1["rounds"] - s0["rounds"], "repeats_served_resident": s1["repeats"] - s0["repeats"]}
 
-    pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2)
-    # the residents of the first run are evicted so that both runs start cold
-    runs = []
-    for load, gs in loads:
-        runs.append(run_at(load, gs))
-        srv.close()
-        srv = mx.Server(200 << 30)
-    srv.close()
-    return {"requests": n, "distinct_grids": len(base), "repeats": n - len(base),
+    runs = [run_at(load, gs) for load, gs in loads]
+    ok_lat = [l for l, ok in zip(plain_lat, plain_ok) if ok]
+    return {"requests": n, "distinct_grids": len(base), "repeats": n - len(base), "max_side": max_side,
+            "sides": {"min": int(min(g.shape[0] for g in base)), "max": int(max(g.shape[0] for g in base))},
+            "thresholds": "uniform [0, 0.05] for every request", "work_limit": work_limit,
             "vertices_mean": int(np.mean([base[gi].size for gi, _ in reqs])),
-            "plain_throughput_per_s": round(n / t_plain, 3),
-            "plain_latency_ms": {"p50": pct(plain_lat, 50), "p95": pct(plain_lat, 95), "p99": pct(plain_lat, 99)},
-            "served": runs, "hbm_budget_GB": 200, "max_side": max_side}
+            "plain_throughput_per_s": round(n / t_plain, 3), "plain_s": round(t_plain, 2),
+            "plain_completed": int(sum(plain_ok)),
+            "plain_latency_ms_all": {"p50": pct(plain_lat, 50), "p95": pct(plain_lat, 95), "p99": pct(plain_lat, 99)},
+            "plain_latency_ms_completed": {"p50": pct(ok_lat, 50), "p95": pct(ok_lat, 95), "p99": pct(ok_lat, 99)},
+            "served": runs, "hbm_budget_GB": 200}
 
 
 def copy_probe():
@@ -376,7 +393,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-worst", action="store_true")
-    ap.add_argument("--served-max-side", type=int, default=4096)
+    ap.add_argument("--served-max-side", type=int, default=8192)
+    ap.add_argument("--served-requests", type=int, default=128)
     ap.add_argument("--batched-grids", type=int, default=1024)
     ap.add_argument("--worst-e2e", action="store_true",
                     help="also run the 16384^2 worst case end to end (two grids, a few minutes, ~145 GB of host "
@@ -523,7 +541,7 @@ def main():
     if not args.no_batched:
         line["batched"] = batched_bench(n=args.batched_grids)
     if not args.no_served:
-        line["served"] = served_bench(max_side=args.served_max_side)
+        line["served"] = served_bench(n=args.served_requests, max_side=args.served_max_side)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)
         line["cpu_baseline"]["product_phase_ms"] = line["phase_ms"]

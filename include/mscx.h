@@ -288,7 +288,8 @@ int mscx_server_create(size_t hbm_budget, int32_t host_threads, mscx_server **ou
  * max(2^24, 2 x factor x its arc entries) arc entries with MSCX_ENOMEM;
  * both in seconds, so the worker moves on.  Default 128 (the 4096^2 value
  * noise root at 1% takes 35x); 0 removes the limit.  Applies to requests
- * that start after the call. */
+ * that start after the call.  Plain and batched runs take the same limit
+ * from the MSCX_WORK_LIMIT environment variable (unset or 0: none). */
 int mscx_server_set_work_limit(mscx_server *s, double factor);
 int mscx_server_submit(mscx_server *s, const float *data, int32_t size_x, int32_t size_y, int32_t lx, int32_t ly,
                        double threshold, uint64_t key, uint32_t flags, uint64_t *ticket);

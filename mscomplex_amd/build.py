@@ -29,6 +29,8 @@ HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline
 # the reference's dmsc command line over the C ABI (links libmscx.so)
 CLI_SOURCE = "dmsc_main.cpp"
 CLI = os.path.join(HERE, "mscx_dmsc")
+DRIVER_SOURCE = "dataset_driver.cpp"
+DRIVER = os.path.join(HERE, "mscx_dataset_driver")
 
 
 def _hipcc():
@@ -80,8 +82,17 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, CLI)
+    # the GridDataManager order through the C++ drop-in header (an
+    # integration example; the GPU tests run it against the oracle)
+    tmp = DRIVER + ".tmp"
+    cmd = [_host_cxx(), "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, DRIVER_SOURCE), "-o", tmp, LIB,
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
     if verbose:
-        print(f"built {LIB} and {CLI}")
+        print(f"built {LIB}, {CLI} and {DRIVER}")
     return LIB
 
 

@@ -137,6 +137,9 @@ void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32
   G->threshold = threshold;
   G->flags = flags;
   G->ooc = (flags & MSCX_RUN_OUT_OF_CORE) != 0;
+  // MSCX_WORK_LIMIT in the environment bounds the simplification's work in
+  // plain and batched runs as the server's work limit does (0 or unset: none)
+  if (const char *wl = getenv("MSCX_WORK_LIMIT")) G->work_limit = std::max(0.0, atof(wl));
   const float *dev = data;
   if (!(flags & MSCX_RUN_DEVICE_INPUT)) {
     size_t n = (size_t)sx * sy;

@@ -114,3 +114,10 @@ def test_tiling_finer_than_the_grid_is_einval(tmp_path):
     assert e.value.code == -1 and "finer than the grid" in str(e.value)
     r = _cli("-gf", str(p), "-d", "2", "2", cwd=tmp_path)
     assert r.returncode == 1 and "finer than the grid" in r.stderr
+
+
+def test_dataset_driver_usage():
+    # the GridDataManager-order driver over include/mscx_dataset.hpp is
+    # built with the library and refuses bad arguments without a GPU
+    r = subprocess.run([mbuild.DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
