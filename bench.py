@@ -260,7 +260,7 @@ def batched_bench(n=1024, seed=7):
             "autotune_s": round(t_tune, 2), "generate_s": round(t_gen, 2)}
 
 
-def served_bench(n=128, seed=11, repeat_frac=0.25, loads=((1.0, 5), (4.0, 6)), max_side=8192, work_limit=128):
+def served_bench(n=100, seed=11, repeat_frac=0.25, loads=((2.0, 5), (8.0, 6)), max_side=8192, work_limit=128):
     """The served configuration: Poisson arrivals of square grids with sides
     log-uniform in [128, max_side], alternating Gaussian sums (32 Gaussians,
     +/-1e-3 noise) and value noise, every request at a threshold uniform in
@@ -306,9 +306,8 @@ def served_bench(n=128, seed=11, repeat_frac=0.25, loads=((1.0, 5), (4.0, 6)), m
             except mx.MscxError:
                 plain_ok.append(False)
             plain_lat.append(time.perf_counter() - a)
-            if k % 16 == 15:
-                print(f"[bench] served: plain {k + 1}/{n} requests, {time.perf_counter() - t0:.1f}s",
-                      file=sys.stderr, flush=True)
+            print(f"[bench] served: plain {k + 1}/{n}: side {base[gi].shape[0]}, threshold {th:.4f}, "
+                  f"{plain_lat[-1]:.2f}s{'' if plain_ok[-1] else ' (failed)'}", file=sys.stderr, flush=True)
         t_plain = time.perf_counter() - t0
     finally:
         del os.environ["MSCX_WORK_LIMIT"]
@@ -345,8 +344,8 @@ def served_bench(n=128, seed=11, repeat_frac=0.25, loads=((1.0, 5), (4.0, 6)), m
             else:
                 codes[str(st)] = codes.get(str(st), 0) + 1
             srv.release(t)
-            if k % 32 == 31:
-                print(f"[bench] served: load {load}x, {k + 1}/{n} collected", file=sys.stderr, flush=True)
+            print(f"[bench] served: load {load}x, request {k + 1}/{n}: status {st}, {l:.2f}s", file=sys.stderr,
+                  flush=True)
         t_served = time.perf_counter() - t0
         s1 = srv.stats()
         srv.close()
@@ -394,7 +393,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-worst", action="store_true")
     ap.add_argument("--served-max-side", type=int, default=8192)
-    ap.add_argument("--served-requests", type=int, default=128)
+    ap.add_argument("--served-requests", type=int, default=100)
     ap.add_argument("--batched-grids", type=int, default=1024)
     ap.add_argument("--worst-e2e", action="store_true",
                     help="also run the 16384^2 worst case end to end (two grids, a few minutes, ~145 GB of host "

@@ -284,7 +284,8 @@ int mscx_server_create(size_t hbm_budget, int32_t host_threads, mscx_server **ou
  * takes minutes to hours as strangulated pairs are resubmitted after every
  * cancellation, and grows arc multisets without bound).  A request whose
  * simplification needs more than max(2^24, factor x its initial heap
- * entries) heap pops fails with MSCX_ELIMIT, one that would insert more than
+ * entries) heap pops (at most 2^27, a few seconds) fails with MSCX_ELIMIT, one
+ * that would insert more than
  * max(2^24, 2 x factor x its arc entries) arc entries with MSCX_ENOMEM;
  * both in seconds, so the worker moves on.  Default 128 (the 4096^2 value
  * noise root at 1% takes 35x); 0 removes the limit.  Applies to requests

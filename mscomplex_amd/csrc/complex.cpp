@@ -209,6 +209,12 @@ static constexpr uint64_t kMaxArcProduct = 1ull << 20;
 // distinct arcs with large multiplicities) after un_simplify.
 static constexpr int64_t kArcEntryBudget = (int64_t)1 << 28;
 
+// Heap pops one simplify call may make under a work limit, whatever the
+// complex's size (a few seconds of work): large noisy grids, where each
+// cancellation re-pushes tens of thousands of deferred pairs, reach it long
+// before the per-entry bound.
+static constexpr double kPopCap = (double)(1ull << 27);
+
 // Arc-entry budget of one simplify / un_simplify call on m under a work
 // limit (see simplify in complex.hpp).
 static int64_t arc_entry_budget(const Complex &m, double work_limit) {
@@ -660,7 +666,8 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
   size_t narcs = 0;
   for (uint32_t i = 0; i < m.size(); ++i) narcs += m.des[i].size();
   m.arc_budget = arc_entry_budget(m, work_limit);
-  m.pop_budget = work_limit > 0 ? (int64_t)std::max((double)(1 << 24), work_limit * (double)narcs) : INT64_MAX;
+  m.pop_budget = work_limit > 0 ? (int64_t)std::min(kPopCap, std::max((double)(1 << 24), work_limit * (double)narcs))
+                                 : INT64_MAX;
   ExactHeap pq;
   pq.reserve(narcs * 2 + 16);
   slots.reserve(narcs * 4 + 16);

@@ -238,7 +238,8 @@ void merge_up(const Complex &a, const Complex &b, Complex &out);
 void merge_down(Complex &parent, Complex &a, Complex &b);
 // work_limit > 0 bounds the work of one call in proportion to the complex
 // (no counterpart in the reference, which runs until done): simplify fails
-// with MSCX_ELIMIT after max(2^24, work_limit x initial heap entries) pops,
+// with MSCX_ELIMIT after min(2^27, max(2^24, work_limit x initial heap
+// entries)) pops,
 // and either call with MSCX_ENOMEM after max(2^24, 2 x work_limit x initial
 // arc entries) inserted arc entries (at most the fixed 2^28 cap).
 void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, double thresh, double work_limit = 0);
