@@ -49,7 +49,11 @@ int mscx_version(void);
  * mscx_profile(1) resets and enables HIP-event timing of each stage on the
  * work stream; mscx_profile_read returns accumulated milliseconds and event
  * pair counts for stages 0 gradient, 1 scan, 2 compaction, 3 trace,
- * 4 owners, 5 incidence, 6 host complex build. */
+ * 4 owners, 5 incidence, 6 host complex build, and the manifold pass's
+ * parts as wall clock of the calls: 7 critical-point pass, 8 label tables,
+ * 10 disc tracing on the GPU (with its transfers and scan), 13 grouping of
+ * the extremum discs; and as host-thread time: 9 waiting for the labels,
+ * 11 copying the traced lists, 12 grouping. */
 int mscx_profile(int enable);
 int mscx_profile_read(double *ms, uint32_t *counts, int n);
 

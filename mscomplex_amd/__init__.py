@@ -115,7 +115,9 @@ def graph_stats():
     return int(c.value), int(r.value)
 
 
-STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex")
+STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex",
+          "disc_prep", "disc_tables", "disc_label_wait_thread_time", "disc_trace", "disc_lists_thread_time",
+          "disc_group_thread_time", "disc_group")
 
 
 def profile_read():

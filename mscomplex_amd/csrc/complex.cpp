@@ -78,10 +78,14 @@ uint32_t CellMap::get_or_zero(uint32_t key) {
 
 DiscSpan Complex::disc(uint32_t i, int j) const {
   const std::vector<uint32_t> &v = j == 0 ? asc_disc[i] : des_disc[i];
+  if (!v.empty()) return DiscSpan{v.data(), v.size()};
+  const size_t sl = 2 * (size_t)i + j;
+  if (sl + 1 < sd_off.size() && sd_off[sl + 1] > sd_off[sl])
+    return DiscSpan{sd_cells.data() + sd_off[sl], sd_off[sl + 1] - sd_off[sl]};
   uint32_t d = cell_dim(cell[i]);
-  if (!xd_off.empty() && i + 1 < xd_off.size() && v.empty() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
+  if (!xd_off.empty() && i + 1 < xd_off.size() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
     return DiscSpan{xd_cells.data() + xd_off[i], xd_off[i + 1] - xd_off[i]};
-  return DiscSpan{v.data(), v.size()};
+  return DiscSpan{v.data(), 0};
 }
 
 void Complex::reserve_nodes(size_t n) {
@@ -778,50 +782,75 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
     }
   };
   CellMap seen_min, seen_max;
-  for (uint32_t i = 0; i < m.size(); ++i) {
-    uint32_t c = m.cell[i];
-    uint32_t dim = cell_dim(c);
-    if (m.flags[i] & CPF_BCANC) {
-      uint32_t pi = m.pair[i];
-      uint32_t pc = m.cell[pi];
-      if (dim == 1) {
-        const ConnSet &conn = (cell_dim(pc) == 0) ? m.des[i] : m.asc[i];
-        if (conn.size() != 1) throw std::logic_error("I should be connected to exactly one surv extrema");
-        uint32_t ec = m.cell[conn[0]];
-        if (m.pair[pi] != i) throw std::logic_error("My pair does not know Im paired with him");
-        if (in_r(c)) set_redirect(c, ec);
-        if (in_r(pc)) set_redirect(pc, ec);
-      } else if (dim == 2) {
-        for (uint32_t s : m.des[i])
-          if (in_r(pc)) m.asc_disc[s].push_back(pc);
-      } else {
-        for (uint32_t s : m.asc[i])
-          if (in_r(pc)) m.des_disc[s].push_back(pc);
-      }
-    } else {
-      if (dim == 0) {
-        if (!seen_min.contains(c)) {  // map::insert keeps the first
-          seen_min.set(c, i);
-          out.minmap.emplace_back(c, i);
+  // the discs' initial entries, in the order the reference appends them
+  // (node order; a saddle's own seeds when its node comes): counted in the
+  // first pass, written in the second
+  const size_t nslot = 2 * (size_t)m.size();
+  out.init_off.assign(nslot + 1, 0);
+  out.traced.assign(nslot, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<uint32_t> cur;
+    if (pass == 1) {
+      for (size_t k = 0; k < nslot; ++k) out.init_off[k + 1] += out.init_off[k];
+      out.init_cells.resize(out.init_off[nslot]);
+      cur.assign(out.init_off.begin(), out.init_off.end() - 1);
+    }
+    auto push = [&](uint32_t node, int j, uint32_t cell) {
+      const size_t sl = 2 * (size_t)node + j;
+      if (pass == 0)
+        out.init_off[sl + 1]++;
+      else
+        out.init_cells[cur[sl]++] = cell;
+    };
+    for (uint32_t i = 0; i < m.size(); ++i) {
+      uint32_t c = m.cell[i];
+      uint32_t dim = cell_dim(c);
+      if (m.flags[i] & CPF_BCANC) {
+        uint32_t pi = m.pair[i];
+        uint32_t pc = m.cell[pi];
+        if (dim == 1) {
+          if (pass == 1) continue;
+          const ConnSet &conn = (cell_dim(pc) == 0) ? m.des[i] : m.asc[i];
+          if (conn.size() != 1) throw std::logic_error("I should be connected to exactly one surv extrema");
+          uint32_t ec = m.cell[conn[0]];
+          if (m.pair[pi] != i) throw std::logic_error("My pair does not know Im paired with him");
+          if (in_r(c)) set_redirect(c, ec);
+          if (in_r(pc)) set_redirect(pc, ec);
+        } else if (dim == 2) {
+          for (uint32_t s : m.des[i])
+            if (in_r(pc)) push(s, 0, pc);
+        } else {
+          for (uint32_t s : m.asc[i])
+            if (in_r(pc)) push(s, 1, pc);
         }
-      } else if (dim == 1) {
-        int x = cx_of(c), y = cy_of(c);
-        const int inc[4][2] = {{x, y + 1}, {x, y - 1}, {x - 1, y}, {x + 1, y}};
-        for (int j = 0; j < 4; ++j) {
-          int ix = inc[j][0], iy = inc[j][1];
-          if (!rect.contains(ix, iy)) continue;
-          uint8_t code = cv.at(ix, iy);
-          if ((code & CF_PAIRED) && !(code & CF_CRITICAL)) {
-            int d = code_dir(code);
-            uint32_t pr = pack_xy(ix + dir_dx(d), iy + dir_dy(d));
-            (((ix & 1) + (iy & 1)) == 0 ? m.des_disc[i] : m.asc_disc[i]).push_back(pr);
+      } else {
+        if (dim == 0) {
+          if (pass == 0 && !seen_min.contains(c)) {  // map::insert keeps the first
+            seen_min.set(c, i);
+            out.minmap.emplace_back(c, i);
           }
-        }
-        out.surv_saddles.push_back(i);
-      } else {
-        if (!seen_max.contains(c)) {
-          seen_max.set(c, i);
-          out.maxmap.emplace_back(c, i);
+        } else if (dim == 1) {
+          int x = cx_of(c), y = cy_of(c);
+          const int inc[4][2] = {{x, y + 1}, {x, y - 1}, {x - 1, y}, {x + 1, y}};
+          for (int j = 0; j < 4; ++j) {
+            int ix = inc[j][0], iy = inc[j][1];
+            if (!rect.contains(ix, iy)) continue;
+            uint8_t code = cv.at(ix, iy);
+            if ((code & CF_PAIRED) && !(code & CF_CRITICAL)) {
+              int d = code_dir(code);
+              uint32_t pr = pack_xy(ix + dir_dx(d), iy + dir_dy(d));
+              push(i, ((ix & 1) + (iy & 1)) == 0 ? 1 : 0, pr);
+            }
+          }
+          if (pass == 0) {
+            out.surv_saddles.push_back(i);
+            out.traced[2 * (size_t)i] = out.traced[2 * (size_t)i + 1] = 1;
+          }
+        } else {
+          if (pass == 0 && !seen_max.contains(c)) {
+            seen_max.set(c, i);
+            out.maxmap.emplace_back(c, i);
+          }
         }
       }
     }
@@ -882,54 +911,72 @@ void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t
       if (lq[r] != UINT32_MAX) m.xd_cells[pos[lq[r]]++] = pack_xy(x, y);
 }
 
-void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv) {
-  disc_paths_range(m, dp, rect, ext, cv, 0, dp.surv_saddles.size());
+// The successors of one path cell (grid_dataset.cpp:1200-1224): its
+// cofacets inside the ext (j = 0) or facets (j = 1), in the reference's
+// order, that are not critical and lie in the rect, each replaced by its
+// pair unless that is the path cell itself.
+static inline int disc_successors(uint32_t pcell, int j, const Rect &rect, const Rect &ext, const CodesView &cv,
+                                  uint32_t succ[4]) {
+  const int px = cx_of(pcell), py = cy_of(pcell);
+  uint32_t cets[4];
+  int ct = 0;
+  const int dx = px & 1, dy = py & 1;
+  if (!ext.contains(px, py)) return 0;  // no path cell lies outside the ext
+  if (j == 0) {  // cofacets, filtered to ext
+    if (dx + dy == 0) {
+      const int c4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
+      for (int q = 0; q < 4; ++q)
+        if (ext.contains(c4[q][0], c4[q][1])) cets[ct++] = pack_xy(c4[q][0], c4[q][1]);
+    } else if (dx + dy == 1) {
+      if (ext.contains(px + dy, py + dx)) cets[ct++] = pack_xy(px + dy, py + dx);
+      if (ext.contains(px - dy, py - dx)) cets[ct++] = pack_xy(px - dy, py - dx);
+    }
+  } else {  // facets
+    if (dx + dy == 1) {
+      cets[ct++] = pack_xy(px + dx, py + dy);
+      cets[ct++] = pack_xy(px - dx, py - dy);
+    } else if (dx + dy == 2) {
+      const int f4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
+      for (int q = 0; q < 4; ++q) cets[ct++] = pack_xy(f4[q][0], f4[q][1]);
+    }
+  }
+  int ns = 0;
+  for (int q = 0; q < ct; ++q) {
+    const int qx = cx_of(cets[q]), qy = cy_of(cets[q]);
+    const uint8_t code = cv.at(qx, qy);
+    if (code & CF_CRITICAL) continue;
+    if (!rect.contains(qx, qy)) continue;
+    const int d = code_dir(code);
+    const uint32_t p = d ? pack_xy(qx + dir_dx(d), qy + dir_dy(d)) : UINT32_MAX;
+    if (p == pcell) continue;
+    succ[ns++] = p;
+  }
+  return ns;
 }
 
-// The saddles [begin, end) of dp.surv_saddles; each saddle's paths are its
-// own, so disjoint ranges may run on different threads.
-void disc_paths_range(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv,
-                      size_t begin, size_t end) {
-  for (size_t q0 = begin; q0 < end; ++q0) {
-    const uint32_t si = dp.surv_saddles[q0];
-    for (int j = 0; j < 2; ++j) {
-      std::vector<uint32_t> &disc = j == 0 ? m.asc_disc[si] : m.des_disc[si];
-      for (size_t k = 0; k != disc.size(); ++k) {
-        uint32_t pcell = disc[k];
-        int px = cx_of(pcell), py = cy_of(pcell);
-        uint32_t cets[4];
-        int ct = 0;
-        int dx = px & 1, dy = py & 1;
-        if (j == 0) {  // cofacets, filtered to ext
-          if (dx + dy == 0) {
-            const int c4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
-            for (int q = 0; q < 4; ++q)
-              if (ext.contains(c4[q][0], c4[q][1])) cets[ct++] = pack_xy(c4[q][0], c4[q][1]);
-          } else if (dx + dy == 1) {
-            if (ext.contains(px + dy, py + dx)) cets[ct++] = pack_xy(px + dy, py + dx);
-            if (ext.contains(px - dy, py - dx)) cets[ct++] = pack_xy(px - dy, py - dx);
-          }
-        } else {  // facets
-          if (dx + dy == 1) {
-            cets[ct++] = pack_xy(px + dx, py + dy);
-            cets[ct++] = pack_xy(px - dx, py - dy);
-          } else if (dx + dy == 2) {
-            const int f4[4][2] = {{px, py + 1}, {px, py - 1}, {px - 1, py}, {px + 1, py}};
-            for (int q = 0; q < 4; ++q) cets[ct++] = pack_xy(f4[q][0], f4[q][1]);
-          }
-        }
-        for (int q = 0; q < ct; ++q) {
-          int qx = cx_of(cets[q]), qy = cy_of(cets[q]);
-          uint8_t code = cv.at(qx, qy);
-          if (code & CF_CRITICAL) continue;
-          if (!rect.contains(qx, qy)) continue;
-          int d = code_dir(code);
-          uint32_t p = d ? pack_xy(qx + dir_dx(d), qy + dir_dy(d)) : UINT32_MAX;
-          if (p == pcell) continue;
-          disc.push_back(p);
-        }
-      }
-    }
+void disc_trace_one(const uint32_t *init, size_t n_init, int j, const Rect &rect, const Rect &ext,
+                    const CodesView &cv, std::vector<uint32_t> &out) {
+  const size_t base = out.size();
+  out.insert(out.end(), init, init + n_init);
+  for (size_t k = base; k != out.size(); ++k) {
+    uint32_t succ[4];
+    const int ns = disc_successors(out[k], j, rect, ext, cv, succ);
+    for (int q = 0; q < ns; ++q) out.push_back(succ[q]);
+  }
+}
+
+void disc_trace_host(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv) {
+  const size_t nslot = 2 * (size_t)m.size();
+  m.sd_off.assign(nslot + 1, 0);
+  m.sd_cells.clear();
+  for (size_t sl = 0; sl < nslot; ++sl) {
+    const uint32_t *init = dp.init_cells.data() + dp.init_off[sl];
+    const size_t n = dp.init_off[sl + 1] - dp.init_off[sl];
+    if (dp.traced[sl] && n)
+      disc_trace_one(init, n, (int)(sl & 1), rect, ext, cv, m.sd_cells);
+    else
+      m.sd_cells.insert(m.sd_cells.end(), init, init + n);
+    m.sd_off[sl + 1] = (uint32_t)m.sd_cells.size();
   }
 }
 
@@ -941,7 +988,7 @@ void fill_discs(Complex &m, const Rect &rect, const Rect &ext, const uint8_t *co
   std::vector<uint32_t> lv, lq;
   disc_labels_host(dp, rect, ext, own.data(), cw, lv, lq);
   disc_group(m, rect, lv.data(), lq.data());
-  disc_paths(m, dp, rect, ext, cv);
+  disc_trace_host(m, dp, rect, ext, cv);
   for (const auto &r : dp.redirect) own[(size_t)(cy_of(r.first) - ext.y0) * cw + (cx_of(r.first) - ext.x0)] = r.second;
 }
 

@@ -203,6 +203,10 @@ struct Complex {
   // discs of surviving extrema (vertices of minima, quads of maxima) as
   // raster-ordered lists in CSR form by critical-point index
   std::vector<uint32_t> xd_off, xd_cells;
+  // the other discs postMergeFillDiscs fills (the saddles' traced 1-manifolds
+  // and the cells it appends to saddles of cancelled extrema) in CSR form by
+  // slot 2 i + j (j = 0 asc, 1 des); empty until the manifold pass ran
+  std::vector<uint32_t> sd_off, sd_cells;
   CellMap idx;
   // Arc insertions cancel_pairs may still make before failing (set by
   // simplify; see cancel_pairs).
@@ -260,14 +264,23 @@ struct DiscPrep {
   std::vector<std::pair<uint32_t, uint32_t>> redirect;  // (cell, owner), sorted by cell
   std::vector<std::pair<uint32_t, uint32_t>> minmap, maxmap;  // (cell, cp index), sorted by cell
   std::vector<uint32_t> surv_saddles;
+  // the entries postMergeFillDiscs appends to each disc before tracing, in
+  // its order (CSR by slot 2 i + j), and per slot whether it is traced (the
+  // discs of surviving saddles)
+  std::vector<uint32_t> init_off, init_cells;
+  std::vector<uint8_t> traced;
 };
+// One disc traced from its initial entries (the reference's loop over the
+// growing list, grid_dataset.cpp:1189-1228): appends the whole list to out.
+void disc_trace_one(const uint32_t *init, size_t n_init, int j, const Rect &rect, const Rect &ext,
+                    const CodesView &cv, std::vector<uint32_t> &out);
+// Every slot's disc (traced or copied) into m.sd_off / m.sd_cells.
+void disc_trace_host(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv);
 void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out);
 void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, const uint32_t *own, int64_t own_pitch,
                       std::vector<uint32_t> &lv, std::vector<uint32_t> &lq);
 void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq);
-void disc_paths(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv);
-void disc_paths_range(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv,
-                      size_t begin, size_t end);
+
 
 // postMergeFillDiscs for n worked tiles and their complexes, label pass on
 // the GPU (discs.cpp).

@@ -12,6 +12,8 @@
 #include <vector>
 
 #include "complex.hpp"
+#include <chrono>
+
 #include "runtime.hpp"
 #include "tile.hpp"
 
@@ -24,8 +26,22 @@ namespace mscx {
 // serving loop's workers) each lease their own set, so their host work runs
 // concurrently; a set is kept for reuse once returned, so steady-state calls
 // allocate nothing.
+static double now_wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 struct DiscScratch {
-  Scratch disc, stage;
+  Scratch disc, stage, trace, lists;
+};
+
+// A block of the device pool that goes back once `s` has finished with it.
+struct PoolBlock {
+  void *p;
+  hipStream_t s;
+  ~PoolBlock() {
+    (void)hipStreamSynchronize(s);
+    device_pool().free(p);
+  }
 };
 static std::mutex g_pool_mu;
 static std::vector<std::unique_ptr<DiscScratch>> g_pool;
@@ -208,6 +224,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   for (int i = 0; i < n; ++i)
     cv[i] = CodesView{(const uint8_t *)host + code_off[i], tiles[i]->d.pitch, ext[i].x0, ext[i].y0};
 
+  StageTimer &tm = stage_timer();
+  double tp = now_wall();
   // 2. critical-point pass (seeds, redirects, surviving maps)
   std::vector<size_t> nk(n);
   for (int i = 0; i < n; ++i) nk[i] = tiles[i]->K;
@@ -219,6 +237,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     });
   }
 
+  tm.add_host(ST_DISC_PREP, now_wall() - tp);
+  tp = now_wall();
   // 3. label tables per critical index (host) + label pass (device)
   std::vector<size_t> tab_off(n + 1, 0);
   for (int i = 0; i < n; ++i) tab_off[i + 1] = tab_off[i] + align_up(2 * (size_t)tiles[i]->K * 4, 256);
@@ -230,14 +250,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   // device tables and labels come from the library's pool (cached between
   // calls, counted against a server's HBM budget while in use); the block
   // goes back once the auxiliary stream has finished with it
-  struct PoolBlock {
-    void *p;
-    hipStream_t s;
-    ~PoolBlock() {
-      (void)hipStreamSynchronize(s);
-      device_pool().free(p);
-    }
-  } devblk{device_pool().alloc(dev_bytes), s};
+  PoolBlock devblk{device_pool().alloc(dev_bytes), s};
   char *dev = (char *)devblk.p;
   char *stage = (char *)g_stage.reserve_host(o_lab);
   DiscDesc *hd = (DiscDesc *)(stage + o_desc);
@@ -252,6 +265,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
                         (uint32_t)items[q].end);
     });
   }
+  tm.add_host(ST_DISC_TABLES, now_wall() - tp);
   size_t lo = 0;
   std::vector<size_t> lab_off(n);
   for (int i = 0; i < n; ++i) {
@@ -300,33 +314,138 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   uint32_t *herr = (uint32_t *)(stage + o_err);
   MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
 
-  // 4. saddle tracing (needs no labels; each saddle's paths are its own, so
-  //    a large leaf's saddles are split) while the labels arrive, then the
-  //    extremum discs (grouping: one item per leaf, after every tracing
-  //    item; it writes other fields of the complex than the tracing does);
-  //    a label the kernel flagged as bad is the none label, which grouping
-  //    skips
-  std::vector<size_t> ns(n);
-  for (int i = 0; i < n; ++i) ns[i] = dp[i].surv_saddles.size();
-  std::vector<Chunk> items = chunks_of(ns, 1 << 13);
-  const int ntrace = (int)items.size();
-  for (int i = 0; i < n; ++i) items.push_back({i, 0, 0});
+  // 4. disc tracing on the GPU (the saddles' 1-manifolds; every slot's list
+  //    counted, the counts scanned here into each complex's offsets, then
+  //    written), overlapping the extremum discs' grouping on the host as
+  //    each leaf's labels arrive
+  tp = now_wall();
+  std::vector<size_t> nslot(n), slot_base(n + 1, 0), init_base(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    nslot[i] = 2 * (size_t)cs[i]->size();
+    slot_base[i + 1] = slot_base[i] + nslot[i];
+    init_base[i + 1] = init_base[i] + dp[i].init_cells.size();
+  }
+  const size_t S = slot_base[n], NI = init_base[n];
+  // device: descriptors | prefix | error | init_off (S + n) | traced (S) |
+  // init cells (NI) | counts (S) | offsets (S); the lists follow in their
+  // own block once their total is known
+  const size_t t_desc = 0, t_pref = align_up(sizeof(DiscTraceDesc) * n, 256);
+  const size_t t_err = align_up(t_pref + 4 * (n + 1), 256);
+  const size_t t_ioff = t_err + 256;
+  const size_t t_trc = align_up(t_ioff + 4 * (S + n), 256);
+  const size_t t_init = align_up(t_trc + S, 256);
+  const size_t t_cnt = align_up(t_init + 4 * NI, 256);
+  const size_t t_off = align_up(t_cnt + 4 * S, 256);
+  const size_t t_bytes = align_up(t_off + 4 * S, 256);
+  PoolBlock trblk{device_pool().alloc(t_bytes), s};
+  char *tdev = (char *)trblk.p;
+  char *th = (char *)lease.s->trace.reserve_host(t_bytes);
+  DiscTraceDesc *td = (DiscTraceDesc *)(th + t_desc);
+  uint32_t *terr_h = (uint32_t *)(th + t_err);
+  par_for(n, [&](int i) {
+    const DiscPrep &P = dp[i];
+    memcpy(th + t_ioff + 4 * (slot_base[i] + i), P.init_off.data(), 4 * (nslot[i] + 1));
+    memcpy(th + t_trc + slot_base[i], P.traced.data(), nslot[i]);
+    if (!P.init_cells.empty()) memcpy(th + t_init + 4 * init_base[i], P.init_cells.data(), 4 * P.init_cells.size());
+  });
+  for (int i = 0; i < n; ++i) {
+    const mscx_tile *t = tiles[i];
+    DiscTraceDesc &D = td[i];
+    D.codes = t->d.codes;
+    D.pitch = t->d.pitch;
+    D.ox = ext[i].x0;
+    D.oy = ext[i].y0;
+    D.ex1 = ext[i].x1;
+    D.ey1 = ext[i].y1;
+    D.rx0 = rect[i].x0;
+    D.ry0 = rect[i].y0;
+    D.rx1 = rect[i].x1;
+    D.ry1 = rect[i].y1;
+    D.init_off = (const uint32_t *)(tdev + t_ioff) + slot_base[i] + i;
+    D.init_cells = (const uint32_t *)(tdev + t_init) + init_base[i];
+    D.traced = (const uint8_t *)(tdev + t_trc) + slot_base[i];
+    D.nslot = (uint32_t)nslot[i];
+    D.count = (uint32_t *)(tdev + t_cnt) + slot_base[i];
+    D.out_off = (const uint32_t *)(tdev + t_off) + slot_base[i];
+    D.out = nullptr;
+    D.err = (uint32_t *)(tdev + t_err);
+  }
+  const int tblocks = plan_disc_trace_blocks(td, n, (int *)(th + t_pref));
+  *terr_h = 0;
+  MSCX_HIP(hipMemcpyAsync(tdev, th, t_cnt, hipMemcpyHostToDevice, s));
+  launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, false, s);
+  MSCX_HIP(hipGetLastError());
+  // counts (and the error word) back
+  uint32_t *cnt_h = (uint32_t *)(th + t_cnt);
+  MSCX_HIP(hipMemcpyAsync(cnt_h, tdev + t_cnt, 4 * S, hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipMemcpyAsync(terr_h, tdev + t_err, 4, hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipStreamSynchronize(s));
+  const bool gpu_trace = *terr_h == 0;
+  std::vector<size_t> out_base(n + 1, 0);
+  if (gpu_trace) {
+    // each complex's offsets by slot; the device offsets are the same per leaf
+    uint32_t *off_h = (uint32_t *)(th + t_off);
+    par_for(n, [&](int i) {
+      std::vector<uint32_t> &o = cs[i]->sd_off;
+      o.assign(nslot[i] + 1, 0);
+      const uint32_t *c = cnt_h + slot_base[i];
+      for (size_t k = 0; k < nslot[i]; ++k) o[k + 1] = o[k] + c[k];
+      memcpy(off_h + slot_base[i], o.data(), 4 * nslot[i]);
+    });
+    for (int i = 0; i < n; ++i) out_base[i + 1] = out_base[i] + cs[i]->sd_off[nslot[i]];
+  }
+  const size_t NO = out_base[n];
+  PoolBlock outblk{device_pool().alloc(std::max<size_t>(4 * NO, 256)), s};
+  uint32_t *out_h = nullptr;
+  if (gpu_trace) {
+    for (int i = 0; i < n; ++i) td[i].out = (uint32_t *)outblk.p + out_base[i];
+    MSCX_HIP(hipMemcpyAsync(tdev, th, t_err, hipMemcpyHostToDevice, s));
+    MSCX_HIP(hipMemcpyAsync(tdev + t_off, th + t_off, 4 * S, hipMemcpyHostToDevice, s));
+    launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, true, s);
+    MSCX_HIP(hipGetLastError());
+    out_h = (uint32_t *)lease.s->lists.reserve_host(std::max<size_t>(4 * NO, 4));
+    if (NO) MSCX_HIP(hipMemcpyAsync(out_h, outblk.p, 4 * NO, hipMemcpyDeviceToHost, s));
+  }
+  tm.add_host(ST_DISC_PATHS, now_wall() - tp);
+
+  // 5. the extremum discs (grouping, one item per leaf as its labels
+  //    arrive); a label the kernel flagged as bad is the none label, which
+  //    grouping skips.  Without the GPU lists (a disc with more chains than
+  //    a thread follows) every leaf's discs are traced here instead.
+  tp = now_wall();
+  std::atomic<int64_t> wait_ns(0), trace_ns(0), group_ns(0);
   try {
-    par_for((int)items.size(), [&](int q) {
-      const int i = items[q].leaf;
-      if (q < ntrace) {
-        disc_paths_range(*cs[i], dp[i], rect[i], ext[i], cv[i], items[q].begin, items[q].end);
+    par_for(2 * n, [&](int q) {
+      const double w0 = now_wall();
+      if (q >= n) {
+        if (gpu_trace) return;
+        const int i = q - n;
+        disc_trace_host(*cs[i], dp[i], rect[i], ext[i], cv[i]);
+        trace_ns += (int64_t)(1e9 * (now_wall() - w0));
         return;
       }
+      const int i = q;
       MSCX_HIP(hipEventSynchronize(ev.e[i]));
+      const double w1 = now_wall();
+      wait_ns += (int64_t)(1e9 * (w1 - w0));
       const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);
       disc_group(*cs[i], rect[i], lv, lv + nv[i]);
+      group_ns += (int64_t)(1e9 * (now_wall() - w1));
     });
   } catch (...) {
     (void)hipStreamSynchronize(s);  // no copy may still be landing in the staging buffers
     throw;
   }
   MSCX_HIP(hipStreamSynchronize(s));
+  if (gpu_trace) {
+    const double w0 = now_wall();
+    par_for(n, [&](int i) { cs[i]->sd_cells.assign(out_h + out_base[i], out_h + out_base[i + 1]); });
+    trace_ns += (int64_t)(1e9 * (now_wall() - w0));
+  }
+  tm.add_host(ST_DISC_LABELS, 1e-9 * (double)wait_ns.load());
+  tm.add_host(ST_DISC_TRACE_CPU, 1e-9 * (double)trace_ns.load());
+  tm.add_host(ST_DISC_GROUP_CPU, 1e-9 * (double)group_ns.load());
+  tm.add_host(ST_DISC_GROUP, now_wall() - tp);
   if (*herr) throw std::logic_error("owner extremum not in surviving map");
 }
 

@@ -813,6 +813,145 @@ __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict_
   if (hasq) D.lab_q[(int64_t)y * (nvx - 1) + x] = lq;
 }
 
+// ------------------------------------------------------- disc tracing
+// postMergeFillDiscs' saddle 1-manifolds (grid_dataset.cpp:1189-1228).  The
+// reference walks a growing list: every entry appends its successors (its
+// cofacets inside the ext for the ascending disc, its facets for the
+// descending one, that are not critical and lie in the rect, each replaced
+// by its pair unless that is the entry itself).  Every entry after the
+// initial ones is a paired cell whose own pair is one of those cofacets or
+// facets, so it has at most one successor: the list is the initial entries,
+// then the chains grown from their successors (in order) interleaved round
+// by round, a chain dropping out when it ends.  A thread follows one slot's
+// chains in that order, first counting, then writing.
+
+__device__ __forceinline__ bool dt_in(int x, int y, int x0, int y0, int x1, int y1) {
+  return x >= x0 && y >= y0 && x <= x1 && y <= y1;
+}
+
+// successors of a path cell; returns their number (at most 4)
+__device__ __forceinline__ int dt_succ(const DiscTraceDesc &D, const GLB uint8_t *codes, uint32_t pc, int j,
+                                       uint32_t succ[4]) {
+  const int px = (int16_t)(pc & 0xFFFF), py = (int16_t)(pc >> 16);
+  if (!dt_in(px, py, D.ox, D.oy, D.ex1, D.ey1)) return 0;
+  const int dx = px & 1, dy = py & 1, dim = dx + dy;
+  int cx[4], cy[4], ct = 0;
+  if (j == 0) {
+    if (dim == 0) {
+      const int ox[4] = {0, 0, -1, 1}, oy[4] = {1, -1, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (dt_in(px + ox[q], py + oy[q], D.ox, D.oy, D.ex1, D.ey1)) cx[ct] = px + ox[q], cy[ct] = py + oy[q], ++ct;
+    } else if (dim == 1) {
+      if (dt_in(px + dy, py + dx, D.ox, D.oy, D.ex1, D.ey1)) cx[ct] = px + dy, cy[ct] = py + dx, ++ct;
+      if (dt_in(px - dy, py - dx, D.ox, D.oy, D.ex1, D.ey1)) cx[ct] = px - dy, cy[ct] = py - dx, ++ct;
+    }
+  } else {
+    if (dim == 1) {
+      cx[0] = px + dx, cy[0] = py + dy, cx[1] = px - dx, cy[1] = py - dy, ct = 2;
+    } else if (dim == 2) {
+      cx[0] = px, cy[0] = py + 1, cx[1] = px, cy[1] = py - 1, cx[2] = px - 1, cy[2] = py, cx[3] = px + 1, cy[3] = py;
+      ct = 4;
+    }
+  }
+  int ns = 0;
+  for (int q = 0; q < ct; ++q) {
+    const uint8_t code = codes[(int64_t)(cy[q] - D.oy) * D.pitch + (cx[q] - D.ox)];
+    if (code & CF_CRITICAL) continue;
+    if (!dt_in(cx[q], cy[q], D.rx0, D.ry0, D.rx1, D.ry1)) continue;
+    const int d = code_dir(code);
+    const uint32_t p = d ? pack_cell(cx[q] + dir_dx(d), cy[q] + dir_dy(d)) : 0xFFFFFFFFu;
+    if (p == pc) continue;
+    succ[ns++] = p;
+  }
+  return ns;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restrict__ descs,
+                                                    const int *__restrict__ prefix, int n) {
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, n, b);
+  const DiscTraceDesc &D = descs[t];
+  const uint32_t s = (uint32_t)(b - prefix[t]) * 256u + threadIdx.x;
+  if (s >= D.nslot) return;
+  const GLB uint8_t *codes = glb(D.codes);
+  const uint32_t i0 = D.init_off[s], ni = D.init_off[s + 1] - i0;
+  const uint32_t *init = D.init_cells + i0;
+  GLB uint32_t *out = WRITE ? glb(D.out) + D.out_off[s] : nullptr;
+  uint32_t len = 0;
+  if (!D.traced[s] || ni == 0) {
+    if (WRITE)
+      for (uint32_t k = 0; k < ni; ++k) out[k] = init[k];
+    else
+      D.count[s] = ni;
+    return;
+  }
+  const int j = (int)(s & 1u);
+  uint32_t cur[DISC_TRACE_CHAINS];
+  int r = 0;
+  uint32_t succ[4];
+  for (uint32_t k = 0; k < ni; ++k) {
+    const uint32_t c = init[k];
+    if (WRITE) out[len] = c;
+    ++len;
+    const int ns = dt_succ(D, codes, c, j, succ);
+    for (int q = 0; q < ns; ++q) {
+      if (r == DISC_TRACE_CHAINS) {
+        atomicOr(D.err, 1u);
+        return;
+      }
+      cur[r++] = succ[q];
+    }
+  }
+  for (int k = 0; k < r; ++k)
+    if (cur[k] == 0xFFFFFFFFu) {  // an unpaired successor: not a gradient the chains can follow
+      atomicOr(D.err, 4u);
+      return;
+    }
+  // round by round: the chains' current cells in order, each then replaced
+  // by its successor (0xFFFFFFFF marks an ended chain).  A V-path of a
+  // gradient is acyclic, so a chain is shorter than the ext has cells; the
+  // bound ends the walk on invalid input.
+  const uint32_t limit = (uint32_t)(D.ex1 - D.ox + 1) * (uint32_t)(D.ey1 - D.oy + 1);
+  int alive = r;
+  while (alive > 0) {
+    for (int k = 0; k < r; ++k) {
+      const uint32_t c = cur[k];
+      if (c == 0xFFFFFFFFu) continue;
+      if (WRITE) out[len] = c;
+      ++len;
+      const int ns = dt_succ(D, codes, c, j, succ);
+      if (ns > 1 || (ns == 1 && succ[0] == 0xFFFFFFFFu) || len > limit + ni) {
+        atomicOr(D.err, 2u);
+        return;
+      }
+      cur[k] = ns ? succ[0] : 0xFFFFFFFFu;
+      alive -= ns ? 0 : 1;
+    }
+  }
+  if (!WRITE) D.count[s] = len;
+}
+
+int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix) {
+  int acc = 0;
+  for (int i = 0; i < n; ++i) {
+    prefix[i] = acc;
+    acc += (int)((d[i].nslot + 255) / 256);
+  }
+  prefix[n] = acc;
+  return acc;
+}
+
+void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, int nblocks, bool write,
+                       hipStream_t s) {
+  if (!nblocks) return;
+  if (write)
+    hipLaunchKernelGGL(k_disc_trace<true>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+  else
+    hipLaunchKernelGGL(k_disc_trace<false>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+}
+
 // -------------------------------------------------------------- launchers
 
 static int blocks_of(int kind, const TileDesc &T) {

@@ -25,6 +25,11 @@ void launch_owner(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int 
 void launch_compact(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 void launch_incidence(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, hipStream_t s);
 
+// Disc tracing over the slots of n leaves (prefix: n+1 block offsets of
+// 256 slots); write = false counts each slot's list, true writes it.
+int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix);
+void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, int nblocks, bool write,
+                       hipStream_t s);
 // Manifold labels for ndisc leaves; prefix has ndisc+1 entries of 256-item blocks.
 int plan_disc_blocks(const DiscDesc *d, int n, int *prefix);
 void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s);

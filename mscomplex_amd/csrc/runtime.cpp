@@ -156,6 +156,14 @@ void StageTimer::end(int k, hipStream_t s) {
   pending[k] = true;
 }
 
+void StageTimer::add_host(int k, double seconds) {
+  if (!on) return;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  ms[k] += 1e3 * seconds;
+  count[k]++;
+}
+
 void StageTimer::collect() {
   if (!on) return;
   for (int k = 0; k < ST_COUNT; ++k) {

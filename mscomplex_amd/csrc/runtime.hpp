@@ -124,7 +124,18 @@ DevicePool &device_pool();
 
 // Optional per-stage device timing with HIP events recorded on the work
 // stream around each stage's launches (enabled by mscx_profile).
-enum Stage { ST_GRADIENT = 0, ST_SCAN, ST_COMPACT, ST_TRACE, ST_OWNER, ST_INCIDENCE, ST_HOST_COMPLEX, ST_COUNT };
+// Stages 0-6 are device stages timed with HIP events; the manifold pass's
+// host parts are wall-clock times of the calls (summed over concurrent
+// calls): 7 critical-point pass, 8 label tables, 10 the disc tracing on the
+// GPU with its transfers and scan, 13 the extremum discs' grouping with the
+// lists' copy into the complexes; 9, 11 and 12 are host-thread times
+// (summed over the threads) of waiting for the labels, of the lists' copy
+// (or host tracing) and of grouping.
+enum Stage {
+  ST_GRADIENT = 0, ST_SCAN, ST_COMPACT, ST_TRACE, ST_OWNER, ST_INCIDENCE, ST_HOST_COMPLEX,
+  ST_DISC_PREP, ST_DISC_TABLES, ST_DISC_LABELS, ST_DISC_PATHS, ST_DISC_TRACE_CPU, ST_DISC_GROUP_CPU, ST_DISC_GROUP,
+  ST_COUNT
+};
 struct StageTimer {
   bool on = false;
   hipEvent_t a[ST_COUNT] = {}, b[ST_COUNT] = {};
@@ -135,6 +146,7 @@ struct StageTimer {
   void begin(int k, hipStream_t s);
   void end(int k, hipStream_t s);
   void collect();  // call after a stream synchronisation
+  void add_host(int k, double seconds);  // a host stage's wall-clock time
 };
 StageTimer &stage_timer();
 

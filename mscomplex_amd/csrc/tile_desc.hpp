@@ -133,4 +133,27 @@ struct DiscDesc {
 
 constexpr uint32_t LAB_ERR = 0xFFFFFFFEu;  // label table entry: owner is no surviving extremum
 
+// Per-leaf inputs of the disc tracing (postMergeFillDiscs' saddle
+// 1-manifolds, grid_dataset.cpp:1189-1228): one thread per disc slot
+// 2 i + j of the leaf's complex (j = 0 ascending, 1 descending).  A slot's
+// list starts with its initial entries (init_cells[init_off[s] ..
+// init_off[s+1])); a traced slot (bit s of `traced` set, a surviving saddle)
+// grows by the successors of every entry in list order.
+struct DiscTraceDesc {
+  const uint8_t *codes;
+  int64_t pitch;
+  int ox, oy, ex1, ey1;    // ext, global cell coords
+  int rx0, ry0, rx1, ry1;  // interior rect, global cell coords
+  const uint32_t *init_off;    // nslot + 1
+  const uint32_t *init_cells;
+  const uint8_t *traced;       // nslot
+  uint32_t nslot;
+  uint32_t *count;             // nslot: list lengths (count pass)
+  const uint32_t *out_off;     // nslot: list offsets in out (write pass)
+  uint32_t *out;               // the lists (write pass)
+  uint32_t *err;  // 1: more chains than a thread follows; 2: a chain cell with two successors, an
+                  // unpaired successor or a chain longer than the ext; 4: an unpaired head
+};
+constexpr int DISC_TRACE_CHAINS = 16;  // chains one disc may grow from its initial entries
+
 }  // namespace mscx
