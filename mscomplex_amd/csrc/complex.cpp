@@ -144,6 +144,119 @@ void Complex::dump(std::vector<uint32_t> &out) const {
   }
 }
 
+// ---------------------------------------------------------------- spill
+
+// Nodes per record chunk of a spilled complex, and the size from which its
+// chunks are written and read on several host threads.
+static constexpr uint32_t kSpillChunk = 1u << 20;
+
+template <class F>
+static void spill_parallel(int n, F f) {
+  const int nt = std::min(n, host_threads());
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::exception_ptr> errs(nt);
+  auto body = [&](int t) {
+    for (int i = next++; i < n; i = next++) {
+      try {
+        f(i);
+      } catch (...) {
+        if (!errs[t]) errs[t] = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(body, t);
+  body(0);
+  for (auto &x : th) x.join();
+  for (auto &e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
+void spill_complex(Complex &m, ComplexSpill &out) {
+  const uint32_t n = m.size();
+  out.rect = m.rect;
+  out.ext = m.ext;
+  out.n = n;
+  const uint32_t nc = (n + kSpillChunk - 1) / kSpillChunk;
+  out.begin.resize(nc + 1);
+  for (uint32_t c = 0; c <= nc; ++c) out.begin[c] = std::min(n, c * kSpillChunk);
+  out.words.assign(nc, {});
+  spill_parallel((int)nc, [&](int c) {
+    std::vector<uint32_t> &w = out.words[c];
+    size_t len = 0;
+    for (uint32_t i = out.begin[c]; i < out.begin[c + 1]; ++i)
+      len += 8 + m.asc[i].size() + m.des[i].size() + m.asc_disc[i].size() + m.des_disc[i].size();
+    w.reserve(len);
+    for (uint32_t i = out.begin[c]; i < out.begin[c + 1]; ++i) {
+      uint32_t fb;
+      memcpy(&fb, &m.fn[i], 4);
+      const uint32_t rec[8] = {m.cell[i], fb, m.pair[i], m.flags[i], m.asc[i].size(), m.des[i].size(),
+                               (uint32_t)m.asc_disc[i].size(), (uint32_t)m.des_disc[i].size()};
+      w.insert(w.end(), rec, rec + 8);
+      w.insert(w.end(), m.asc[i].begin(), m.asc[i].end());
+      w.insert(w.end(), m.des[i].begin(), m.des[i].end());
+      w.insert(w.end(), m.asc_disc[i].begin(), m.asc_disc[i].end());
+      w.insert(w.end(), m.des_disc[i].begin(), m.des_disc[i].end());
+    }
+  });
+  out.idx = std::move(m.idx);
+  out.xd_off = std::move(m.xd_off);
+  out.xd_cells = std::move(m.xd_cells);
+  out.sd_off = std::move(m.sd_off);
+  out.sd_cells = std::move(m.sd_cells);
+  m = Complex();
+}
+
+void restore_complex(ComplexSpill &in, Complex &m) {
+  Complex c;
+  const uint32_t n = in.n;
+  c.rect = in.rect;
+  c.ext = in.ext;
+  c.reserve_nodes(Complex::with_headroom(n));
+  c.cell.resize(n);
+  c.fn.resize(n);
+  c.pair.resize(n);
+  c.flags.resize(n);
+  c.asc.resize(n);
+  c.des.resize(n);
+  c.asc_disc.resize(n);
+  c.des_disc.resize(n);
+  const int nc = (int)in.words.size();
+  spill_parallel(nc, [&](int k) {
+    const uint32_t *p = in.words[k].data();
+    for (uint32_t i = in.begin[k]; i < in.begin[k + 1]; ++i) {
+      c.cell[i] = p[0];
+      memcpy(&c.fn[i], &p[1], 4);
+      c.pair[i] = p[2];
+      c.flags[i] = (uint8_t)p[3];
+      const uint32_t na = p[4], nd = p[5], nad = p[6], ndd = p[7];
+      p += 8;
+      for (uint32_t e = 0; e < na; ++e) c.asc[i].push_sorted(p[e]);
+      p += na;
+      for (uint32_t e = 0; e < nd; ++e) c.des[i].push_sorted(p[e]);
+      p += nd;
+      if (nad) c.asc_disc[i].assign(p, p + nad);
+      p += nad;
+      if (ndd) c.des_disc[i].assign(p, p + ndd);
+      p += ndd;
+    }
+    std::vector<uint32_t>().swap(in.words[k]);
+  });
+  c.idx = std::move(in.idx);
+  c.xd_off = std::move(in.xd_off);
+  c.xd_cells = std::move(in.xd_cells);
+  c.sd_off = std::move(in.sd_off);
+  c.sd_cells = std::move(in.sd_cells);
+  in.words.clear();
+  in.begin.clear();
+  in.n = 0;
+  m = std::move(c);
+}
+
 // writeout_connectivity (grid_dataset.cpp:1858-1907) and
 // addCriticalPointsToMSComplex (discreteMorseAlgorithm.h:244-268): nodes in
 // raster order; each saddle connects to the owners of its facets (des) and

@@ -295,6 +295,22 @@ void load_complex(const std::string &path, Complex &m);
 // are malformed (`what` names their source in the message).
 void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::string &what);
 
+// A complex set aside by the out-of-core mode: its nodes as records in the
+// dump's per-node format (the discs there are only the per-node vectors) in
+// chunks that are written and read on several threads, its cell map and
+// its disc CSRs moved as they are.  spill_complex leaves m empty;
+// restore_complex rebuilds it exactly and leaves `in` empty.
+struct ComplexSpill {
+  Rect rect{}, ext{};
+  uint32_t n = 0;
+  std::vector<uint32_t> begin;  // first node of each chunk, then n
+  std::vector<std::vector<uint32_t>> words;
+  CellMap idx;
+  std::vector<uint32_t> xd_off, xd_cells, sd_off, sd_cells;
+};
+void spill_complex(Complex &m, ComplexSpill &out);
+void restore_complex(ComplexSpill &in, Complex &m);
+
 }  // namespace mscx
 
 // Opaque C handle of a complex.

@@ -321,7 +321,8 @@ static std::string spill_path(const mscx_grid *G, int i) { return G->spill_dir +
 // directory of archive files there.
 static void init_spill(mscx_grid *G) {
   const size_t n = 2 * G->leaves.size() - 1;
-  G->spilled.assign(n, std::vector<uint32_t>());
+  G->spilled.clear();
+  G->spilled.resize(n);
   G->spilled_file.assign(n, 0);
   const char *dir = getenv("MSCX_SPILL_DIR");
   if (dir && *dir) {
@@ -337,10 +338,9 @@ void spill_piece(mscx_grid *G, int i) {
     save_complex(G->pieces[i]->c, spill_path(G, i));
     G->spilled_file[i] = 1;
   } else {
-    std::vector<uint32_t> w;
-    G->pieces[i]->c.dump(w);
-    w.shrink_to_fit();
-    G->spilled[i].swap(w);
+    std::unique_ptr<ComplexSpill> sp(new ComplexSpill);
+    spill_complex(G->pieces[i]->c, *sp);
+    G->spilled[i] = std::move(sp);
   }
   delete G->pieces[i];
   G->pieces[i] = nullptr;
@@ -355,9 +355,9 @@ mscx_complex *restore_piece(mscx_grid *G, int i) {
     unlink(spill_path(G, i).c_str());
     G->spilled_file[i] = 0;
   } else {
-    const std::vector<uint32_t> &w = G->spilled[i];
-    complex_from_words(w.data(), w.size(), c->c, "spilled piece " + std::to_string(i));
-    std::vector<uint32_t>().swap(G->spilled[i]);
+    MSCX_REQUIRE(G->spilled[i], MSCX_ESTATE, "piece is neither live nor spilled");
+    restore_complex(*G->spilled[i], c->c);
+    G->spilled[i].reset();
   }
   G->pieces[i] = c.release();
   return G->pieces[i];

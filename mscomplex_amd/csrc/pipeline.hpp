@@ -34,7 +34,7 @@ struct mscx_grid {
   // file there) instead of a live complex, and brought back when a phase or
   // a caller needs it; pieces[i] is then NULL.
   bool ooc = false;
-  std::vector<std::vector<uint32_t>> spilled;
+  std::vector<std::unique_ptr<mscx::ComplexSpill>> spilled;
   std::string spill_dir;
   std::vector<char> spilled_file;
   std::mutex spill_mu;  // lazy restores by the accessors
