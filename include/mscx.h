@@ -71,6 +71,13 @@ int mscx_profile_read(double *ms, uint32_t *counts, int n);
 int mscx_set_graphs(int on, int *previous);
 int mscx_graph_stats(uint64_t *builds, uint64_t *replays);
 
+/* Counters of the manifold pass's disc tracing since the library loaded (no
+ * reference counterpart): disc slots traced on the GPU, slots traced on the
+ * host (a disc whose initial entries grow more chains than one GPU thread
+ * follows), and calls whose GPU lists were dropped for the host's because
+ * the chains were not a gradient's V-paths (never expected). */
+int mscx_disc_trace_stats(uint64_t out[3]);
+
 /* Device buffers (no reference counterpart: the reference keeps inputs in
  * host memory).  Lets a caller keep grids resident in HBM and bind tiles to
  * them with mscx_tile_bind_device. */

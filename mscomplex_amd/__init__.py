@@ -115,6 +115,14 @@ def graph_stats():
     return int(c.value), int(r.value)
 
 
+def disc_trace_stats():
+    """(disc slots traced on the GPU, slots traced on the host, calls whose
+    GPU lists were dropped) since the library loaded."""
+    o = np.zeros(3, np.uint64)
+    check(lib().mscx_disc_trace_stats(ptr(o)))
+    return tuple(int(v) for v in o)
+
+
 STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex",
           "disc_prep", "disc_tables", "disc_label_wait_thread_time", "disc_trace", "disc_lists_thread_time",
           "disc_group_thread_time", "disc_group")

@@ -26,6 +26,10 @@ namespace mscx {
 // serving loop's workers) each lease their own set, so their host work runs
 // concurrently; a set is kept for reuse once returned, so steady-state calls
 // allocate nothing.
+// disc slots traced on the GPU and on the host, and calls whose GPU lists
+// were dropped because the chains were no gradient's V-paths
+static std::atomic<uint64_t> g_trace_gpu{0}, g_trace_host{0}, g_trace_dropped{0};
+
 static double now_wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -382,6 +386,9 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   MSCX_HIP(hipStreamSynchronize(s));
   const bool gpu_trace = *terr_h == 0;
   std::vector<size_t> out_base(n + 1, 0);
+  // slots with more chains than a thread follows are traced here: their
+  // lists, by leaf, in slot order
+  std::vector<std::vector<std::pair<uint32_t, std::vector<uint32_t>>>> host_slots(n);
   if (gpu_trace) {
     // each complex's offsets by slot; the device offsets are the same per leaf
     uint32_t *off_h = (uint32_t *)(th + t_off);
@@ -389,10 +396,28 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
       std::vector<uint32_t> &o = cs[i]->sd_off;
       o.assign(nslot[i] + 1, 0);
       const uint32_t *c = cnt_h + slot_base[i];
-      for (size_t k = 0; k < nslot[i]; ++k) o[k + 1] = o[k] + c[k];
+      const DiscPrep &P = dp[i];
+      for (size_t k = 0; k < nslot[i]; ++k) {
+        uint32_t len = c[k];
+        if (len == DISC_TRACE_HOST) {
+          std::vector<uint32_t> lst;
+          disc_trace_one(P.init_cells.data() + P.init_off[k], P.init_off[k + 1] - P.init_off[k], (int)(k & 1),
+                         rect[i], ext[i], cv[i], lst);
+          len = (uint32_t)lst.size();
+          host_slots[i].emplace_back((uint32_t)k, std::move(lst));
+        }
+        o[k + 1] = o[k] + len;
+      }
       memcpy(off_h + slot_base[i], o.data(), 4 * nslot[i]);
     });
     for (int i = 0; i < n; ++i) out_base[i + 1] = out_base[i] + cs[i]->sd_off[nslot[i]];
+    size_t nh = 0;
+    for (int i = 0; i < n; ++i) nh += host_slots[i].size();
+    g_trace_gpu += S - nh;
+    g_trace_host += nh;
+  } else {
+    g_trace_dropped++;
+    g_trace_host += S;
   }
   const size_t NO = out_base[n];
   PoolBlock outblk{device_pool().alloc(std::max<size_t>(4 * NO, 256)), s};
@@ -410,8 +435,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
 
   // 5. the extremum discs (grouping, one item per leaf as its labels
   //    arrive); a label the kernel flagged as bad is the none label, which
-  //    grouping skips.  Without the GPU lists (a disc with more chains than
-  //    a thread follows) every leaf's discs are traced here instead.
+  //    grouping skips.  Without the GPU lists (the chains were no gradient's
+  //    V-paths) every leaf's discs are traced here instead.
   tp = now_wall();
   std::atomic<int64_t> wait_ns(0), trace_ns(0), group_ns(0);
   try {
@@ -439,7 +464,11 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   MSCX_HIP(hipStreamSynchronize(s));
   if (gpu_trace) {
     const double w0 = now_wall();
-    par_for(n, [&](int i) { cs[i]->sd_cells.assign(out_h + out_base[i], out_h + out_base[i + 1]); });
+    par_for(n, [&](int i) {
+      std::vector<uint32_t> &sc = cs[i]->sd_cells;
+      sc.assign(out_h + out_base[i], out_h + out_base[i + 1]);
+      for (const auto &hs : host_slots[i]) std::copy(hs.second.begin(), hs.second.end(), sc.begin() + cs[i]->sd_off[hs.first]);
+    });
     trace_ns += (int64_t)(1e9 * (now_wall() - w0));
   }
   tm.add_host(ST_DISC_LABELS, 1e-9 * (double)wait_ns.load());
@@ -450,3 +479,12 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
 }
 
 }  // namespace mscx
+
+extern "C" int mscx_disc_trace_stats(uint64_t out[3]) {
+  return mscx::guarded([&] {
+    MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
+    out[0] = mscx::g_trace_gpu.load();
+    out[1] = mscx::g_trace_host.load();
+    out[2] = mscx::g_trace_dropped.load();
+  });
+}

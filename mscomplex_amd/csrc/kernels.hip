@@ -898,7 +898,8 @@ __global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restr
     const int ns = dt_succ(D, codes, c, j, succ);
     for (int q = 0; q < ns; ++q) {
       if (r == DISC_TRACE_CHAINS) {
-        atomicOr(D.err, 1u);
+        // more chains than a thread follows: the host traces this slot
+        if (!WRITE) D.count[s] = DISC_TRACE_HOST;
         return;
       }
       cur[r++] = succ[q];

@@ -151,9 +151,12 @@ struct DiscTraceDesc {
   uint32_t *count;             // nslot: list lengths (count pass)
   const uint32_t *out_off;     // nslot: list offsets in out (write pass)
   uint32_t *out;               // the lists (write pass)
-  uint32_t *err;  // 1: more chains than a thread follows; 2: a chain cell with two successors, an
-                  // unpaired successor or a chain longer than the ext; 4: an unpaired head
+  uint32_t *err;  // 2: a chain cell with two successors, an unpaired successor or a chain
+                  // longer than the ext; 4: an unpaired head (not a gradient's V-paths)
 };
-constexpr int DISC_TRACE_CHAINS = 16;  // chains one disc may grow from its initial entries
+constexpr int DISC_TRACE_CHAINS = 16;  // chains a thread follows for one disc
+// count of a slot whose initial entries grow more chains than that: the host
+// traces it, the write pass leaves its range alone
+constexpr uint32_t DISC_TRACE_HOST = 0xFFFFFFFFu;
 
 }  // namespace mscx

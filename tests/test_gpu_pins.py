@@ -93,3 +93,19 @@ def test_pipeline_pins(case):
         print(f"[pins] (ii)-(iv) {label} {tiling} threshold={th}", flush=True)
         h = check_run(mx.GridRun(data, threshold=th, **tiling), f"{label}@{th}")
         assert h["saddle"] > 0
+
+
+def test_discs_are_traced_on_the_gpu():
+    """The saddle 1-manifolds of the manifold pass come from the GPU tracing
+    (k_disc_trace), not from the host fallback, and equal the oracle's."""
+    import oracle
+    g = mx.gen_value_noise(600, 520, seed=13, period=150)
+    before = mx.disc_trace_stats()
+    run = mx.GridRun(g, lx=1, ly=1, threshold=0.01)
+    after = mx.disc_trace_stats()
+    gpu, host, dropped = (a - b for a, b in zip(after, before))
+    print(f"[pins] disc slots: {gpu} on the GPU, {host} on the host, {dropped} calls dropped", flush=True)
+    assert dropped == 0 and gpu > 0 and gpu > 10 * host
+    orc = oracle.Pipeline(g, lx=1, ly=1, threshold=0.01)
+    for i in range(run.num_pieces()):
+        assert np.array_equal(run.piece(i).dump(), orc.dump(4, i)), i
