@@ -213,6 +213,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
 
   // 1. gradient codes to pinned host memory (needed by the critical-point
   //    pass and the saddle tracing)
+  StageTimer &tm = stage_timer();
+  double tp = now_wall();
   std::vector<DiscPrep> dp(n);
   // sizes of the tables are known only after the critical-point pass, so
   // the codes live in their own region at the start of the host scratch
@@ -228,8 +230,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   for (int i = 0; i < n; ++i)
     cv[i] = CodesView{(const uint8_t *)host + code_off[i], tiles[i]->d.pitch, ext[i].x0, ext[i].y0};
 
-  StageTimer &tm = stage_timer();
-  double tp = now_wall();
+  tm.add_host(ST_DISC_CODES, now_wall() - tp);
+  tp = now_wall();
   // 2. critical-point pass (seeds, redirects, surviving maps)
   std::vector<size_t> nk(n);
   for (int i = 0; i < n; ++i) nk[i] = tiles[i]->K;
@@ -242,86 +244,17 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
 
   tm.add_host(ST_DISC_PREP, now_wall() - tp);
-  tp = now_wall();
-  // 3. label tables per critical index (host) + label pass (device)
-  std::vector<size_t> tab_off(n + 1, 0);
-  for (int i = 0; i < n; ++i) tab_off[i + 1] = tab_off[i] + align_up(2 * (size_t)tiles[i]->K * 4, 256);
-  const size_t o_desc = 0, o_pref = align_up(sizeof(DiscDesc) * n, 256);
-  const size_t o_err = align_up(o_pref + 4 * (n + 1), 256);
-  const size_t o_tab = o_err + 256;
-  const size_t o_lab = o_tab + tab_off[n];
-  const size_t dev_bytes = o_lab + lab_total;
-  // device tables and labels come from the library's pool (cached between
-  // calls, counted against a server's HBM budget while in use); the block
-  // goes back once the auxiliary stream has finished with it
-  PoolBlock devblk{device_pool().alloc(dev_bytes), s};
-  char *dev = (char *)devblk.p;
-  char *stage = (char *)g_stage.reserve_host(o_lab);
-  DiscDesc *hd = (DiscDesc *)(stage + o_desc);
-  {
-    // entries of a leaf's tables are independent: large leaves are split
-    const std::vector<Chunk> items = chunks_of(nk, 1 << 16);
-    par_for((int)items.size(), [&](int q) {
-      const int i = items[q].leaf;
-      const uint32_t K = tiles[i]->K;
-      uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
-      disc_label_tables(*cs[i], K, rect[i], dp[i], tiles[i], owned, owned + K, (uint32_t)items[q].begin,
-                        (uint32_t)items[q].end);
-    });
-  }
-  tm.add_host(ST_DISC_TABLES, now_wall() - tp);
-  size_t lo = 0;
-  std::vector<size_t> lab_off(n);
-  for (int i = 0; i < n; ++i) {
-    const mscx_tile *t = tiles[i];
-    DiscDesc &D = hd[i];
-    D.own = t->d.own;
-    D.own_pitch = t->d.own_pitch;
-    D.codes = t->d.codes;
-    D.pitch = t->d.pitch;
-    D.ox = t->ext[0];
-    D.oy = t->ext[1];
-    D.rx0 = rect[i].x0;
-    D.ry0 = rect[i].y0;
-    D.rx1 = rect[i].x1;
-    D.ry1 = rect[i].y1;
-    D.lab_owned = (const uint32_t *)(dev + o_tab + tab_off[i]);
-    D.lab_self = D.lab_owned + t->K;
-    D.nlab = t->K;
-    lab_off[i] = lo;
-    D.lab_v = (uint32_t *)(dev + o_lab + lo);
-    D.lab_q = D.lab_v + nv[i];
-    lo += align_up((nv[i] + nq[i]) * 4, 256);
-    D.err = (uint32_t *)(dev + o_err);
-  }
-  int nblocks = plan_disc_blocks(hd, n, (int *)(stage + o_pref));
-  *(uint32_t *)(stage + o_err) = 0;
-  MSCX_HIP(hipMemcpyAsync(dev, stage, o_lab, hipMemcpyHostToDevice, s));
-  launch_disc_labels((const DiscDesc *)dev, (const int *)(dev + o_pref), n, nblocks, s);
-  MSCX_HIP(hipGetLastError());
-  // the labels come back leaf by leaf, each followed by an event, so that a
-  // leaf's grouping starts as soon as its own labels have arrived
-  char *labels = host + codes_bytes;
-  struct Events {
-    std::vector<hipEvent_t> e;
-    ~Events() {
-      for (hipEvent_t x : e) (void)hipEventDestroy(x);
-    }
-  } ev;
-  ev.e.assign(n, nullptr);
-  for (int i = 0; i < n; ++i) MSCX_HIP(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming));
-  for (int i = 0; i < n; ++i) {
-    MSCX_HIP(hipMemcpyAsync(labels + lab_off[i], dev + o_lab + lab_off[i], (nv[i] + nq[i]) * 4,
-                            hipMemcpyDeviceToHost, s));
-    MSCX_HIP(hipEventRecord(ev.e[i], s));
-  }
-  uint32_t *herr = (uint32_t *)(stage + o_err);
-  MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
-
-  // 4. disc tracing on the GPU (the saddles' 1-manifolds; every slot's list
-  //    counted, the counts scanned here into each complex's offsets, then
-  //    written), overlapping the extremum discs' grouping on the host as
-  //    each leaf's labels arrive
+  // the event after the trace counts
+  struct OneEvent {
+    hipEvent_t e = nullptr;
+    OneEvent() { MSCX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    ~OneEvent() { (void)hipEventDestroy(e); }
+  } ev_cnt;
+  // 3. disc tracing on the GPU (the saddles' 1-manifolds): every slot's
+  //    list is counted first, while the host builds the label tables; the
+  //    counts are scanned into each complex's offsets once the labels are
+  //    under way, and the lists are written while the host groups the
+  //    extremum discs as each leaf's labels arrive
   tp = now_wall();
   std::vector<size_t> nslot(n), slot_base(n + 1, 0), init_base(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -383,10 +316,101 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   uint32_t *cnt_h = (uint32_t *)(th + t_cnt);
   MSCX_HIP(hipMemcpyAsync(cnt_h, tdev + t_cnt, 4 * S, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipMemcpyAsync(terr_h, tdev + t_err, 4, hipMemcpyDeviceToHost, s));
-  MSCX_HIP(hipStreamSynchronize(s));
+  MSCX_HIP(hipEventRecord(ev_cnt.e, s));
+  double paths_s = now_wall() - tp;
+  tp = now_wall();
+  // 4. label tables per critical index (host) + label pass (device)
+  std::vector<size_t> tab_off(n + 1, 0);
+  for (int i = 0; i < n; ++i) tab_off[i + 1] = tab_off[i] + align_up(2 * (size_t)tiles[i]->K * 4, 256);
+  const size_t o_desc = 0, o_pref = align_up(sizeof(DiscDesc) * n, 256);
+  const size_t o_err = align_up(o_pref + 4 * (n + 1), 256);
+  const size_t o_tab = o_err + 256;
+  const size_t o_lab = o_tab + tab_off[n];
+  const size_t dev_bytes = o_lab + lab_total;
+  // device tables and labels come from the library's pool (cached between
+  // calls, counted against a server's HBM budget while in use); the block
+  // goes back once the auxiliary stream has finished with it
+  PoolBlock devblk{device_pool().alloc(dev_bytes), s};
+  char *dev = (char *)devblk.p;
+  char *stage = (char *)g_stage.reserve_host(o_lab);
+  DiscDesc *hd = (DiscDesc *)(stage + o_desc);
+  {
+    // entries of a leaf's tables are independent: large leaves are split
+    const std::vector<Chunk> items = chunks_of(nk, 1 << 16);
+    par_for((int)items.size(), [&](int q) {
+      const int i = items[q].leaf;
+      const uint32_t K = tiles[i]->K;
+      uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
+      disc_label_tables(*cs[i], K, rect[i], dp[i], tiles[i], owned, owned + K, (uint32_t)items[q].begin,
+                        (uint32_t)items[q].end);
+    });
+  }
+  tm.add_host(ST_DISC_TABLES, now_wall() - tp);
+  size_t lo = 0;
+  std::vector<size_t> lab_off(n);
+  for (int i = 0; i < n; ++i) {
+    const mscx_tile *t = tiles[i];
+    DiscDesc &D = hd[i];
+    D.own = t->d.own;
+    D.own_pitch = t->d.own_pitch;
+    D.codes = t->d.codes;
+    D.pitch = t->d.pitch;
+    D.ox = t->ext[0];
+    D.oy = t->ext[1];
+    D.rx0 = rect[i].x0;
+    D.ry0 = rect[i].y0;
+    D.rx1 = rect[i].x1;
+    D.ry1 = rect[i].y1;
+    D.lab_owned = (const uint32_t *)(dev + o_tab + tab_off[i]);
+    D.lab_self = D.lab_owned + t->K;
+    D.nlab = t->K;
+    lab_off[i] = lo;
+    D.lab_v = (uint32_t *)(dev + o_lab + lo);
+    D.lab_q = D.lab_v + nv[i];
+    lo += align_up((nv[i] + nq[i]) * 4, 256);
+    D.err = (uint32_t *)(dev + o_err);
+  }
+  int nblocks = plan_disc_blocks(hd, n, (int *)(stage + o_pref));
+  *(uint32_t *)(stage + o_err) = 0;
+  MSCX_HIP(hipMemcpyAsync(dev, stage, o_lab, hipMemcpyHostToDevice, s));
+  launch_disc_labels((const DiscDesc *)dev, (const int *)(dev + o_pref), n, nblocks, s);
+  MSCX_HIP(hipGetLastError());
+  // the labels come back in runs of leaves, each followed by an event, so
+  // that a leaf's grouping starts as soon as its own labels have arrived
+  char *labels = host + codes_bytes;
+  struct Events {
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    }
+  } ev;
+  // Consecutive leaves' labels are adjacent on both sides, so they travel
+  // in runs of at least LAB_RUN bytes (a copy has a fixed cost of the order
+  // of 0.1 ms here): one copy and one event per run.
+  constexpr size_t LAB_RUN = 8u << 20;
+  std::vector<int> run_of(n);
+  std::vector<std::pair<int, int>> runs;  // leaves [first, last]
+  for (int i = 0; i < n; ++i) {
+    if (runs.empty() || lab_off[i] - lab_off[runs.back().first] >= LAB_RUN) runs.push_back({i, i});
+    runs.back().second = i;
+    run_of[i] = (int)runs.size() - 1;
+  }
+  ev.e.assign(runs.size(), nullptr);
+  for (auto &e : ev.e) MSCX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (size_t r = 0; r < runs.size(); ++r) {
+    const int a = runs[r].first, b = runs[r].second;
+    MSCX_HIP(hipMemcpyAsync(labels + lab_off[a], dev + o_lab + lab_off[a],
+                            lab_off[b] - lab_off[a] + (nv[b] + nq[b]) * 4, hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipEventRecord(ev.e[r], s));
+  }
+  uint32_t *herr = (uint32_t *)(stage + o_err);
+  MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
+
+  tp = now_wall();
+  MSCX_HIP(hipEventSynchronize(ev_cnt.e));
   const bool gpu_trace = *terr_h == 0;
   std::vector<size_t> out_base(n + 1, 0);
-  // slots with more chains than a thread follows are traced here: their
+  // slots with more chains than a wave follows are traced here: their
   // lists, by leaf, in slot order
   std::vector<std::vector<std::pair<uint32_t, std::vector<uint32_t>>>> host_slots(n);
   if (gpu_trace) {
@@ -399,6 +423,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
       const DiscPrep &P = dp[i];
       for (size_t k = 0; k < nslot[i]; ++k) {
         uint32_t len = c[k];
+        if (len == DISC_TRACE_WIDE) throw std::logic_error("disc slot left unmeasured by the wide tracing");
+        if (len != DISC_TRACE_HOST && len >= DISC_TRACE_WIDE_LEN) len -= DISC_TRACE_WIDE_LEN;
         if (len == DISC_TRACE_HOST) {
           std::vector<uint32_t> lst;
           disc_trace_one(P.init_cells.data() + P.init_off[k], P.init_off[k + 1] - P.init_off[k], (int)(k & 1),
@@ -431,7 +457,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     out_h = (uint32_t *)lease.s->lists.reserve_host(std::max<size_t>(4 * NO, 4));
     if (NO) MSCX_HIP(hipMemcpyAsync(out_h, outblk.p, 4 * NO, hipMemcpyDeviceToHost, s));
   }
-  tm.add_host(ST_DISC_PATHS, now_wall() - tp);
+  tm.add_host(ST_DISC_PATHS, paths_s + now_wall() - tp);
 
   // 5. the extremum discs (grouping, one item per leaf as its labels
   //    arrive); a label the kernel flagged as bad is the none label, which
@@ -450,7 +476,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
         return;
       }
       const int i = q;
-      MSCX_HIP(hipEventSynchronize(ev.e[i]));
+      MSCX_HIP(hipEventSynchronize(ev.e[run_of[i]]));
       const double w1 = now_wall();
       wait_ns += (int64_t)(1e9 * (w1 - w0));
       const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);

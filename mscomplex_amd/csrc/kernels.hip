@@ -878,6 +878,8 @@ __global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restr
   const GLB uint8_t *codes = glb(D.codes);
   const uint32_t i0 = D.init_off[s], ni = D.init_off[s + 1] - i0;
   const uint32_t *init = D.init_cells + i0;
+  // the write pass leaves the wide and host-traced slots to their tracers
+  if (WRITE && D.count[s] >= DISC_TRACE_WIDE_LEN) return;
   GLB uint32_t *out = WRITE ? glb(D.out) + D.out_off[s] : nullptr;
   uint32_t len = 0;
   if (!D.traced[s] || ni == 0) {
@@ -898,8 +900,8 @@ __global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restr
     const int ns = dt_succ(D, codes, c, j, succ);
     for (int q = 0; q < ns; ++q) {
       if (r == DISC_TRACE_CHAINS) {
-        // more chains than a thread follows: the host traces this slot
-        if (!WRITE) D.count[s] = DISC_TRACE_HOST;
+        // more chains than a thread follows: a wave traces this slot
+        if (!WRITE) D.count[s] = DISC_TRACE_WIDE;
         return;
       }
       cur[r++] = succ[q];
@@ -934,6 +936,112 @@ __global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restr
   if (!WRITE) D.count[s] = len;
 }
 
+// The wide slots (more chains than a thread follows): the block's slots are
+// those of k_disc_trace's block; each wave takes the wide slots among its 64
+// one after another, with a lane per chain.  The list grows exactly as in
+// k_disc_trace: the initial entries, then round by round every live chain's
+// current cell in chain order; a round's positions are the ranks of the live
+// chains (ballot and mbcnt over each group of 64 chains).  The chains' cells
+// live in LDS (DISC_TRACE_WIDE_CAP per wave).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_disc_trace_wide(const DiscTraceDesc *__restrict__ descs,
+                                                         const int *__restrict__ prefix, int n) {
+  constexpr int CAP = DISC_TRACE_WIDE_CAP;
+  __shared__ uint32_t chains_all[4][CAP];
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, n, b);
+  const DiscTraceDesc &D = descs[t];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t *chains = chains_all[wv];
+  const uint32_t s0 = (uint32_t)(b - prefix[t]) * 256u + (uint32_t)(wv * 64);
+  const uint32_t my = s0 + (uint32_t)lane;
+  const uint32_t mc = my < D.nslot ? D.count[my] : 0u;
+  // count pass: the slots the thread pass marked; write pass: the slots the
+  // wave's count pass measured
+  const bool wide = WRITE ? (mc >= DISC_TRACE_WIDE_LEN && mc < DISC_TRACE_WIDE) : (mc == DISC_TRACE_WIDE);
+  uint64_t todo = __ballot(wide);
+  const GLB uint8_t *codes = glb(D.codes);
+  const uint32_t limit = (uint32_t)(D.ex1 - D.ox + 1) * (uint32_t)(D.ey1 - D.oy + 1);
+  while (todo) {
+    // the chain table is the wave's own; the LDS accesses of one wave are
+    // performed in order, so it is enough that the compiler keeps them so
+    __builtin_amdgcn_wave_barrier();
+    const int bit = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const uint32_t s = s0 + (uint32_t)bit;
+    const uint32_t i0 = D.init_off[s], ni = D.init_off[s + 1] - i0;
+    const uint32_t *init = D.init_cells + i0;
+    GLB uint32_t *out = WRITE ? glb(D.out) + D.out_off[s] : nullptr;
+    const int j = (int)(s & 1u);
+    uint32_t succ[4];
+    // the initial entries and their successors (the chains' heads, in the
+    // order of their entries, then of the successors)
+    uint32_t r = 0;
+    bool over = false, bad_head = false;
+    for (uint32_t k0 = 0; k0 < ni; k0 += 64) {
+      const uint32_t k = k0 + (uint32_t)lane;
+      const uint32_t c = k < ni ? init[k] : 0xFFFFFFFFu;
+      if (WRITE && k < ni) out[k] = c;
+      const int ns = k < ni ? dt_succ(D, codes, c, j, succ) : 0;
+      // exclusive prefix of ns over the lanes
+      int incl = ns;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+      }
+      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+      if (r + total > (uint32_t)CAP) {  // uniform
+        over = true;
+        break;
+      }
+      const uint32_t at = r + (uint32_t)(incl - ns);
+      for (int q = 0; q < ns; ++q) {
+        chains[at + q] = succ[q];
+        bad_head |= succ[q] == 0xFFFFFFFFu;
+      }
+      r += total;
+    }
+    if (over) {
+      if (!WRITE && lane == 0) D.count[s] = DISC_TRACE_HOST;
+      continue;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (__ballot(bad_head)) {  // an unpaired successor: not a gradient the chains can follow
+      if (lane == 0) atomicOr(D.err, 4u);
+      return;
+    }
+    // rounds: every live chain emits its cell and steps to its successor
+    // (0xFFFFFFFF marks an ended chain)
+    uint32_t len = ni;
+    bool live = true, bad = false;
+    while (live && !bad) {
+      live = false;
+      for (uint32_t g = 0; g < r; g += 64) {
+        const uint32_t k = g + (uint32_t)lane;
+        const uint32_t c = k < r ? chains[k] : 0xFFFFFFFFu;
+        const bool alive = c != 0xFFFFFFFFu;
+        const uint64_t bal = __ballot(alive);
+        if (!bal) continue;
+        live = true;
+        if (WRITE && alive) out[rank_below(bal, len)] = c;
+        len += (uint32_t)__popcll(bal);
+        if (alive) {
+          const int ns = dt_succ(D, codes, c, j, succ);
+          bad |= ns > 1 || (ns == 1 && succ[0] == 0xFFFFFFFFu);
+          chains[k] = ns ? succ[0] : 0xFFFFFFFFu;
+        }
+      }
+      bad = __ballot(bad) != 0 || len > limit + ni;
+    }
+    if (bad) {
+      if (lane == 0) atomicOr(D.err, 2u);
+      return;
+    }
+    if (!WRITE && lane == 0) D.count[s] = DISC_TRACE_WIDE_LEN | len;
+  }
+}
+
 int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix) {
   int acc = 0;
   for (int i = 0; i < n; ++i) {
@@ -947,10 +1055,13 @@ int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix) {
 void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, int nblocks, bool write,
                        hipStream_t s) {
   if (!nblocks) return;
-  if (write)
+  if (write) {
     hipLaunchKernelGGL(k_disc_trace<true>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
-  else
+    hipLaunchKernelGGL(k_disc_trace_wide<true>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+  } else {
     hipLaunchKernelGGL(k_disc_trace<false>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+    hipLaunchKernelGGL(k_disc_trace_wide<false>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+  }
 }
 
 // -------------------------------------------------------------- launchers

@@ -148,15 +148,22 @@ struct DiscTraceDesc {
   const uint32_t *init_cells;
   const uint8_t *traced;       // nslot
   uint32_t nslot;
-  uint32_t *count;             // nslot: list lengths (count pass)
+  uint32_t *count;             // nslot: list lengths (count pass; see DISC_TRACE_*)
   const uint32_t *out_off;     // nslot: list offsets in out (write pass)
   uint32_t *out;               // the lists (write pass)
   uint32_t *err;  // 2: a chain cell with two successors, an unpaired successor or a chain
                   // longer than the ext; 4: an unpaired head (not a gradient's V-paths)
 };
 constexpr int DISC_TRACE_CHAINS = 16;  // chains a thread follows for one disc
-// count of a slot whose initial entries grow more chains than that: the host
-// traces it, the write pass leaves its range alone
+// A slot whose initial entries grow more chains than that is traced by a
+// whole wave (a lane per chain, up to DISC_TRACE_WIDE_CAP chains): the
+// thread's count pass marks it DISC_TRACE_WIDE, the wave's count pass
+// replaces the mark by DISC_TRACE_WIDE_LEN | length, or by DISC_TRACE_HOST
+// when even the wave's chain table is too small (the host traces that slot
+// and the write passes leave its range alone).
+constexpr int DISC_TRACE_WIDE_CAP = 1024;
+constexpr uint32_t DISC_TRACE_WIDE_LEN = 0x80000000u;
+constexpr uint32_t DISC_TRACE_WIDE = 0xFFFFFFFEu;
 constexpr uint32_t DISC_TRACE_HOST = 0xFFFFFFFFu;
 
 }  // namespace mscx
