@@ -78,6 +78,11 @@ int mscx_graph_stats(uint64_t *builds, uint64_t *replays);
  * follows), and calls whose GPU lists were dropped for the host's because
  * the chains were not a gradient's V-paths (never expected). */
 int mscx_disc_trace_stats(uint64_t out[3]);
+/* Counters of the device merge tree since the library loaded (no reference
+ * counterpart): merges run on the GPU (up and down), grids whose tree went
+ * back to the host path (a case the kernels do not follow), and host
+ * passes over a shared line. */
+int mscx_merge_stats(uint64_t out[3]);
 
 /* Device buffers (no reference counterpart: the reference keeps inputs in
  * host memory).  Lets a caller keep grids resident in HBM and bind tiles to
@@ -228,6 +233,18 @@ int mscx_complex_destroy(mscx_complex *c);
  * (a 16384^2 white-noise grid's tree holds ~228 M critical points per
  * level).  Results are the same as without the flag. */
 #define MSCX_RUN_OUT_OF_CORE 4
+/* The merge tree on the GPU: the pieces' complexes stay in HBM, merge_up
+ * and merge_down (grid_mscomplex.cpp:24-322) run as kernels (the shared
+ * line's cancellations on the host over the line's gathered sub-complex),
+ * the root comes to the host for the sequential simplification, and a
+ * piece is copied to the host when a phase or mscx_grid_piece needs it.
+ * Results are the same as without the flag (a merge the kernels do not
+ * follow sends the whole tree to the host path).  mscx_grid_run and
+ * mscx_grid_run_file set it by themselves for grids of at least
+ * MSCX_DEVICE_MERGE_MIN_VERTICES vertices (environment; default 2^24);
+ * MSCX_RUN_HOST_MERGE keeps the tree on the host. */
+#define MSCX_RUN_DEVICE_MERGE 8
+#define MSCX_RUN_HOST_MERGE 16
 int mscx_grid_run(const float *data, int32_t size_x, int32_t size_y, int32_t levels, int32_t lx, int32_t ly,
                   double threshold, uint32_t flags, mscx_grid **out);
 /* The whole reference driver over a raw grid file: GridDataManager(filename,

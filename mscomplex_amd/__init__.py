@@ -123,6 +123,14 @@ def disc_trace_stats():
     return tuple(int(v) for v in o)
 
 
+def merge_stats():
+    """(merges run on the GPU, grids whose merge tree fell back to the host
+    path, host passes over shared lines) since the library loaded."""
+    o = np.zeros(3, np.uint64)
+    check(lib().mscx_merge_stats(ptr(o)))
+    return tuple(int(v) for v in o)
+
+
 STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex",
           "disc_prep", "disc_tables", "disc_label_wait_thread_time", "disc_trace", "disc_lists_thread_time",
           "disc_group_thread_time", "disc_group", "disc_codes")
@@ -486,13 +494,30 @@ class Server:
         return {k: int(v) for k, v in zip(keys, o)}
 
 
+RUN_DEVICE_MERGE, RUN_HOST_MERGE = 8, 16
+
+
+def _merge_flags(merge):
+    if merge is None:
+        return 0
+    if merge == "device":
+        return RUN_DEVICE_MERGE
+    if merge == "host":
+        return RUN_HOST_MERGE
+    raise ValueError("merge must be None, 'device' or 'host'")
+
+
 class GridRun:
     """A whole GridDataManager compute run over one grid (mscx_grid_run)."""
 
     def __init__(self, data, levels=1, lx=0, ly=0, threshold=0.0, discs=True, device_ptr=None, size=None,
-                 out_of_core=False):
+                 out_of_core=False, merge=None):
+        """merge: None lets the library choose where the merge tree runs,
+        "device" keeps the pieces' complexes in HBM and merges them with
+        the GPU kernels, "host" merges host complexes (the results are the
+        same)."""
         h = ctypes.c_void_p()
-        flags = (0 if discs else 2) | (4 if out_of_core else 0)
+        flags = (0 if discs else 2) | (4 if out_of_core else 0) | _merge_flags(merge)
         if device_ptr is not None:
             sx, sy = size
             flags |= 1
@@ -506,7 +531,7 @@ class GridRun:
 
     @classmethod
     def from_file(cls, path, size, levels=1, lx=0, ly=0, threshold=0.0, num_parallel=0, discs=True,
-                  disc_prefix=None, out_of_core=False):
+                  disc_prefix=None, out_of_core=False, merge=None):
         """The reference driver over a raw float32 grid file (x fastest) of
         size (sx, sy) (mscx_grid_run_file): GridDataManager(filename, sx, sy,
         levels, ..., simp_tresh, ..., num_parallel).  disc_prefix, when
@@ -517,7 +542,8 @@ class GridRun:
         h = ctypes.c_void_p()
         pre = None if disc_prefix is None else os.fsencode(disc_prefix)
         check(lib().mscx_grid_run_file(os.fsencode(path), sx, sy, levels, lx, ly, float(threshold), int(num_parallel),
-                                       (0 if discs else 2) | (4 if out_of_core else 0), pre, ctypes.byref(h)))
+                                       (0 if discs else 2) | (4 if out_of_core else 0) | _merge_flags(merge),
+                                       pre, ctypes.byref(h)))
         self = cls.__new__(cls)
         self._h = h
         return self

@@ -30,6 +30,7 @@ _SIGS = {
     "mscx_set_graphs": (c_int, [c_int, P]),
     "mscx_graph_stats": (c_int, [P, P]),
     "mscx_disc_trace_stats": (c_int, [P]),
+    "mscx_merge_stats": (c_int, [P]),
     "mscx_dev_alloc": (c_int, [c_size_t, ctypes.POINTER(P)]),
     "mscx_dev_free": (c_int, [P]),
     "mscx_dev_upload": (c_int, [P, P, c_size_t]),

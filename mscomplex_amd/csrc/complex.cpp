@@ -431,10 +431,12 @@ void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2) {
   m.flags[cp2] &= (uint8_t)~CPF_CANCELLED;
 }
 
-static void merge_ixn(const Complex &a, const Complex &b, Rect &ixn) {
+void merge_ixn(const Complex &a, const Complex &b, Rect &ixn) { merge_ixn(a.rect, b.rect, ixn); }
+
+void merge_ixn(const Rect &ra, const Rect &rb, Rect &ixn) {
   // rect_t::intersection (rectangle_complex.h:208-220), called as
   // msc2.m_rect.intersection(msc1.m_rect)
-  const Rect &r1 = b.rect, &r2 = a.rect;
+  const Rect &r1 = rb, &r2 = ra;
   bool inter = !(r1.x1 < r2.x0 || r2.x1 < r1.x0 || r1.y1 < r2.y0 || r2.y1 < r1.y0);
   if (!inter) throw std::logic_error("rects should intersect for merge");
   ixn = Rect{std::max(r2.x0, r1.x0), std::max(r2.y0, r1.y0), std::min(r2.x1, r1.x1), std::min(r2.y1, r1.y1)};
@@ -453,19 +455,25 @@ static Rect rect_of_points(int ax, int ay, int bx, int by) {
   return Rect{std::min(ax, bx), std::min(ay, by), std::max(ax, bx), std::max(ay, by)};
 }
 
+// The union's rect and ext (grid_mscomplex.cpp:44-50): rect_t of the
+// lexicographic min of the bottom-left corners and max of the top-right ones.
+void merge_rects(const Rect &r1, const Rect &e1, const Rect &r2, const Rect &e2, Rect &r, Rect &e) {
+  int bx, by, tx, ty;
+  pmin(r1.x0, r1.y0, r2.x0, r2.y0, bx, by);
+  pmax(r1.x1, r1.y1, r2.x1, r2.y1, tx, ty);
+  r = rect_of_points(bx, by, tx, ty);
+  pmin(e1.x0, e1.y0, e2.x0, e2.y0, bx, by);
+  pmax(e1.x1, e1.y1, e2.x1, e2.y1, tx, ty);
+  e = rect_of_points(bx, by, tx, ty);
+}
+
 // merge_up (grid_mscomplex.cpp:24-156)
 void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   Rect ixn;
   merge_ixn(m1, m2, ixn);
-  int bx, by, tx, ty;
-  pmin(m1.rect.x0, m1.rect.y0, m2.rect.x0, m2.rect.y0, bx, by);
-  pmax(m1.rect.x1, m1.rect.y1, m2.rect.x1, m2.rect.y1, tx, ty);
   res = Complex();
   Complex *out = &res;
-  out->rect = rect_of_points(bx, by, tx, ty);
-  pmin(m1.ext.x0, m1.ext.y0, m2.ext.x0, m2.ext.y0, bx, by);
-  pmax(m1.ext.x1, m1.ext.y1, m2.ext.x1, m2.ext.y1, tx, ty);
-  out->ext = rect_of_points(bx, by, tx, ty);
+  merge_rects(m1.rect, m1.ext, m2.rect, m2.ext, out->rect, out->ext);
   const Complex *arr[2] = {&m1, &m2};
 
   // The reference walks msc1's then msc2's critical points, replicating
@@ -598,25 +606,29 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
       if (!(m.flags[j] & CPF_CANCELLED) && (m.flags[j] & CPF_BCANC))
         out->pair[ai[j]] = out->idx.get_or_zero(m.cell[m.pair[j]]);
   }
+  merge_up_line(*out, ixn);
+}
+
+// The shared-line cancellations of merge_up (grid_mscomplex.cpp:124-153).
+void merge_up_line(Complex &m, const Rect &ixn) {
   for (int y = ixn.y0; y <= ixn.y1; ++y) {
     for (int x = ixn.x0; x <= ixn.x1; ++x) {
       uint32_t c = (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16);
-      uint32_t si = out->idx.find(c);
+      uint32_t si = m.idx.find(c);
       if (si == UINT32_MAX) throw std::logic_error("missing common bndry cp");
-      if ((out->flags[si] & CPF_CANCELLED) || !(out->flags[si] & CPF_BCANC)) continue;
-      uint32_t pi = out->pair[si];
-      uint32_t p = out->cell[pi];
-      if (!out->rect.interior(x, y) && !out->ext.on_boundary(x, y)) continue;
-      if (!out->rect.interior(cx_of(p), cy_of(p)) && !out->ext.on_boundary(cx_of(p), cy_of(p))) continue;
-      cancel_pairs(*out, si, pi, nullptr);
+      if ((m.flags[si] & CPF_CANCELLED) || !(m.flags[si] & CPF_BCANC)) continue;
+      uint32_t pi = m.pair[si];
+      uint32_t p = m.cell[pi];
+      if (!m.rect.interior(x, y) && !m.ext.on_boundary(x, y)) continue;
+      if (!m.rect.interior(cx_of(p), cy_of(p)) && !m.ext.on_boundary(cx_of(p), cy_of(p))) continue;
+      cancel_pairs(m, si, pi, nullptr);
     }
   }
 }
 
-// merge_down (grid_mscomplex.cpp:158-322)
-void merge_down(Complex &P, Complex &m1, Complex &m2) {
-  Rect ixn;
-  merge_ixn(m1, m2, ixn);
+// The shared-line uncancellations of merge_down (grid_mscomplex.cpp:172-202),
+// in reverse raster order.
+void merge_down_line(Complex &P, const Rect &ixn) {
   for (int y = ixn.y1; y >= ixn.y0; --y) {
     for (int x = ixn.x1; x >= ixn.x0; --x) {
       uint32_t c = (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16);
@@ -630,6 +642,13 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
       uncancel_pairs(P, si, pi);
     }
   }
+}
+
+// merge_down (grid_mscomplex.cpp:158-322)
+void merge_down(Complex &P, Complex &m1, Complex &m2) {
+  Rect ixn;
+  merge_ixn(m1, m2, ixn);
+  merge_down_line(P, ixn);
   // The two children are updated from P (which is only read from here on)
   // independently of each other: large ones on two threads.  Within a child,
   // a first pass follows the reference's walk over the boundary-cancellable

@@ -240,6 +240,14 @@ void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> 
 void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2);
 void merge_up(const Complex &a, const Complex &b, Complex &out);
 void merge_down(Complex &parent, Complex &a, Complex &b);
+// Parts of the merges: the shared line of a and b (rect_t::intersection
+// with the reference's checks), the line's cancellations after merge_up's
+// union and its uncancellations before merge_down's copies.
+void merge_ixn(const Complex &a, const Complex &b, Rect &ixn);
+void merge_ixn(const Rect &ra, const Rect &rb, Rect &ixn);
+void merge_rects(const Rect &r1, const Rect &e1, const Rect &r2, const Rect &e2, Rect &r, Rect &e);
+void merge_up_line(Complex &m, const Rect &ixn);
+void merge_down_line(Complex &P, const Rect &ixn);
 // work_limit > 0 bounds the work of one call in proportion to the complex
 // (no counterpart in the reference, which runs until done): simplify fails
 // with MSCX_ELIMIT after min(2^27, max(2^24, work_limit x initial heap

@@ -6,10 +6,12 @@
 // manifold label pass of every leaf is again one launch.  The stages are
 // also driven by the serving loop (server.cpp).
 #include "pipeline.hpp"
+#include "dmerge.hpp"
 
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <cstdio>
 #include <cstdlib>
 
 #include <algorithm>
@@ -120,6 +122,16 @@ static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_
                  "tiling is finer than the grid: a piece's ext rect leaves the grid");
 }
 
+// Where the merge tree of a plain or file-driven run goes when the caller
+// does not say: HBM for grids of at least MSCX_DEVICE_MERGE_MIN_VERTICES
+// vertices (environment).
+static uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags) {
+  if (flags & (MSCX_RUN_DEVICE_MERGE | MSCX_RUN_HOST_MERGE)) return flags;
+  int64_t min_v = (int64_t)1 << 62;
+  if (const char *e = getenv("MSCX_DEVICE_MERGE_MIN_VERTICES")) min_v = atoll(e);
+  return (int64_t)sx * sy >= min_v ? (flags | MSCX_RUN_DEVICE_MERGE) : flags;
+}
+
 static void check_grid_args(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly) {
   MSCX_REQUIRE(sx >= 2 && sy >= 2 && sx <= 16384 && sy <= 16384, MSCX_EINVAL, "grid sides must be in [2, 16384]");
   MSCX_REQUIRE(levels >= 0 && lx >= 0 && ly >= 0 && levels <= 12 && lx + ly <= 12, MSCX_EINVAL, "bad tiling");
@@ -137,6 +149,7 @@ void prepare_grid(mscx_grid *G, const float *data, int32_t sx, int32_t sy, int32
   G->threshold = threshold;
   G->flags = flags;
   G->ooc = (flags & MSCX_RUN_OUT_OF_CORE) != 0;
+  G->dmerge = (flags & MSCX_RUN_DEVICE_MERGE) && !(flags & MSCX_RUN_HOST_MERGE);
   // MSCX_WORK_LIMIT in the environment bounds the simplification's work in
   // plain and batched runs as the server's work limit does (0 or unset: none)
   if (const char *wl = getenv("MSCX_WORK_LIMIT")) G->work_limit = std::max(0.0, atof(wl));
@@ -180,19 +193,17 @@ size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int
 static std::mutex g_crit_mu;
 static Scratch g_crit_host;
 
-void grids_compute(std::vector<mscx_grid *> &grids) {
+// Host leaf complexes (writeout_connectivity) of the grids from one host
+// copy of the tiles' critical block.
+static void build_host_leaves(std::vector<mscx_grid *> &grids) {
   const int ng = (int)grids.size();
   if (!ng) return;
-  const double t0 = now_s();
-  std::vector<mscx_tile *> tiles;
-  for (mscx_grid *G : grids) tiles.insert(tiles.end(), G->tiles.begin(), G->tiles.end());
-  run_tiles(tiles.data(), (int)tiles.size());
-
   // every tile's critical data lives in one block: one copy to host
   std::lock_guard<std::mutex> lk(g_crit_mu);
-  const size_t bytes = tiles[0]->crit_block_bytes;
+  const mscx_tile *t0 = grids[0]->tiles[0];
+  const size_t bytes = t0->crit_block_bytes;
   char *host = (char *)g_crit_host.reserve_host(bytes);
-  MSCX_HIP(hipMemcpyAsync(host, tiles[0]->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
+  MSCX_HIP(hipMemcpyAsync(host, t0->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
   MSCX_HIP(hipStreamSynchronize(runtime().stream));
 
   std::vector<std::pair<int, int>> leafs;  // (grid, leaf)
@@ -217,14 +228,153 @@ void grids_compute(std::vector<mscx_grid *> &grids) {
     std::vector<uint32_t> pr((const uint32_t *)(b + L.pair), (const uint32_t *)(b + L.pair) + K);
     complex_build(G->pieces[i]->c, G->leaves[i].rect, G->leaves[i].ext, crit, fn, conn, pr);
   });
+}
+
+void grids_compute(std::vector<mscx_grid *> &grids) {
+  const int ng = (int)grids.size();
+  if (!ng) return;
+  const double t0 = now_s();
+  std::vector<mscx_tile *> tiles;
+  for (mscx_grid *G : grids) tiles.insert(tiles.end(), G->tiles.begin(), G->tiles.end());
+  run_tiles(tiles.data(), (int)tiles.size());
+  std::vector<mscx_grid *> host, dev;
+  for (mscx_grid *G : grids) (G->dmerge ? dev : host).push_back(G);
+  build_host_leaves(host);
+  // device merge tree: the leaves are built in HBM from the critical block
+  std::vector<std::pair<int, int>> leafs;
+  for (int gi = 0; gi < (int)dev.size(); ++gi) {
+    mscx_grid *G = dev[gi];
+    const size_t nl = G->leaves.size();
+    G->pieces.assign(2 * nl - 1, nullptr);
+    G->dp.clear();
+    G->dp.resize(2 * nl - 1);
+    for (size_t i = 0; i < nl; ++i) {
+      G->dp[i].reset(new DComplex);
+      leafs.emplace_back(gi, (int)i);
+    }
+  }
+  parallel_for((int)leafs.size(), [&](int k) {
+    mscx_grid *G = dev[leafs[k].first];
+    const int i = leafs[k].second;
+    StreamLease sl;
+    dcomplex_from_tile(G->tiles[i], G->leaves[i].rect, G->leaves[i].ext, *G->dp[i], sl.s);
+  });
   const double t1 = now_s();
   for (mscx_grid *G : grids) G->t[0] += t1 - t0;
+}
+
+// The device tree could not be followed: the tree runs on the host from
+// host leaves built from the tiles' critical block.
+static void device_tree_fallback(mscx_grid *G, const DFallback &why) {
+  g_dfallbacks++;
+  if (getenv("MSCX_DEBUG_MERGE")) fprintf(stderr, "[mscx] device merge tree falls back to the host: %s\n", why.why.c_str());
+  for (mscx_complex *&c : G->pieces) delete c, c = nullptr;
+  G->dp.clear();
+  G->dmerge = false;
+  std::vector<mscx_grid *> one{G};
+  build_host_leaves(one);
+}
+
+// A failing device merge is redone on the host path, which raises the
+// error the reference meets first (device runtime failures excepted).
+template <class F>
+static void device_merge_step(F f) {
+  try {
+    f();
+  } catch (const Error &e) {
+    if (e.code == MSCX_EDEVICE) throw;
+    throw DFallback(e.what());
+  } catch (const std::logic_error &e) {
+    throw DFallback(e.what());
+  }
+}
+
+static void grid_merge_up_dev(mscx_grid *G) {
+  const int nleaf = (int)G->leaves.size();
+  const double t1 = now_s();
+  for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
+    parallel_for(width / 2, [&](int k) {
+      const int j = lo + 2 * k;
+      StreamLease sl;
+      std::unique_ptr<DComplex> o(new DComplex);
+      device_merge_step([&] { dmerge_up(*G->dp[j], *G->dp[j + 1], *o, sl.s); });
+      G->dp[nleaf + j / 2] = std::move(o);
+    });
+  G->t[1] += now_s() - t1;
+}
+
+static void grid_simplify_down_dev(mscx_grid *G) {
+  const int nleaf = (int)G->leaves.size();
+  const int root = 2 * nleaf - 2;
+  const double t2 = now_s();
+  if (G->threshold > 0.0) {
+    // the simplification is one sequential chain of heap pops: the root
+    // comes to the host and goes back to HBM for the merge down
+    StreamLease sl;
+    G->pieces[root] = new mscx_complex;
+    Complex &R = G->pieces[root]->c;
+    dcomplex_download(*G->dp[root], R, sl.s);
+    std::vector<std::pair<uint32_t, uint32_t>> canc;
+    simplify(R, canc, G->threshold, G->work_limit);
+    un_simplify(R, canc, G->work_limit);
+    if (nleaf > 1) {
+      std::unique_ptr<DComplex> up(new DComplex);
+      dcomplex_upload(R, *up, sl.s);
+      G->dp[root] = std::move(up);
+      // merge_down uncancels the root's line pairs in HBM: the host copy is
+      // made again when asked for
+      delete G->pieces[root];
+      G->pieces[root] = nullptr;
+    } else {
+      G->dp[root].reset();  // the host copy is the piece
+    }
+  }
+  const double t3 = now_s();
+  G->t[2] += t3 - t2;
+  if (nleaf > 1) {
+    std::vector<int> level{root};
+    while (!level.empty()) {
+      parallel_for((int)level.size(), [&](int k) {
+        const int p = level[k];
+        const int c = 2 * (p - nleaf);
+        StreamLease sl;
+        device_merge_step([&] { dmerge_down(*G->dp[p], *G->dp[c], *G->dp[c + 1], sl.s); });
+      });
+      std::vector<int> next;
+      for (int p : level) {
+        const int c = 2 * (p - nleaf);
+        if (c >= nleaf) next.push_back(c);
+        if (c + 1 >= nleaf) next.push_back(c + 1);
+      }
+      if (!next.empty() && 2 * (next[0] - nleaf) >= nleaf) std::sort(next.rbegin(), next.rend());
+      else std::sort(next.begin(), next.end());
+      level.swap(next);
+    }
+  }
+  // the manifold pass works on host copies of the leaves
+  if (!(G->flags & MSCX_RUN_NO_DISCS))
+    parallel_for(nleaf, [&](int i) {
+      if (G->pieces[i]) return;
+      StreamLease sl;
+      std::unique_ptr<mscx_complex> c(new mscx_complex);
+      dcomplex_download(*G->dp[i], c->c, sl.s);
+      G->pieces[i] = c.release();
+    });
+  G->t[3] += now_s() - t3;
 }
 
 static void init_spill(mscx_grid *G);
 
 // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
 void grid_merge_up(mscx_grid *G, bool inner_parallel) {
+  if (G->dmerge) {
+    try {
+      grid_merge_up_dev(G);
+      return;
+    } catch (const DFallback &e) {
+      device_tree_fallback(G, e);
+    }
+  }
   const int nleaf = (int)G->leaves.size();
   const int nt = inner_parallel ? 0 : 1;
   const double t1 = now_s();
@@ -253,6 +403,17 @@ void grid_merge_up(mscx_grid *G, bool inner_parallel) {
 // parents upwards), so that of several failing merges the one reported is
 // the one the reference meets first.
 void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
+  if (G->dmerge) {
+    try {
+      grid_simplify_down_dev(G);
+      return;
+    } catch (const DFallback &e) {
+      // the whole tree again, on the host
+      G->t[1] = G->t[2] = G->t[3] = 0;
+      device_tree_fallback(G, e);
+      grid_merge_up(G, inner_parallel);
+    }
+  }
   const int nleaf = (int)G->leaves.size();
   const int nt = inner_parallel ? 0 : 1;
   const double t2 = now_s();
@@ -348,6 +509,15 @@ void spill_piece(mscx_grid *G, int i) {
 
 mscx_complex *restore_piece(mscx_grid *G, int i) {
   if (G->pieces[i]) return G->pieces[i];
+  if (G->dmerge && (size_t)i < G->dp.size() && G->dp[i] &&
+      !(G->ooc && (size_t)i < G->spilled.size() && (G->spilled[i] || G->spilled_file[i]))) {
+    // device merge tree: a host copy of the piece's complex
+    StreamLease sl;
+    std::unique_ptr<mscx_complex> c(new mscx_complex);
+    dcomplex_download(*G->dp[i], c->c, sl.s);
+    G->pieces[i] = c.release();
+    return G->pieces[i];
+  }
   MSCX_REQUIRE(G->ooc && (size_t)i < G->spilled.size(), MSCX_ESTATE, "piece is neither live nor spilled");
   std::unique_ptr<mscx_complex> c(new mscx_complex);
   if (G->spilled_file[i]) {
@@ -465,7 +635,7 @@ extern "C" int mscx_grid_run(const float *data, int32_t sx, int32_t sy, int32_t 
     ensure_init();
     std::unique_ptr<mscx_grid> G(new mscx_grid);
     const double t0 = now_s();
-    prepare_grid(G.get(), data, sx, sy, levels, lx, ly, threshold, flags);
+    prepare_grid(G.get(), data, sx, sy, levels, lx, ly, threshold, merge_policy(sx, sy, flags));
     G->t[0] += now_s() - t0;
     std::vector<mscx_grid *> gs{G.get()};
     run_grids(gs, nullptr, nullptr);
@@ -555,7 +725,7 @@ extern "C" int mscx_grid_run_file(const char *path, int32_t sx, int32_t sy, int3
     G->dev_input = device_pool().alloc(f.bytes);
     stream_grid_file(f, sx, (float *)G->dev_input);
     prepare_grid(G.get(), (const float *)G->dev_input, sx, sy, levels, lx, ly, threshold,
-                 flags | MSCX_RUN_DEVICE_INPUT);
+                 merge_policy(sx, sy, flags) | MSCX_RUN_DEVICE_INPUT);
     G->t[0] += now_s() - t0;
     std::vector<mscx_grid *> gs{G.get()};
     run_grids(gs, nullptr, nullptr);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "complex.hpp"
+#include "dmerge.hpp"
 #include "runtime.hpp"
 #include "tile.hpp"
 
@@ -40,6 +41,11 @@ struct mscx_grid {
   std::mutex spill_mu;  // lazy restores by the accessors
   std::vector<mscx::LeafGeom> leaves;
   std::vector<mscx_complex *> pieces;  // leaves, then internal nodes in merge order; root last
+  // Device merge tree (MSCX_RUN_DEVICE_MERGE): the pieces' complexes live in
+  // HBM (dp[i]); pieces[i] is their host copy, made when a phase or a caller
+  // needs it.
+  bool dmerge = false;
+  std::vector<std::unique_ptr<mscx::DComplex>> dp;
   std::vector<mscx_tile *> tiles;
   bool owns_tiles = true;         // false when the tiles belong to a resident grid of the server
   std::shared_ptr<void> keep;     // keeps the tiles' owner alive (server results)
