@@ -271,6 +271,9 @@ int mscx_grid_run_file(const char *path, int32_t size_x, int32_t size_y, int32_t
 int mscx_grid_num_pieces(const mscx_grid *g, int32_t *n);
 /* Borrowed pointer to piece i's complex (leaves first, root last). */
 int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out);
+/* Number of critical points of piece i without making a host copy of a
+ * device-resident piece (no reference counterpart). */
+int mscx_grid_piece_num_cps(const mscx_grid *g, int32_t i, uint32_t *n);
 /* Stage times in seconds: compute, merge-up, simplify, merge-down, discs. */
 int mscx_grid_times(const mscx_grid *g, double t5[5]);
 int mscx_grid_destroy(mscx_grid *g);

@@ -567,6 +567,12 @@ class GridRun:
         check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
         return GridMSComplex(h, owned=False, owner=self)
 
+    def piece_num_cps(self, i):
+        """Critical points of piece i (no host copy of a device-resident piece)."""
+        n = ctypes.c_uint32()
+        check(lib().mscx_grid_piece_num_cps(self._h, i, ctypes.byref(n)))
+        return n.value
+
     def times(self):
         t = np.zeros(5, np.float64)
         check(lib().mscx_grid_times(self._h, ptr(t)))

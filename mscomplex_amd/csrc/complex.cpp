@@ -35,6 +35,13 @@ void CellMap::reserve(size_t n) {
     if (e.key != UINT32_MAX) set(e.key, e.val);
 }
 
+void *CellMap::raw_table(size_t slots, size_t n) {
+  slots_.resize(slots);
+  mask_ = slots - 1;
+  n_ = n;
+  return slots_.data();
+}
+
 void CellMap::grow() { reserve(slots_.empty() ? 16 : slots_.size()); }
 
 size_t CellMap::slot(uint32_t key) const {

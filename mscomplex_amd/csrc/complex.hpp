@@ -163,6 +163,11 @@ class CellMap {
     return k;
   }
   size_t size() const { return n_; }
+  // A table filled elsewhere with the same layout (home slot hash(key) &
+  // (slots - 1), linear probing, {key, value} pairs, key UINT32_MAX empty):
+  // returns room for `slots` entries (a power of two above 2 n) holding n
+  // keys, for the caller to fill.
+  void *raw_table(size_t slots, size_t n);
 
  private:
   struct Slot {

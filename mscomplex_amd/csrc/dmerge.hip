@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <atomic>
 #include <cstring>
 #include <mutex>
@@ -80,12 +83,14 @@ __device__ __forceinline__ uint32_t dhash(uint32_t k) {
   return k;
 }
 
+// A slot is {key, value} as two little-endian words (the host CellMap's
+// layout, so that a host copy of a complex takes the table as it is).
 __device__ uint32_t map_find(const unsigned long long *map, uint32_t mask, uint32_t key) {
   uint32_t i = dhash(key) & mask;
   for (;;) {
     const unsigned long long e = map[i];
     if (e == MAP_EMPTY) return NONE;
-    if ((uint32_t)(e >> 32) == key) return (uint32_t)e;
+    if ((uint32_t)e == key) return (uint32_t)(e >> 32);
     i = (i + 1) & mask;
   }
 }
@@ -93,11 +98,11 @@ __device__ uint32_t map_find(const unsigned long long *map, uint32_t mask, uint3
 // false when key is already present
 __device__ bool map_insert(unsigned long long *map, uint32_t mask, uint32_t key, uint32_t val) {
   uint32_t i = dhash(key) & mask;
-  const unsigned long long v = ((unsigned long long)key << 32) | val;
+  const unsigned long long v = ((unsigned long long)val << 32) | key;
   for (;;) {
     const unsigned long long old = atomicCAS(&map[i], MAP_EMPTY, v);
     if (old == MAP_EMPTY) return true;
-    if ((uint32_t)(old >> 32) == key) return false;
+    if ((uint32_t)old == key) return false;
     i = (i + 1) & mask;
   }
 }
@@ -929,15 +934,57 @@ void check_err(const uint32_t *d_err, hipStream_t s) {
   if (const uint32_t e = read_u32(d_err, s)) throw DFallback("device check bits " + std::to_string(e));
 }
 
-// pinned host staging for the copies of this module
-struct Staging {
-  std::mutex mu;
-  Scratch sc;
+// pinned host staging for the copies of this module, leased per call so
+// that concurrent copies (the leaves of one level) do not wait on each other
+std::mutex g_pin_mu;
+std::vector<Scratch *> g_pins;
+struct PinLease {
+  Scratch *sc;
+  PinLease() {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (g_pins.empty()) {
+      sc = new Scratch;
+    } else {
+      sc = g_pins.back();
+      g_pins.pop_back();
+    }
+  }
+  ~PinLease() {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pins.push_back(sc);
+  }
+  PinLease(const PinLease &) = delete;
+  PinLease &operator=(const PinLease &) = delete;
 };
-Staging &staging() {
-  static Staging st;
-  return st;
+// copies up to this size go through pinned staging; larger ones (a
+// 16384^2 root) straight into pageable memory
+constexpr size_t PIN_MAX = (size_t)256 << 20;
+
+// MSCX_DEBUG_MERGE in the environment: per-call timings on stderr
+bool dbg_on() {
+  static const bool on = getenv("MSCX_DEBUG_MERGE") != nullptr;
+  return on;
 }
+struct DbgClock {
+  const char *what;
+  uint32_t n;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now(), last = t0;
+  std::string marks;
+  DbgClock(const char *w, uint32_t nn) : what(w), n(nn) {}
+  void mark(const char *m) {
+    if (!dbg_on()) return;
+    const auto t = std::chrono::steady_clock::now();
+    char b[64];
+    snprintf(b, sizeof b, " %s %.3f", m, std::chrono::duration<double, std::milli>(t - last).count());
+    marks += b;
+    last = t;
+  }
+  ~DbgClock() {
+    if (!dbg_on()) return;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    fprintf(stderr, "[dmerge] %s n=%u %.3f ms:%s\n", what, n, ms, marks.c_str());
+  }
+};
 
 // host complex of the nodes a line pass touches
 struct LineSub {
@@ -957,6 +1004,7 @@ uint32_t local_of(const std::vector<uint32_t> &ids, uint32_t g) {
 // writes the changed nodes back.
 void line_pass(DComplex &d, const Rect &ixn, bool up, hipStream_t s) {
   g_line_passes++;
+  DbgClock dc(up ? "line_up" : "line_down", d.n);
   Temps T(s);
   const uint32_t nw = (d.n + 31) / 32;
   uint32_t *bm = T.get<uint32_t>(nw), *bm2 = T.get<uint32_t>(nw);
@@ -982,6 +1030,7 @@ void line_pass(DComplex &d, const Rect &ixn, bool up, hipStream_t s) {
   // (a missing line cell: the host path raises the reference's error)
   if (hn[0]) throw DFallback("line pass check bits " + std::to_string(hn[0]));
   const uint32_t nfull = hn[1], nmeta = hn[2];
+  dc.mark("mark");
   uint32_t *meta_rec = T.get<uint32_t>(3 * (uint64_t)nmeta);
   uint32_t *rsz = T.get<uint32_t>(nfull), *roff = T.get<uint32_t>(nfull);
   uint32_t *dtot = T.get<uint32_t>(1);
@@ -1002,6 +1051,7 @@ void line_pass(DComplex &d, const Rect &ixn, bool up, hipStream_t s) {
   MSCX_HIP(hipMemcpyAsync(lw.data(), lrec, (size_t)lwords * 4, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipStreamSynchronize(s));
 
+  dc.mark("gather");
   Complex &c = L.c;
   c.rect = d.rect;
   c.ext = d.ext;
@@ -1043,8 +1093,10 @@ void line_pass(DComplex &d, const Rect &ixn, bool up, hipStream_t s) {
   std::vector<uint8_t> isfull(nmeta, 0);
   for (uint32_t k = 0; k < nfull; ++k) isfull[floc[k]] = 1;
 
+  dc.mark("build");
   if (up) merge_up_line(c, ixn);
   else merge_down_line(c, ixn);
+  dc.mark("pass");
 
   // only the full nodes may change (their lists hold every node a pass
   // touches); anything else means the sub-complex was not closed
@@ -1150,6 +1202,7 @@ void dcomplex_from_tile(const mscx_tile *t, const Rect &rect, const Rect &ext, D
 }
 
 void dmerge_up(const DComplex &A, const DComplex &B, DComplex &O, hipStream_t s) {
+  DbgClock dc("merge_up", A.n + B.n);
   Rect ixn;
   merge_ixn(A.rect, B.rect, ixn);
   merge_rects(A.rect, A.ext, B.rect, B.ext, O.rect, O.ext);
@@ -1166,6 +1219,7 @@ void dmerge_up(const DComplex &A, const DComplex &B, DComplex &O, hipStream_t s)
   MSCX_HIP(hipMemcpyAsync(hn, dn, 16, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipStreamSynchronize(s));
   const uint32_t n1 = hn[1], n = n1 + hn[2];
+  dc.mark("index");
   alloc_nodes(O, (uint32_t)Complex::with_headroom(n), s);
   O.n = n;
   uint32_t *src0 = T.get<uint32_t>(n), *src1 = T.get<uint32_t>(n);
@@ -1184,14 +1238,17 @@ void dmerge_up(const DComplex &A, const DComplex &B, DComplex &O, hipStream_t s)
   sort_lists(O, s, T, dn);
   k_mu_pairs<<<blocks(n), 256, 0, s>>>(va, vb, src0, src1, vo, dn);
   check_err(dn, s);
+  dc.mark("arcs");
   line_pass(O, ixn, true, s);
   g_dmerges++;
 }
 
 void dmerge_down(DComplex &P, DComplex &A, DComplex &B, hipStream_t s) {
+  DbgClock dc("merge_down", P.n);
   Rect ixn;
   merge_ixn(A.rect, B.rect, ixn);
   line_pass(P, ixn, false, s);
+  dc.mark("line");
   DComplex *kids[2] = {&A, &B};
   for (DComplex *mp : kids) {
     DComplex &m = *mp;
@@ -1317,11 +1374,13 @@ void dmerge_down(DComplex &P, DComplex &A, DComplex &B, hipStream_t s) {
     m.aused = words;
     sort_lists(m, s, T, dn);
     check_err(dn, s);
+    dc.mark("child");
   }
   g_dmerges++;
 }
 
 void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
+  DbgClock dc("download", d.n);
   const uint32_t n = d.n;
   Temps T(s);
   uint32_t *sz = T.get<uint32_t>(n + 1), *tot = T.get<uint32_t>(1);
@@ -1331,20 +1390,25 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
   const uint32_t words = read_u32(tot, s);
   uint32_t *w = T.get<uint32_t>(words);
   k_pack_all<true><<<blocks(n), 256, 0, s>>>(v, nullptr, sz, w);
-  std::vector<uint32_t> hw(words), ho(n);
-  {
-    Staging &st = staging();
-    std::lock_guard<std::mutex> lk(st.mu);
-    char *h = (char *)st.sc.reserve_host(((size_t)words + n) * 4);
-    MSCX_HIP(hipMemcpyAsync(h, w, (size_t)words * 4, hipMemcpyDeviceToHost, s));
-    MSCX_HIP(hipMemcpyAsync(h + (size_t)words * 4, sz, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    MSCX_HIP(hipStreamSynchronize(s));
-    memcpy(hw.data(), h, (size_t)words * 4);
-    memcpy(ho.data(), h + (size_t)words * 4, (size_t)n * 4);
-  }
   Complex c;
   c.rect = d.rect;
   c.ext = d.ext;
+  // the cell map: the device table as it is
+  const size_t slots = (size_t)d.mmask + 1;
+  MSCX_HIP(hipMemcpyAsync(c.idx.raw_table(slots, n), d.map, slots * 8, hipMemcpyDeviceToHost, s));
+  const size_t bytes = ((size_t)words + n) * 4;
+  PinLease pin;
+  std::unique_ptr<uint32_t[]> big;
+  uint32_t *hw;
+  if (bytes <= PIN_MAX) {
+    hw = (uint32_t *)pin.sc->reserve_host(bytes);
+  } else {
+    big.reset(new uint32_t[(size_t)words + n]);
+    hw = big.get();
+  }
+  uint32_t *ho = hw + words;
+  MSCX_HIP(hipMemcpyAsync(hw, w, (size_t)words * 4, hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipMemcpyAsync(ho, sz, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   c.reserve_nodes(Complex::with_headroom(n));
   c.cell.resize(n);
   c.fn.resize(n);
@@ -1354,9 +1418,10 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
   c.des.resize(n);
   c.asc_disc.resize(n);
   c.des_disc.resize(n);
+  MSCX_HIP(hipStreamSynchronize(s));
   auto parse = [&](uint32_t lo, uint32_t hi) {
     for (uint32_t i = lo; i < hi; ++i) {
-      const uint32_t *r = hw.data() + ho[i];
+      const uint32_t *r = hw + ho[i];
       c.cell[i] = r[0];
       memcpy(&c.fn[i], &r[1], 4);
       c.pair[i] = r[2];
@@ -1371,22 +1436,16 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
     std::vector<std::thread> th;
     const uint32_t per = (n + nt - 1) / nt;
     for (int t = 1; t < nt; ++t) th.emplace_back(parse, std::min(n, t * per), std::min(n, (t + 1) * per));
-    std::thread mapper([&] {
-      c.idx.reserve(n);
-      for (uint32_t i = 0; i < n; ++i) c.idx.insert(hw[ho[i]], i);
-    });
     parse(0, std::min(n, per));
     for (auto &x : th) x.join();
-    mapper.join();
   } else {
     parse(0, n);
-    c.idx.reserve(n);
-    for (uint32_t i = 0; i < n; ++i) c.idx.insert(c.cell[i], i);
   }
   out = std::move(c);
 }
 
 void dcomplex_upload(const Complex &c, DComplex &o, hipStream_t s) {
+  DbgClock dc("upload", c.size());
   const uint32_t n = c.size();
   o.rect = c.rect;
   o.ext = c.ext;
@@ -1406,22 +1465,36 @@ void dcomplex_upload(const Complex &c, DComplex &o, hipStream_t s) {
   uint32_t *dfl = T.get<uint32_t>(n), *err = T.get<uint32_t>(1);
   MSCX_HIP(hipMemsetAsync(err, 0, 4, s));
   {
-    Staging &st = staging();
-    std::lock_guard<std::mutex> lk(st.mu);
-    const size_t bytes = ((size_t)n * 4 + 4 * (size_t)n + words) * 4;
-    uint32_t *h = (uint32_t *)st.sc.reserve_host(bytes);
+    PinLease pin;
+    const size_t bytes = (6 * (size_t)n + words) * 4;
+    std::unique_ptr<uint32_t[]> big;
+    uint32_t *h;
+    if (bytes <= PIN_MAX) {
+      h = (uint32_t *)pin.sc->reserve_host(bytes);
+    } else {
+      big.reset(new uint32_t[6 * (size_t)n + words]);
+      h = big.get();
+    }
     uint32_t *hc = h, *hp = h + n, *hf = h + 2 * (size_t)n, *hfl = h + 3 * (size_t)n, *hcnt = h + 4 * (size_t)n,
              *ha = h + 6 * (size_t)n;
-    for (uint32_t i = 0; i < n; ++i) {
-      hc[i] = c.cell[i];
-      hp[i] = c.pair[i];
-      memcpy(&hf[i], &c.fn[i], 4);
-      hfl[i] = c.flags[i];
-      hcnt[2 * (size_t)i] = cnt[2 * (size_t)i];
-      hcnt[2 * (size_t)i + 1] = cnt[2 * (size_t)i + 1];
-      std::copy(c.asc[i].begin(), c.asc[i].end(), ha + off[2 * (size_t)i]);
-      std::copy(c.des[i].begin(), c.des[i].end(), ha + off[2 * (size_t)i + 1]);
-    }
+    auto fill = [&](uint32_t lo, uint32_t hi) {
+      for (uint32_t i = lo; i < hi; ++i) {
+        hc[i] = c.cell[i];
+        hp[i] = c.pair[i];
+        memcpy(&hf[i], &c.fn[i], 4);
+        hfl[i] = c.flags[i];
+        hcnt[2 * (size_t)i] = cnt[2 * (size_t)i];
+        hcnt[2 * (size_t)i + 1] = cnt[2 * (size_t)i + 1];
+        std::copy(c.asc[i].begin(), c.asc[i].end(), ha + off[2 * (size_t)i]);
+        std::copy(c.des[i].begin(), c.des[i].end(), ha + off[2 * (size_t)i + 1]);
+      }
+    };
+    const int nt = n >= (1u << 16) ? std::min(host_threads(), 16) : 1;
+    const uint32_t per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(fill, std::min(n, t * per), std::min(n, (t + 1) * per));
+    fill(0, std::min(n, per));
+    for (auto &x : th) x.join();
     MSCX_HIP(hipMemcpyAsync(o.cell, hc, (size_t)n * 4, hipMemcpyHostToDevice, s));
     MSCX_HIP(hipMemcpyAsync(o.pair, hp, (size_t)n * 4, hipMemcpyHostToDevice, s));
     MSCX_HIP(hipMemcpyAsync(o.fn, hf, (size_t)n * 4, hipMemcpyHostToDevice, s));

@@ -127,7 +127,7 @@ static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_
 // vertices (environment).
 static uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags) {
   if (flags & (MSCX_RUN_DEVICE_MERGE | MSCX_RUN_HOST_MERGE)) return flags;
-  int64_t min_v = (int64_t)1 << 62;
+  int64_t min_v = (int64_t)1 << 24;
   if (const char *e = getenv("MSCX_DEVICE_MERGE_MIN_VERTICES")) min_v = atoll(e);
   return (int64_t)sx * sy >= min_v ? (flags | MSCX_RUN_DEVICE_MERGE) : flags;
 }
@@ -758,6 +758,19 @@ extern "C" int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex
       restore_piece(G, i);
     }
     *out = g->pieces[i];
+  });
+}
+
+extern "C" int mscx_grid_piece_num_cps(const mscx_grid *g, int32_t i, uint32_t *n) {
+  return guarded([&] {
+    MSCX_REQUIRE(g && n && i >= 0 && i < (int32_t)g->pieces.size(), MSCX_EINVAL, "bad piece index");
+    if (g->pieces[i]) *n = g->pieces[i]->c.size();
+    else if ((size_t)i < g->dp.size() && g->dp[i]) *n = g->dp[i]->n;
+    else {
+      mscx_grid *G = const_cast<mscx_grid *>(g);
+      std::lock_guard<std::mutex> lk(G->spill_mu);
+      *n = restore_piece(G, i)->c.size();
+    }
   });
 }
 
