@@ -1379,6 +1379,25 @@ void dmerge_down(DComplex &P, DComplex &A, DComplex &B, hipStream_t s) {
   g_dmerges++;
 }
 
+void dcomplex_copy(const DComplex &a, DComplex &o, hipStream_t s) {
+  o.rect = a.rect;
+  o.ext = a.ext;
+  alloc_nodes(o, a.ncap, s);
+  alloc_arcs(o, a.acap);
+  o.n = a.n;
+  o.aused = a.aused;
+  const size_t n = a.n;
+  MSCX_HIP(hipMemcpyAsync(o.cell, a.cell, n * 4, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.pair, a.pair, n * 4, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.fn, a.fn, n * 4, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.flags, a.flags, n, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.off, a.off, n * 8, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.cnt, a.cnt, n * 8, hipMemcpyDeviceToDevice, s));
+  if (a.aused) MSCX_HIP(hipMemcpyAsync(o.arcs, a.arcs, a.aused * 4, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipMemcpyAsync(o.map, a.map, ((size_t)a.mmask + 1) * 8, hipMemcpyDeviceToDevice, s));
+  MSCX_HIP(hipStreamSynchronize(s));
+}
+
 void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
   DbgClock dc("download", d.n);
   const uint32_t n = d.n;

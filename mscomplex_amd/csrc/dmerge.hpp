@@ -69,6 +69,8 @@ void dcomplex_from_tile(const mscx_tile *t, const Rect &rect, const Rect &ext, D
 void dmerge_up(const DComplex &a, const DComplex &b, DComplex &out, hipStream_t s);
 // merge_down(P, a, b); P's line pairs are uncancelled in place.
 void dmerge_down(DComplex &P, DComplex &a, DComplex &b, hipStream_t s);
+// Device copy (same capacities) on stream s.
+void dcomplex_copy(const DComplex &src, DComplex &dst, hipStream_t s);
 // Host copies.
 void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s);
 void dcomplex_upload(const Complex &c, DComplex &out, hipStream_t s);

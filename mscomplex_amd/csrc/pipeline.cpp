@@ -125,7 +125,7 @@ static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_
 // Where the merge tree of a plain or file-driven run goes when the caller
 // does not say: HBM for grids of at least MSCX_DEVICE_MERGE_MIN_VERTICES
 // vertices (environment).
-static uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags) {
+uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags) {
   if (flags & (MSCX_RUN_DEVICE_MERGE | MSCX_RUN_HOST_MERGE)) return flags;
   int64_t min_v = (int64_t)1 << 24;
   if (const char *e = getenv("MSCX_DEVICE_MERGE_MIN_VERTICES")) min_v = atoll(e);
@@ -536,14 +536,41 @@ mscx_complex *restore_piece(mscx_grid *G, int i) {
 void copy_pieces(const mscx_grid *src, mscx_grid *dst) {
   for (mscx_complex *c : dst->pieces) delete c;
   dst->pieces.assign(src->pieces.size(), nullptr);
+  dst->dp.clear();
+  dst->dmerge = src->dmerge;
+  if (src->dmerge) {
+    // device-resident complexes: copied in HBM
+    dst->dp.resize(src->dp.size());
+    try {
+      parallel_for((int)src->dp.size(), [&](int i) {
+        if (!src->dp[i]) return;
+        StreamLease sl;
+        std::unique_ptr<DComplex> d(new DComplex);
+        dcomplex_copy(*src->dp[i], *d, sl.s);
+        dst->dp[i] = std::move(d);
+      });
+    } catch (...) {
+      dst->dp.clear();
+      throw;
+    }
+  }
   // the pieces are independent: copied on the host threads
   try {
-    parallel_for((int)src->pieces.size(), [&](int i) { dst->pieces[i] = new mscx_complex(*src->pieces[i]); });
+    parallel_for((int)src->pieces.size(), [&](int i) {
+      if (src->pieces[i]) dst->pieces[i] = new mscx_complex(*src->pieces[i]);
+    });
   } catch (...) {
     for (mscx_complex *&c : dst->pieces) delete c, c = nullptr;
     dst->pieces.clear();
     throw;
   }
+}
+
+size_t device_tree_bytes(const mscx_grid *G) {
+  size_t b = 0;
+  for (const auto &d : G->dp)
+    if (d) b += d->device_bytes();
+  return b;
 }
 
 }  // namespace mscx

@@ -89,8 +89,15 @@ void open_grid_file(const char *path, int32_t sx, int32_t sy, GridFile &f);
 // slabs, each copy overlapping the next read; rejects NaN (MSCX_EINVAL).
 void stream_grid_file(GridFile &f, int32_t sx, float *dev);
 
-// Deep copy of a grid's pieces (the complexes); tiles are not copied.
+// Deep copy of a grid's pieces (the complexes, in HBM for a device merge
+// tree); tiles are not copied.
 void copy_pieces(const mscx_grid *src, mscx_grid *dst);
+// HBM held by a grid's device-resident complexes.
+size_t device_tree_bytes(const mscx_grid *G);
+// The run flags with the merge tree's placement decided for an sx x sy grid
+// (MSCX_RUN_DEVICE_MERGE from MSCX_DEVICE_MERGE_MIN_VERTICES vertices, 2^24
+// by default) unless the caller chose one.
+uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags);
 
 // Out-of-core mode: spill_piece stores piece i (when live) and frees it;
 // restore_piece makes it live again and returns it.  Both are no-ops for a

@@ -11,8 +11,8 @@
 // that arrives while earlier grids are in flight overlaps with them.  A
 // computed grid stays resident under the request's key -- its tiles (owner
 // images, codes) in HBM and its complexes as merged, before simplification,
-// on the host -- until the budget needs the memory (least recently used
-// first).  A request with a resident key skips the GPU stages and the merge
+// in HBM for grids whose merge tree runs on the GPU (on the host for small
+// ones) -- until the budget needs the memory (least recently used first).  A request with a resident key skips the GPU stages and the merge
 // up: it copies the merged complexes, simplifies at its own threshold, merges
 // down and fills its manifolds from the resident owner images.
 #include <cmath>
@@ -273,7 +273,8 @@ struct mscx_server {
     for (Request *R : batch) {
       std::unique_ptr<mscx_grid> G(new mscx_grid);
       try {
-        prepare_grid(G.get(), R->data.data(), R->sx, R->sy, 0, R->lx, R->ly, R->threshold, R->flags);
+        prepare_grid(G.get(), R->data.data(), R->sx, R->sy, 0, R->lx, R->ly, R->threshold,
+                     merge_policy(R->sx, R->sy, R->flags));
         G->work_limit = work_limit;
         R->result = std::move(G);
         gs.push_back(R->result.get());
@@ -352,7 +353,7 @@ struct mscx_server {
         copy_pieces(G, base.get());
         G->owns_tiles = false;
         G->keep = R.res;
-        size_t bytes = 0;
+        size_t bytes = device_tree_bytes(base.get());
         for (mscx_tile *t : G->tiles) bytes += t->fixed_bytes + (t->crit_block ? t->crit_block_bytes : 0);
         std::vector<uint64_t> waiters;
         {

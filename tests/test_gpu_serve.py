@@ -121,6 +121,32 @@ def test_repeat_requests_resimplify_resident_grid():
     srv.close()
 
 
+def test_repeats_from_device_resident_complexes(monkeypatch):
+    # with the merge tree on the GPU the resident grid keeps its merged
+    # complexes in HBM; repeats copy them there, simplify at their own
+    # threshold and merge down on the GPU
+    monkeypatch.setenv("MSCX_DEVICE_MERGE_MIN_VERTICES", "1")
+    g = _grid("v", 300, 260, 23)
+    srv = mx.Server(2 << 30, host_threads=4)
+    s0 = mx.merge_stats()
+    t0 = srv.submit(g, 2, 1, 0.01, key=91)
+    st, err, _, view = srv.wait(t0, timeout=120)
+    assert st == 0, err
+    _same(view, mx.GridRun(g, levels=0, lx=2, ly=1, threshold=0.01, merge="host"), "device-resident first run")
+    for th in (0.0, 0.03, 0.05):
+        t = srv.submit(None, 2, 1, th, key=91, size=(g.shape[1], g.shape[0]))
+        st, err, _, view = srv.wait(t, timeout=120)
+        assert st == 0, err
+        _same(view, mx.GridRun(g, levels=0, lx=2, ly=1, threshold=th, merge="host"), f"device-resident repeat {th}")
+        srv.release(t)
+    s1 = mx.merge_stats()
+    # 7 merges up once, 7 down per request, no fallback
+    assert s1[1] == s0[1] and s1[0] - s0[0] == 7 * 5, (s0, s1)
+    srv.release(t0)
+    assert srv.stats()["repeats"] == 3
+    srv.close()
+
+
 def test_budget_admission_eviction_and_rejection():
     grids = [_grid("g", 220, 200, 31 + i) for i in range(4)]
     one = 0
