@@ -246,17 +246,27 @@ def batched_bench(n=1024, seed=7):
     t_batch = time.perf_counter() - t0
     print(f"[bench] batched: autotune {t_tune:.1f}s, batch {t_batch:.2f}s", file=sys.stderr, flush=True)
     bad = [i for i in range(n) if b.status(i)[0] != 0]
+    # word-for-word parity of every grid, batched vs plain (checksums of the
+    # pieces' dump words, outside the timed regions)
+    sums_b = [b.grid(i).checksum() if i not in bad else None for i in range(n)]
     del b
-    t0 = time.perf_counter()
-    for (p, sx, sy), a, c, th in zip(dev, lx, ly, ths):
+    t_plain = 0.0
+    mismatch = []
+    for i, ((p, sx, sy), a, c, th) in enumerate(zip(dev, lx, ly, ths)):
+        t0 = time.perf_counter()
         r = mx.GridRun(None, levels=0, lx=a, ly=c, threshold=th, device_ptr=p, size=(sx, sy))
+        t_plain += time.perf_counter() - t0
+        if sums_b[i] is not None and r.checksum() != sums_b[i]:
+            mismatch.append(i)
         del r
-    t_plain = time.perf_counter() - t0
-    print(f"[bench] batched: plain {t_plain:.2f}s", file=sys.stderr, flush=True)
+    print(f"[bench] batched: plain {t_plain:.2f}s, {len(mismatch)} grids differ from the batch", file=sys.stderr,
+          flush=True)
     vert = sum(g.size for g in grids)
     return {"grids": n, "vertices": int(vert), "batched_grids_per_s": round(n / t_batch, 2),
             "plain_grids_per_s": round(n / t_plain, 2), "speedup": round(t_plain / t_batch, 2),
             "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3), "failed": bad,
+            "parity_vs_plain": {"grids_compared": n - len(bad), "differing": mismatch,
+                                "method": "FNV-1a of every piece's dump words (mscx_grid_checksum)"},
             "tiling_policy": {"side<256": policy(255, 255), "side<1024": policy(1023, 1023),
                               "side>=1024": policy(2048, 2048)},
             "autotune_table": {f"{b}:{c}": round(v, 4) for (b, c), v in table.items()},

@@ -241,7 +241,9 @@ int mscx_complex_destroy(mscx_complex *c);
  * Results are the same as without the flag (a merge the kernels do not
  * follow sends the whole tree to the host path).  mscx_grid_run and
  * mscx_grid_run_file set it by themselves for grids of at least
- * MSCX_DEVICE_MERGE_MIN_VERTICES vertices (environment; default 2^24);
+ * MSCX_DEVICE_MERGE_MIN_VERTICES vertices (environment; default 2^25:
+ * smaller trees are latency-bound on the GPU, where the host tree is as
+ * fast);
  * MSCX_RUN_HOST_MERGE keeps the tree on the host. */
 #define MSCX_RUN_DEVICE_MERGE 8
 #define MSCX_RUN_HOST_MERGE 16
@@ -274,6 +276,11 @@ int mscx_grid_piece(const mscx_grid *g, int32_t i, const mscx_complex **out);
 /* Number of critical points of piece i without making a host copy of a
  * device-resident piece (no reference counterpart). */
 int mscx_grid_piece_num_cps(const mscx_grid *g, int32_t i, uint32_t *n);
+/* FNV-1a checksum of the dump words of every piece in piece order (no
+ * reference counterpart): equal checksums of two runs mean their pieces
+ * agree word for word, for comparing forms (batched, served, plain) at
+ * sizes where keeping every dump is too costly. */
+int mscx_grid_checksum(const mscx_grid *g, uint64_t *out);
 /* Stage times in seconds: compute, merge-up, simplify, merge-down, discs. */
 int mscx_grid_times(const mscx_grid *g, double t5[5]);
 int mscx_grid_destroy(mscx_grid *g);

@@ -369,6 +369,12 @@ class _GridView:
         check(lib().mscx_grid_times(self._handle(), ptr(t)))
         return t
 
+    def checksum(self):
+        """FNV-1a of every piece's dump words (mscx_grid_checksum)."""
+        h = ctypes.c_uint64()
+        check(lib().mscx_grid_checksum(self._handle(), ctypes.byref(h)))
+        return h.value
+
 
 class Batch:
     """Many grids through the pipeline together (mscx_batch_run).  grids:
@@ -566,6 +572,12 @@ class GridRun:
         h = ctypes.c_void_p()
         check(lib().mscx_grid_piece(self._h, i, ctypes.byref(h)))
         return GridMSComplex(h, owned=False, owner=self)
+
+    def checksum(self):
+        """FNV-1a of every piece's dump words (mscx_grid_checksum)."""
+        h = ctypes.c_uint64()
+        check(lib().mscx_grid_checksum(self._h, ctypes.byref(h)))
+        return h.value
 
     def piece_num_cps(self, i):
         """Critical points of piece i (no host copy of a device-resident piece)."""

@@ -66,6 +66,7 @@ _SIGS = {
     "mscx_grid_num_pieces": (c_int, [P, ctypes.POINTER(c_int32)]),
     "mscx_grid_piece": (c_int, [P, c_int32, ctypes.POINTER(P)]),
     "mscx_grid_piece_num_cps": (c_int, [P, c_int32, P]),
+    "mscx_grid_checksum": (c_int, [P, P]),
     "mscx_grid_times": (c_int, [P, P]),
     "mscx_grid_destroy": (c_int, [P]),
     "mscx_batch_run": (c_int, [c_int32, P, P, P, P, P, P, c_uint32, ctypes.POINTER(P)]),

@@ -127,7 +127,7 @@ static void leaves_of(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_
 // vertices (environment).
 uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags) {
   if (flags & (MSCX_RUN_DEVICE_MERGE | MSCX_RUN_HOST_MERGE)) return flags;
-  int64_t min_v = (int64_t)1 << 24;
+  int64_t min_v = (int64_t)1 << 25;
   if (const char *e = getenv("MSCX_DEVICE_MERGE_MIN_VERTICES")) min_v = atoll(e);
   return (int64_t)sx * sy >= min_v ? (flags | MSCX_RUN_DEVICE_MERGE) : flags;
 }
@@ -798,6 +798,27 @@ extern "C" int mscx_grid_piece_num_cps(const mscx_grid *g, int32_t i, uint32_t *
       std::lock_guard<std::mutex> lk(G->spill_mu);
       *n = restore_piece(G, i)->c.size();
     }
+  });
+}
+
+// FNV-1a over the dump words of every piece, in piece order.
+extern "C" int mscx_grid_checksum(const mscx_grid *g, uint64_t *out) {
+  return guarded([&] {
+    MSCX_REQUIRE(g && out, MSCX_EINVAL, "null argument");
+    mscx_grid *G = const_cast<mscx_grid *>(g);
+    uint64_t h = 1469598103934665603ull;
+    std::vector<uint32_t> w;
+    for (size_t i = 0; i < G->pieces.size(); ++i) {
+      const mscx_complex *c = G->pieces[i];
+      if (!c) {
+        std::lock_guard<std::mutex> lk(G->spill_mu);
+        c = restore_piece(G, (int)i);
+      }
+      c->c.dump(w);
+      const unsigned char *b = (const unsigned char *)w.data();
+      for (size_t k = 0; k < w.size() * 4; ++k) h = (h ^ b[k]) * 1099511628211ull;
+    }
+    *out = h;
   });
 }
 

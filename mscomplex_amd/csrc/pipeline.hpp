@@ -95,7 +95,7 @@ void copy_pieces(const mscx_grid *src, mscx_grid *dst);
 // HBM held by a grid's device-resident complexes.
 size_t device_tree_bytes(const mscx_grid *G);
 // The run flags with the merge tree's placement decided for an sx x sy grid
-// (MSCX_RUN_DEVICE_MERGE from MSCX_DEVICE_MERGE_MIN_VERTICES vertices, 2^24
+// (MSCX_RUN_DEVICE_MERGE from MSCX_DEVICE_MERGE_MIN_VERTICES vertices, 2^25
 // by default) unless the caller chose one.
 uint32_t merge_policy(int32_t sx, int32_t sy, uint32_t flags);
 

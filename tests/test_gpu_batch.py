@@ -55,3 +55,23 @@ def test_failing_grid_is_isolated():
     for i, g in ((0, good[0]), (3, good[1])):
         ref = oracle.Pipeline(g, levels=0, threshold=0.01)
         assert np.array_equal(ref.dump(4, 0), b.grid(i).piece(0).dump())
+
+
+def test_batch_at_scale_equals_plain_runs():
+    # the batched configuration's field mix at a larger scale than the oracle
+    # reaches in a test: 96 grids with long sides log-uniform in [64, 1024],
+    # aspect up to 1:4, value noise at 1% and Gaussian sums at 0, various
+    # tilings; every grid's pieces equal the plain run's word for word
+    # (checksums of the dump words)
+    mix = mx.uneven_mix(96, 64, 1024, 5, aspect_max=4.0, thresholds=0.01)
+    grids = [g for g, _ in mix]
+    th = [0.0 if i % 2 else t for i, (_, t) in enumerate(mix)]
+    r = np.random.default_rng(5)
+    lx = [int(r.integers(0, 3)) if min(g.shape) > 200 else 0 for g in grids]
+    ly = [int(r.integers(0, 3)) if min(g.shape) > 200 else 0 for g in grids]
+    print(f"[batch] {len(grids)} grids, {sum(g.size for g in grids)} vertices", flush=True)
+    b = mx.Batch(grids, lx, ly, th)
+    for i, g in enumerate(grids):
+        assert b.status(i)[0] == 0, b.status(i)
+        single = mx.GridRun(g, levels=0, lx=lx[i], ly=ly[i], threshold=th[i])
+        assert b.grid(i).checksum() == single.checksum(), f"grid {i} {g.shape} differs from its plain run"

@@ -1,8 +1,8 @@
 """The worst-case configuration end to end (merge tree, simplification,
 merge down, manifold discs) through mscx_grid_run: white noise U[0,1) and
 its 256-level quantisation, 8x8 tiles, with the merge tree on the GPU (the
-library's choice at these sizes) and, at 4096^2, also on the host with the
-reference's out-of-core mode.  The oracle does not reach these sizes, so
+library's choice at 16384^2) and, at 4096^2, both on the GPU and on the
+host with the reference's out-of-core mode.  The oracle does not reach these sizes, so
 the result is checked by the math-fixed pins (tests/pins.py): the root's
 survivors form the Morse complex of a disc over Z/2, every leaf's surviving
 extrema's discs partition its rect, and every leaf's surviving nodes are the
@@ -58,7 +58,7 @@ def _run(side, levels, threshold, merge=None):
     print(f"[e2e] {label}: root {h['min']} min / {h['saddle']} saddles / {h['max']} max, pins hold", flush=True)
 
 
-@pytest.mark.parametrize("merge", [None, "host"])
+@pytest.mark.parametrize("merge", ["device", "host"])
 @pytest.mark.parametrize("threshold", [0.0, 0.005])
 @pytest.mark.parametrize("levels", [0, 256])
 def test_worst_case_4096(levels, threshold, merge):
