@@ -23,7 +23,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
               "-Wno-unused-command-line-argument"]
 
-DEVICE_SOURCES = ["kernels.hip", "noalloc.hip", "dmerge.hip"]
+DEVICE_SOURCES = ["kernels.hip", "noalloc.hip", "dmerge.hip", "devsort.hip"]
 HOST_SOURCES = ["runtime.cpp", "tile.cpp", "complex.cpp", "discs.cpp", "pipeline.cpp", "reader.cpp", "server.cpp",
                 "gen.cpp"]
 # the reference's dmsc command line over the C ABI (links libmscx.so)

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "complex.hpp"
+#include "devsort.hpp"
 #include <chrono>
 
 #include "runtime.hpp"
@@ -19,7 +20,7 @@
 
 namespace mscx {
 
-// Staging of one call: host codes + labels (disc), host staging of
+// Staging of one call: host codes + grouped disc cells (disc), host staging of
 // descriptors and tables (stage); the device descriptors, tables and labels
 // are a block of the device pool for the duration of the call.  Calls
 // from different threads (grids finishing together in a batch or on the
@@ -218,10 +219,19 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   std::vector<DiscPrep> dp(n);
   // sizes of the tables are known only after the critical-point pass, so
   // the codes live in their own region at the start of the host scratch
-  size_t lab_total = 0;
-  for (int i = 0; i < n; ++i) lab_total += align_up((nv[i] + nq[i]) * 4, 256);
+  size_t lab_total = 0, nelem = 0, nlab = 0;
+  std::vector<uint32_t> lab_base(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    lab_total += align_up((nv[i] + nq[i]) * 4, 256);
+    nelem += nv[i] + nq[i];
+    lab_base[i + 1] = lab_base[i] + cs[i]->size();
+  }
+  nlab = lab_base[n];
+  MSCX_REQUIRE(nlab < 0xFFFFFFFFull && nelem < (1ull << 40), MSCX_EINVAL, "too many cells for one manifold pass");
   const size_t codes_bytes = code_off[n];
-  char *host = (char *)g_disc.reserve_host(codes_bytes + lab_total);
+  // host: codes | grouped cells (one word per vertex and quad) | disc offsets
+  const size_t h_cells = codes_bytes, h_off = align_up(h_cells + 4 * nelem, 256);
+  char *host = (char *)g_disc.reserve_host(h_off + 4 * (nlab + 1));
   for (int i = 0; i < n; ++i)
     MSCX_HIP(hipMemcpyAsync(host + code_off[i], tiles[i]->d.codes, (size_t)tiles[i]->d.g.ch * tiles[i]->d.pitch,
                             hipMemcpyDeviceToHost, s));
@@ -252,9 +262,8 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   } ev_cnt;
   // 3. disc tracing on the GPU (the saddles' 1-manifolds): every slot's
   //    list is counted first, while the host builds the label tables; the
-  //    counts are scanned into each complex's offsets once the labels are
-  //    under way, and the lists are written while the host groups the
-  //    extremum discs as each leaf's labels arrive
+  //    counts are scanned into each complex's offsets once the labels and
+  //    their grouping are under way, and the lists are written after them
   tp = now_wall();
   std::vector<size_t> nslot(n), slot_base(n + 1, 0), init_base(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -347,7 +356,6 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
   tm.add_host(ST_DISC_TABLES, now_wall() - tp);
   size_t lo = 0;
-  std::vector<size_t> lab_off(n);
   for (int i = 0; i < n; ++i) {
     const mscx_tile *t = tiles[i];
     DiscDesc &D = hd[i];
@@ -364,45 +372,58 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     D.lab_owned = (const uint32_t *)(dev + o_tab + tab_off[i]);
     D.lab_self = D.lab_owned + t->K;
     D.nlab = t->K;
-    lab_off[i] = lo;
     D.lab_v = (uint32_t *)(dev + o_lab + lo);
     D.lab_q = D.lab_v + nv[i];
     lo += align_up((nv[i] + nq[i]) * 4, 256);
     D.err = (uint32_t *)(dev + o_err);
+    D.lab_base = lab_base[i];
+    D.elem_base = 0;
+  }
+  {
+    size_t eb = 0;
+    for (int i = 0; i < n; ++i) {
+      hd[i].elem_base = eb;
+      eb += nv[i] + nq[i];
+    }
   }
   int nblocks = plan_disc_blocks(hd, n, (int *)(stage + o_pref));
   *(uint32_t *)(stage + o_err) = 0;
   MSCX_HIP(hipMemcpyAsync(dev, stage, o_lab, hipMemcpyHostToDevice, s));
   launch_disc_labels((const DiscDesc *)dev, (const int *)(dev + o_pref), n, nblocks, s);
   MSCX_HIP(hipGetLastError());
-  // the labels come back in runs of leaves, each followed by an event, so
-  // that a leaf's grouping starts as soon as its own labels have arrived
-  char *labels = host + codes_bytes;
-  struct Events {
-    std::vector<hipEvent_t> e;
-    ~Events() {
-      for (hipEvent_t x : e) (void)hipEventDestroy(x);
-    }
-  } ev;
-  // Consecutive leaves' labels are adjacent on both sides, so they travel
-  // in runs of at least LAB_RUN bytes (a copy has a fixed cost of the order
-  // of 0.1 ms here): one copy and one event per run.
-  constexpr size_t LAB_RUN = 8u << 20;
-  std::vector<int> run_of(n);
-  std::vector<std::pair<int, int>> runs;  // leaves [first, last]
-  for (int i = 0; i < n; ++i) {
-    if (runs.empty() || lab_off[i] - lab_off[runs.back().first] >= LAB_RUN) runs.push_back({i, i});
-    runs.back().second = i;
-    run_of[i] = (int)runs.size() - 1;
-  }
-  ev.e.assign(runs.size(), nullptr);
-  for (auto &e : ev.e) MSCX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  for (size_t r = 0; r < runs.size(); ++r) {
-    const int a = runs[r].first, b = runs[r].second;
-    MSCX_HIP(hipMemcpyAsync(labels + lab_off[a], dev + o_lab + lab_off[a],
-                            lab_off[b] - lab_off[a] + (nv[b] + nq[b]) * 4, hipMemcpyDeviceToHost, s));
-    MSCX_HIP(hipEventRecord(ev.e[r], s));
-  }
+  // 5. grouping of the extremum discs on the GPU: the rect's vertices and
+  //    quads sorted by label (a stable radix sort of (label, cell) pairs
+  //    whose input is in raster order), the discs' offsets found in the
+  //    sorted labels; both come back in one copy each
+  const uint32_t sentinel = (uint32_t)nlab;
+  int nbits = 0;
+  while (nbits < 32 && ((uint64_t)sentinel >> nbits)) ++nbits;
+  const size_t g_tmp = sort_temp_words(nelem);
+  const size_t go_keys = 0, go_vals = align_up(4 * nelem, 256), go_keys2 = 2 * go_vals, go_vals2 = 3 * go_vals;
+  const size_t go_off = 4 * go_vals;
+  const size_t go_tmp = align_up(go_off + 4 * (nlab + 1), 256), go_pref = align_up(go_tmp + 4 * g_tmp, 256);
+  const size_t g_bytes = go_pref + 4 * (n + 1);
+  PoolBlock grpblk{device_pool().alloc(g_bytes), s};
+  char *gdev = (char *)grpblk.p;
+  std::vector<int> gpref(n + 1);
+  const int gblocks = plan_disc_group_blocks(hd, n, gpref.data());
+  MSCX_HIP(hipMemcpyAsync(gdev + go_pref, gpref.data(), 4 * (n + 1), hipMemcpyHostToDevice, s));
+  launch_disc_group_keys((const DiscDesc *)dev, (const int *)(gdev + go_pref), n, gblocks, sentinel,
+                         (uint32_t *)(gdev + go_keys), (uint32_t *)(gdev + go_vals), s);
+  MSCX_HIP(hipGetLastError());
+  const bool alt = radix_sort_pairs((uint32_t *)(gdev + go_keys), (uint32_t *)(gdev + go_vals),
+                                    (uint32_t *)(gdev + go_keys2), (uint32_t *)(gdev + go_vals2), nelem, nbits,
+                                    (uint32_t *)(gdev + go_tmp), s);
+  launch_disc_group_offsets((const uint32_t *)(gdev + (alt ? go_keys2 : go_keys)), nelem, sentinel,
+                            (uint32_t *)(gdev + go_off), s);
+  MSCX_HIP(hipGetLastError());
+  uint32_t *hcells = (uint32_t *)(host + h_cells), *hoff = (uint32_t *)(host + h_off);
+  MSCX_HIP(hipMemcpyAsync(hoff, gdev + go_off, 4 * (nlab + 1), hipMemcpyDeviceToHost, s));
+  // only the labelled cells (the sentinel sorts last); their count is the
+  // last offset, so the copy takes every cell and the host uses the front
+  MSCX_HIP(hipMemcpyAsync(hcells, gdev + (alt ? go_vals2 : go_vals), 4 * nelem, hipMemcpyDeviceToHost, s));
+  OneEvent ev_grp;
+  MSCX_HIP(hipEventRecord(ev_grp.e, s));
   uint32_t *herr = (uint32_t *)(stage + o_err);
   MSCX_HIP(hipMemcpyAsync(herr, dev + o_err, 4, hipMemcpyDeviceToHost, s));
 
@@ -459,10 +480,9 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
   tm.add_host(ST_DISC_PATHS, paths_s + now_wall() - tp);
 
-  // 5. the extremum discs (grouping, one item per leaf as its labels
-  //    arrive); a label the kernel flagged as bad is the none label, which
-  //    grouping skips.  Without the GPU lists (the chains were no gradient's
-  //    V-paths) every leaf's discs are traced here instead.
+  // 6. the extremum discs into the complexes as the grouping's copies land;
+  //    without the GPU lists (the chains were no gradient's V-paths) every
+  //    leaf's saddle discs are traced here instead
   tp = now_wall();
   std::atomic<int64_t> wait_ns(0), trace_ns(0), group_ns(0);
   try {
@@ -476,11 +496,14 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
         return;
       }
       const int i = q;
-      MSCX_HIP(hipEventSynchronize(ev.e[run_of[i]]));
+      MSCX_HIP(hipEventSynchronize(ev_grp.e));
       const double w1 = now_wall();
       wait_ns += (int64_t)(1e9 * (w1 - w0));
-      const uint32_t *lv = (const uint32_t *)(labels + lab_off[i]);
-      disc_group(*cs[i], rect[i], lv, lv + nv[i]);
+      Complex &m = *cs[i];
+      const uint32_t b0 = lab_base[i], nn = m.size();
+      m.xd_off.resize((size_t)nn + 1);
+      for (uint32_t k = 0; k <= nn; ++k) m.xd_off[k] = hoff[b0 + k] - hoff[b0];
+      m.xd_cells.assign(hcells + hoff[b0], hcells + hoff[b0 + nn]);
       group_ns += (int64_t)(1e9 * (now_wall() - w1));
     });
   } catch (...) {

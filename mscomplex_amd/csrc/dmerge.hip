@@ -1,9 +1,8 @@
 // Device-resident complexes and the merge tree on the GPU (see dmerge.hpp).
 //
 // Every kernel here is a sweep over nodes or list entries (integer work on
-// HBM-resident arrays; no step is a contraction).  Scans are three-phase
-// (block sums, scan of the sums, block-local scan plus offset), 2048 items
-// per 256-thread block.  Host code drives the kernels on the caller's
+// HBM-resident arrays; no step is a contraction); scans are devsort.hip's.
+// Host code drives the kernels on the caller's
 // stream and synchronises only where a count decides an allocation and
 // around the host pass over a shared line.
 #include <hip/hip_runtime.h>
@@ -19,6 +18,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include "devsort.hpp"
 #include "dmerge.hpp"
 #include "runtime.hpp"
 #include "tile.hpp"
@@ -46,7 +46,6 @@ enum : uint32_t {
 constexpr int SORT_SHORT = 64;     // lists sorted by one thread
 constexpr int SORT_LDS = 8192;     // longest list a workgroup sorts in LDS
 constexpr uint32_t LONGQ_CAP = 1u << 20;
-constexpr int SCAN_PER = 8, SCAN_BLK = 256 * SCAN_PER;
 
 struct DView {
   uint32_t *cell, *pair;
@@ -108,64 +107,6 @@ __device__ bool map_insert(unsigned long long *map, uint32_t mask, uint32_t key,
 }
 
 __device__ __forceinline__ void set_err(uint32_t *err, uint32_t bit) { atomicOr(err, bit); }
-
-// 256-thread exclusive scan of one value per thread (wave64 shuffles).
-__device__ uint32_t block_scan(uint32_t v, uint32_t &total) {
-  __shared__ uint32_t sw[4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) sw[w] = x;
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (q < w) base += sw[q];
-    tot += sw[q];
-  }
-  __syncthreads();
-  total = tot;
-  return base + x - v;
-}
-
-// ------------------------------------------------------------------ scans
-
-__global__ __launch_bounds__(256) void k_scan_sum(const uint32_t *__restrict__ in, uint64_t n,
-                                                  uint32_t *__restrict__ part) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_BLK + (uint64_t)threadIdx.x * SCAN_PER;
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k)
-    if (base + k < n) s += in[base + k];
-  uint32_t tot;
-  block_scan(s, tot);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-// in and out may be the same array
-__global__ __launch_bounds__(256) void k_scan_down(const uint32_t *in, uint32_t *out, uint64_t n,
-                                                   const uint32_t *__restrict__ partex, uint32_t *total) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_BLK + (uint64_t)threadIdx.x * SCAN_PER;
-  uint32_t v[SCAN_PER];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0u;
-    s += v[k];
-  }
-  uint32_t tot;
-  uint32_t ex = block_scan(s, tot) + (partex ? partex[blockIdx.x] : 0u);
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    if (base + k < n) out[base + k] = ex;
-    ex += v[k];
-  }
-  if (total && threadIdx.x == 0) *total = tot;
-}
 
 // ------------------------------------------------------------- list sorts
 
@@ -834,19 +775,7 @@ struct Temps {
 
 // exclusive scan of n words; the total lands in *d_total when given
 void dscan(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total, hipStream_t s, Temps &T) {
-  if (n == 0) {
-    if (d_total) MSCX_HIP(hipMemsetAsync(d_total, 0, 4, s));
-    return;
-  }
-  const uint64_t nb = (n + SCAN_BLK - 1) / SCAN_BLK;
-  if (nb == 1) {
-    k_scan_down<<<1, 256, 0, s>>>(in, out, n, nullptr, d_total);
-    return;
-  }
-  uint32_t *part = T.get<uint32_t>(nb);
-  k_scan_sum<<<(unsigned)nb, 256, 0, s>>>(in, n, part);
-  dscan(part, part, nb, d_total, s, T);
-  k_scan_down<<<(unsigned)nb, 256, 0, s>>>(in, out, n, part, nullptr);
+  dev_scan(in, out, n, d_total, T.get<uint32_t>(scan_temp_words(n)), s);
 }
 
 void sort_lists(DComplex &d, hipStream_t s, Temps &T, uint32_t *d_err) {

@@ -813,6 +813,57 @@ __global__ __launch_bounds__(256) void k_disc_labels(const DiscDesc *__restrict_
   if (hasq) D.lab_q[(int64_t)y * (nvx - 1) + x] = lq;
 }
 
+// Sort input of the extremum discs' grouping (postMergeFillDiscs,
+// grid_dataset.cpp:1142-1187, appends every vertex / quad of the rect to
+// its extremum's disc in raster order): each vertex and quad of the rect,
+// vertices then quads in raster order, keyed by its label plus the leaf's
+// label base (unlabelled ones by the sentinel, which sorts last), valued by
+// its cell id.  A stable sort by key then lists every disc's cells in
+// raster order (one key per basin: counting them with atomics would
+// serialise on a large basin's counter, so the offsets come from the sorted
+// keys instead, k_disc_group_offsets).
+__global__ __launch_bounds__(256) void k_disc_group_keys(const DiscDesc *__restrict__ descs,
+                                                         const int *__restrict__ prefix, int n, uint32_t sentinel,
+                                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, n, b);
+  const DiscDesc &D = descs[t];
+  const int64_t nvx = (D.rx1 - D.rx0) / 2 + 1, nvy = (D.ry1 - D.ry0) / 2 + 1;
+  const int64_t nv = nvx * nvy, nq = (nvx - 1) * (nvy - 1);
+  const int64_t e = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
+  if (e >= nv + nq) return;
+  uint32_t lab;
+  int x, y;
+  if (e < nv) {
+    lab = D.lab_v[e];
+    x = D.rx0 + 2 * (int)(e % nvx);
+    y = D.ry0 + 2 * (int)(e / nvx);
+  } else {
+    const int64_t e2 = e - nv, w = nvx - 1;
+    lab = D.lab_q[e2];
+    x = D.rx0 + 1 + 2 * (int)(e2 % w);
+    y = D.ry0 + 1 + 2 * (int)(e2 / w);
+  }
+  const uint32_t key = lab == 0xFFFFFFFFu ? sentinel : D.lab_base + lab;
+  keys[D.elem_base + e] = key;
+  vals[D.elem_base + e] = pack_cell(x, y);
+}
+
+// Disc offsets: off[k] = the first position of a key >= k in the sorted
+// keys, for k in [0, nkeys] (off[nkeys] = the labelled cells' count).
+__global__ __launch_bounds__(256) void k_disc_group_offsets(const uint32_t *__restrict__ sorted, uint64_t n,
+                                                            uint32_t nkeys, uint32_t *__restrict__ off) {
+  const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k > nkeys) return;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < (uint32_t)k) lo = mid + 1;
+    else hi = mid;
+  }
+  off[k] = (uint32_t)lo;
+}
+
 // ------------------------------------------------------- disc tracing
 // postMergeFillDiscs' saddle 1-manifolds (grid_dataset.cpp:1189-1228).  The
 // reference walks a growing list: every entry appends its successors (its
@@ -1111,6 +1162,25 @@ int plan_disc_blocks(const DiscDesc *d, int n, int *prefix) {
 
 void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s) {
   if (nblocks) hipLaunchKernelGGL(k_disc_labels, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
+}
+int plan_disc_group_blocks(const DiscDesc *d, int n, int *prefix) {
+  int acc = 0;
+  for (int t = 0; t < n; ++t) {
+    prefix[t] = acc;
+    const int64_t nvx = (d[t].rx1 - d[t].rx0) / 2 + 1, nvy = (d[t].ry1 - d[t].ry0) / 2 + 1;
+    acc += (int)((nvx * nvy + (nvx - 1) * (nvy - 1) + 255) / 256);
+  }
+  prefix[n] = acc;
+  return acc;
+}
+void launch_disc_group_keys(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, uint32_t sentinel,
+                            uint32_t *keys, uint32_t *vals, hipStream_t s) {
+  if (nblocks)
+    hipLaunchKernelGGL(k_disc_group_keys, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n, sentinel, keys, vals);
+}
+void launch_disc_group_offsets(const uint32_t *sorted, uint64_t n, uint32_t nkeys, uint32_t *off, hipStream_t s) {
+  hipLaunchKernelGGL(k_disc_group_offsets, dim3((unsigned)((nkeys + 256) / 256)), dim3(256), 0, s, sorted, n, nkeys,
+                     off);
 }
 void launch_grad_local(const TileDesc *d_tiles, const int *d_prefix, int ntiles, int nblocks, uint32_t *d_err,
                        hipStream_t s) {

@@ -33,6 +33,13 @@ void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, 
 // Manifold labels for ndisc leaves; prefix has ndisc+1 entries of 256-item blocks.
 int plan_disc_blocks(const DiscDesc *d, int n, int *prefix);
 void launch_disc_labels(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, hipStream_t s);
+// Sort input of the extremum discs' grouping over n leaves (prefix: n+1
+// block offsets of 256 elements, vertices then quads of each rect).
+int plan_disc_group_blocks(const DiscDesc *d, int n, int *prefix);
+void launch_disc_group_keys(const DiscDesc *d_desc, const int *d_prefix, int n, int nblocks, uint32_t sentinel,
+                            uint32_t *keys, uint32_t *vals, hipStream_t s);
+// off[k] for k in [0, nkeys]: the first position of a key >= k in sorted[0, n).
+void launch_disc_group_offsets(const uint32_t *sorted, uint64_t n, uint32_t nkeys, uint32_t *off, hipStream_t s);
 // The stage chain of run_tiles up to the critical counts (upload of the
 // descriptor block, gradient, scan, ring resolution, owner images, download
 // of the counts) as a HIP graph built node by node, so that no stream is

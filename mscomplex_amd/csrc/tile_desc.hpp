@@ -129,6 +129,10 @@ struct DiscDesc {
   uint32_t *lab_v;  // per interior vertex, raster order: critical-point index or ~0
   uint32_t *lab_q;  // per interior quad
   uint32_t *err;    // set when an owner is not a surviving extremum
+  // grouping: the leaf's first sort key (labels are offset by it) and its
+  // first element (its vertices, then its quads) in the sort input
+  uint32_t lab_base;
+  uint64_t elem_base;
 };
 
 constexpr uint32_t LAB_ERR = 0xFFFFFFFEu;  // label table entry: owner is no surviving extremum
