@@ -904,9 +904,18 @@ void un_simplify(Complex &m, const std::vector<std::pair<uint32_t, uint32_t>> &c
 
 static inline uint32_t pack_xy(int x, int y) { return (uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16); }
 
-void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out) {
+uint32_t surviving_extremum(const Complex &m, uint32_t c, uint32_t dim) {
+  const uint32_t i = m.idx.find(c);
+  // (a host-merged complex's map may hold zero entries for cells it lacks)
+  if (i == UINT32_MAX || i >= m.size() || m.cell[i] != c) return UINT32_MAX;
+  return (cell_dim(c) == dim && !(m.flags[i] & CPF_BCANC)) ? i : UINT32_MAX;
+}
+
+void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out, bool with_maps) {
   auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
-  CellMap redir;  // cell -> slot in out.redirect (later assignments overwrite)
+  CellMap &redir = out.redir;  // cell -> slot in out.redirect (later assignments overwrite)
+  redir = CellMap();
+  redir.reserve(m.size() / 4 + 16);
   out.redirect.clear();
   out.minmap.clear();
   out.maxmap.clear();
@@ -964,7 +973,7 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
         }
       } else {
         if (dim == 0) {
-          if (pass == 0 && !seen_min.contains(c)) {  // map::insert keeps the first
+          if (pass == 0 && with_maps && !seen_min.contains(c)) {  // map::insert keeps the first
             seen_min.set(c, i);
             out.minmap.emplace_back(c, i);
           }
@@ -986,7 +995,7 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
             out.traced[2 * (size_t)i] = out.traced[2 * (size_t)i + 1] = 1;
           }
         } else {
-          if (pass == 0 && !seen_max.contains(c)) {
+          if (pass == 0 && with_maps && !seen_max.contains(c)) {
             seen_max.set(c, i);
             out.maxmap.emplace_back(c, i);
           }
@@ -994,7 +1003,10 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
       }
     }
   }
+  if (!with_maps) return;
+  // sorted: the slots in redir move with their entries
   std::sort(out.redirect.begin(), out.redirect.end());
+  for (uint32_t k = 0; k < out.redirect.size(); ++k) redir.set(out.redirect[k].first, k);
   std::sort(out.minmap.begin(), out.minmap.end());
   std::sort(out.maxmap.begin(), out.maxmap.end());
 }

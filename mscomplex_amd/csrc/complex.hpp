@@ -274,8 +274,17 @@ struct CodesView {
   uint8_t at(int x, int y) const { return p[(int64_t)(y - oy) * pitch + (x - ox)]; }
 };
 struct DiscPrep {
-  std::vector<std::pair<uint32_t, uint32_t>> redirect;  // (cell, owner), sorted by cell
-  std::vector<std::pair<uint32_t, uint32_t>> minmap, maxmap;  // (cell, cp index), sorted by cell
+  // (cell, owner), sorted by cell when built with the maps; redir finds a
+  // cell's entry either way
+  std::vector<std::pair<uint32_t, uint32_t>> redirect;
+  CellMap redir;
+  // (cell, cp index) of the surviving minima / maxima, sorted by cell (only
+  // when built with the maps)
+  std::vector<std::pair<uint32_t, uint32_t>> minmap, maxmap;
+  uint32_t redirect_of(uint32_t cell) const {
+    const uint32_t s = redir.find(cell);
+    return s == UINT32_MAX ? UINT32_MAX : redirect[s].second;
+  }
   std::vector<uint32_t> surv_saddles;
   // the entries postMergeFillDiscs appends to each disc before tracing, in
   // its order (CSR by slot 2 i + j), and per slot whether it is traced (the
@@ -289,7 +298,13 @@ void disc_trace_one(const uint32_t *init, size_t n_init, int j, const Rect &rect
                     const CodesView &cv, std::vector<uint32_t> &out);
 // Every slot's disc (traced or copied) into m.sd_off / m.sd_cells.
 void disc_trace_host(Complex &m, const DiscPrep &dp, const Rect &rect, const Rect &ext, const CodesView &cv);
-void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out);
+// The critical-point pass.  with_maps = false (the GPU manifold pass, which
+// looks surviving extrema up through the complex's own cell map) leaves the
+// sorted vectors out.
+void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &out, bool with_maps = true);
+// The surviving (not boundary-cancellable) extremum of dimension dim on
+// cell c, or UINT32_MAX: what a lookup in minmap / maxmap gives.
+uint32_t surviving_extremum(const Complex &m, uint32_t c, uint32_t dim);
 void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, const uint32_t *own, int64_t own_pitch,
                       std::vector<uint32_t> &lv, std::vector<uint32_t> &lq);
 void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq);

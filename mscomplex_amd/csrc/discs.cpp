@@ -141,10 +141,6 @@ static std::vector<Chunk> chunks_of(const std::vector<size_t> &count, size_t chu
 // that starts at 0).
 static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, const DiscPrep &dp,
                               const mscx_tile *t, uint32_t *owned, uint32_t *self, uint32_t kb, uint32_t ke) {
-  auto find = [](const std::vector<std::pair<uint32_t, uint32_t>> &v, uint32_t key) {
-    auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(key, (uint32_t)0));
-    return (it != v.end() && it->first == key) ? it->second : UINT32_MAX;
-  };
   auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
   auto owner_of = [&](uint32_t c) -> uint32_t {  // own[c] for a cell c inside the ext
     uint32_t i = m.idx.find(c);
@@ -155,13 +151,15 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
     return oi == OWN_NONE ? UINT32_MAX : m.cell[oi];
   };
   auto own2 = [&](uint32_t c) {
-    uint32_t r = find(dp.redirect, c);
+    uint32_t r = dp.redirect_of(c);
     return r != UINT32_MAX ? r : owner_of(c);
   };
-  auto label = [&](uint32_t o, const std::vector<std::pair<uint32_t, uint32_t>> &mp) -> uint32_t {
+  // minmap / maxmap lookups (the surviving extrema by cell) through the
+  // complex's cell map
+  auto label = [&](uint32_t o, uint32_t edim) -> uint32_t {
     if (in_r(o)) o = own2(o);
     if (o == UINT32_MAX || cx_of(o) == -1 || cy_of(o) == -1) return UINT32_MAX;
-    uint32_t k = find(mp, o);
+    uint32_t k = surviving_extremum(m, o, edim);
     return k == UINT32_MAX ? LAB_ERR : k;
   };
   // a redirect at a vertex or quad that the leaf does not hold as critical
@@ -180,10 +178,9 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
       owned[k] = self[k] = LAB_ERR;
       continue;
     }
-    const auto &mp = dim == 0 ? dp.minmap : dp.maxmap;
-    owned[k] = label(c, mp);
-    const uint32_t r = find(dp.redirect, c);
-    self[k] = label(r != UINT32_MAX ? r : c, mp);
+    owned[k] = label(c, dim);
+    const uint32_t r = dp.redirect_of(c);
+    self[k] = label(r != UINT32_MAX ? r : c, dim);
   }
 }
 
@@ -249,7 +246,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     const std::vector<Chunk> whole = chunks_of(nk, ~(size_t)0);  // one item per leaf, largest first
     par_for((int)whole.size(), [&](int q) {
       const int i = whole[q].leaf;
-      disc_prepare(*cs[i], rect[i], cv[i], dp[i]);
+      disc_prepare(*cs[i], rect[i], cv[i], dp[i], false);
     });
   }
 
