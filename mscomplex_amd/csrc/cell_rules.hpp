@@ -507,50 +507,45 @@ MSCX_HD void row_next(const KeyWin &K, const TileGeom &g, int &vc11, int &ec10) 
   ec10 = e;
 }
 
-// The four codes of cells (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1)
-// (code_from_cands) from the seven candidates they depend on.  Cells
-// outside the ext get code 0.
+// Pair directions of the cells (2i, 2j), (2i+1, 2j), (2i, 2j+1),
+// (2i+1, 2j+1) (code_from_cands) from the seven candidates they depend on.
+// An edge without a usable quad is paired with its higher end when that
+// vertex's candidate is this edge: a vertex only picks an edge to a lower
+// neighbour, so the candidate alone says which end is higher, and at most
+// one end can pick the edge.  A quad is paired with its largest edge when
+// that edge's candidate is the quad; an edge only picks a quad whose largest
+// edge it is, so at most one of the four edges can point at the quad and
+// the one that does is its largest.
+MSCX_HD void row_cell_dirs(int vc00, int vc10, int hc00, int vc01, int hc01, int ec00, int ec10, int &dv, int &dh,
+                           int &de, int &dq) {
+  dv = vc00;
+  const int hv = vc00 == DIR_PX ? DIR_NX : (vc10 == DIR_NX ? DIR_PX : DIR_NONE);
+  dh = hc00 != DIR_NONE ? hc00 : hv;
+  const int ev = vc00 == DIR_PY ? DIR_NY : (vc01 == DIR_NY ? DIR_PY : DIR_NONE);
+  de = ec00 != DIR_NONE ? ec00 : ev;
+  const int q1 = ec10 == DIR_NX ? DIR_PX : DIR_NONE;
+  const int q2 = ec00 == DIR_PX ? DIR_NX : q1;
+  const int q3 = hc01 == DIR_NY ? DIR_PY : q2;
+  dq = hc00 == DIR_PY ? DIR_NY : q3;
+}
+
+// Code of a cell strictly inside the interior rect from its pair direction.
+MSCX_HD uint8_t interior_code(int d) { return (uint8_t)((d ? CF_PAIRED : CF_CRITICAL) | (d << 2)); }
+
+// The four codes of the same cells.  Cells outside the ext get code 0.
 template <bool INTERIOR>
 MSCX_HD void row_cell_codes(const KeyWin &K, const TileGeom &g, int vc00, int vc10, int hc00, int vc01, int hc01,
                             int ec00, int ec10, uint8_t &cv, uint8_t &chh, uint8_t &ce, uint8_t &cq) {
   const int i = K.i, j = K.j;
   const VClass c0 = vclass(i, g.vx, g.rx0, g.rx1, g.cw - 1), c1 = vclass(i + 1, g.vx, g.rx0, g.rx1, g.cw - 1);
   const VClass r0 = vclass(j, g.vy, g.ry0, g.ry1, g.ch - 1), r1 = vclass(j + 1, g.vy, g.ry0, g.ry1, g.ch - 1);
-  // codes (code_from_cands), all values computed before they are selected
-  const int dv = vc00;
-  // an edge without a usable quad takes its higher end's vertex if that
-  // vertex's candidate is this edge
-  const bool h_hi = K.w11 > K.w12;
-  const bool h_v = (h_hi & (vc00 == DIR_PX)) | (!h_hi & (vc10 == DIR_NX));
-  const int h_d = h_hi ? DIR_NX : DIR_PX;
-  const int dh = hc00 != DIR_NONE ? hc00 : (h_v ? h_d : DIR_NONE);
-  const bool e_hi = K.w11 > K.w21;
-  const bool e_v = (e_hi & (vc00 == DIR_PY)) | (!e_hi & (vc01 == DIR_NY));
-  const int e_d = e_hi ? DIR_NY : DIR_PY;
-  const int de = ec00 != DIR_NONE ? ec00 : (e_v ? e_d : DIR_NONE);
-  int dq;
-  {
-    // the quad is paired with its largest edge when that edge's candidate is
-    // this quad
-    const uint32_t k00 = K.w11, k10 = K.w12, k01 = K.w21, k11 = K.w22;
-    // position order 00 < 01 < 10 < 11 (x, then y)
-    const bool g00_10 = k00 > k10, g00_01 = k00 > k01, g00_11 = k00 > k11;
-    const bool g10_01 = k10 >= k01, g10_11 = k10 > k11, g01_11 = k01 > k11;
-    const int m00 = g10_01 ? DIR_NY : DIR_NX, m10 = g00_11 ? DIR_NY : DIR_PX;
-    const int m01 = g00_11 ? DIR_NX : DIR_PY, m11 = g10_01 ? DIR_PX : DIR_PY;
-    const int r2 = g01_11 ? m01 : m11;
-    const int r1 = (g10_01 && g10_11) ? m10 : r2;
-    const int edir = (g00_10 && g00_01 && g00_11) ? m00 : r1;
-    const int ea = edir == DIR_NX ? ec00 : ec10, eb = edir == DIR_PY ? hc01 : ea;
-    const int ec = edir == DIR_NY ? hc00 : eb;
-    const int flip = ((edir - 1) ^ 1) + 1;  // edir is 1..4
-    dq = ec == flip ? edir : DIR_NONE;
-  }
+  int dv, dh, de, dq;
+  row_cell_dirs(vc00, vc10, hc00, vc01, hc01, ec00, ec10, dv, dh, de, dq);
   if (INTERIOR) {
-    cv = (uint8_t)((dv ? CF_PAIRED : CF_CRITICAL) | (dv << 2));
-    chh = (uint8_t)((dh ? CF_PAIRED : CF_CRITICAL) | (dh << 2));
-    ce = (uint8_t)((de ? CF_PAIRED : CF_CRITICAL) | (de << 2));
-    cq = (uint8_t)((dq ? CF_PAIRED : CF_CRITICAL) | (dq << 2));
+    cv = interior_code(dv);
+    chh = interior_code(dh);
+    ce = interior_code(de);
+    cq = interior_code(dq);
   } else {
     // flags: in rect (exterior unpaired cells are not critical) and the
     // boundary-cancellable sites of a cell and of its pair

@@ -94,6 +94,25 @@ __device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, ui
   return (c & CF_CRITICAL) ? root : paired;
 }
 
+// link_of for an entry strictly inside the interior rect, from its pair
+// direction d (0: critical, its root terminal).
+__device__ __forceinline__ uint32_t link_dir(int d, int li, int lj, int q, uint32_t rank) {
+  constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)ENT_W << 16) | ((uint32_t)(256 - ENT_W) << 24);
+  const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
+  const int step = __builtin_amdgcn_sbfe((int)STEP, 8 * (d - 1), 8);
+  const uint32_t link = 2u * (uint32_t)(ent_at(q, li, lj) + step);
+  return d ? link : root;
+}
+
+// Two interior cells' codes (interior_code of da | interior_code of db << 8)
+// by one byte permute: selector byte k picks table byte k of the 8-byte
+// table {2, 5, 9, 13 | 17, 0, 0, 0} (codes of directions 0..4).
+__device__ __forceinline__ uint32_t interior_codes2(int da, int db) {
+  static_assert(CF_PAIRED == 1 && CF_CRITICAL == 2 && DIR_NY == 4, "code table");
+  constexpr uint32_t LO = 0x0D090502u, HI = 0x00000011u;
+  return __builtin_amdgcn_perm(HI, LO, (uint32_t)da | ((uint32_t)db << 8)) & 0xFFFFu;
+}
+
 // Step 2 for one wave: rows lj0 .. lj0 + GRAD_TY/4 - 1 of the block tile,
 // with the row-step rules on a register window.  A lane computes the new
 // candidates of its own column (row_own) and takes the next column's from
@@ -145,7 +164,6 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
   uint32_t crow = (uint32_t)((int64_t)(2 * K.j) * pitch + 2 * i);
   int64_t rs0 = (int64_t)(2 * K.j) * nseg + bx;
   GLB uint64_t *const critbits = glb(T.critbits);
-  GLB uint32_t *const seg_counts = glb(T.seg_counts);
   // interior blocks lie inside the ext: every row exists, and the walk is
   // unrolled so that the window slides by register renaming
   constexpr int UNROLL = INTERIOR ? RW : 1;
@@ -160,30 +178,48 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
     const int sv = __shfl_down(vc01, 1, 64), se = __shfl_down(ec00, 1, 64);
     const int xv1 = __builtin_amdgcn_readlane(xv, rr + 1), xe0 = __builtin_amdgcn_readlane(xe, rr);
     const int vc11 = lane == 63 ? xv1 : sv, ec10 = lane == 63 ? xe0 : se;
-    uint8_t cv, chh, ce, cq;
-    row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
-    const bool ein = INTERIOR || j + 1 < g.vy;  // uniform across the wave
-    *(GLB uint16_t *)(codes + crow) = (uint16_t)(cv | (chh << 8));
-    const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
-    // with no odd row below (ext edge) its words are not written
-    const uint64_t b2 = __ballot(ein && (ce & CF_CRITICAL)), b3 = __ballot(ein && (cq & CF_CRITICAL));
-    if (ein) *(GLB uint16_t *)(codes + (crow + (uint32_t)pitch)) = (uint16_t)(ce | (cq << 8));
-    if (lane == 0) {
-      *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
-      seg_counts[rs0] = (uint32_t)(__popcll(b0) + __popcll(b1));
-      if (ein) {
+    if constexpr (INTERIOR) {
+      // every cell is in the rect: a cell is critical exactly when it has
+      // no pair, and its code is a function of its pair direction
+      int dv, dh, de, dq;
+      row_cell_dirs(vc00, vc10, hc00, vc01, hc01, ec00, ec10, dv, dh, de, dq);
+      *(GLB uint16_t *)(codes + crow) = (uint16_t)interior_codes2(dv, dh);
+      *(GLB uint16_t *)(codes + (crow + (uint32_t)pitch)) = (uint16_t)interior_codes2(de, dq);
+      const uint64_t b0 = __ballot(dv == DIR_NONE), b1 = __ballot(dh == DIR_NONE);
+      const uint64_t b2 = __ballot(de == DIR_NONE), b3 = __ballot(dq == DIR_NONE);
+      if (lane == 0) {
+        *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
         *(GLB u64x2 *)(critbits + 2 * (rs0 + nseg)) = u64x2{b2, b3};
-        seg_counts[rs0 + nseg] = (uint32_t)(__popcll(b2) + __popcll(b3));
       }
+      // ranks among the row segment's critical cells before this lane's
+      // (the quad's also counts the edge of its own column: bits 0..lane of b2)
+      const uint32_t rank_v = rank_below(b1, rank_below(b0, 0));
+      const uint32_t rank_q = rank_below(b3, rank_below(b2 >> 1, (uint32_t)(b2 & 1u)));
+      crow += 2 * (uint32_t)pitch;
+      rs0 += 2 * nseg;
+      sl[ent_at(0, li, lj)] = (uint16_t)link_dir(dv, li, lj, 0, rank_v);
+      sl[ent_at(1, li, lj)] = (uint16_t)link_dir(dq, li, lj, 1, rank_q);
+    } else {
+      uint8_t cv, chh, ce, cq;
+      row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
+      const bool ein = j + 1 < g.vy;  // uniform across the wave
+      *(GLB uint16_t *)(codes + crow) = (uint16_t)(cv | (chh << 8));
+      const uint64_t b0 = __ballot(cv & CF_CRITICAL), b1 = __ballot(chh & CF_CRITICAL);
+      // with no odd row below (ext edge) its words are not written
+      const uint64_t b2 = __ballot(ein && (ce & CF_CRITICAL)), b3 = __ballot(ein && (cq & CF_CRITICAL));
+      if (ein) *(GLB uint16_t *)(codes + (crow + (uint32_t)pitch)) = (uint16_t)(ce | (cq << 8));
+      if (lane == 0) {
+        *(GLB u64x2 *)(critbits + 2 * rs0) = u64x2{b0, b1};
+        if (ein) *(GLB u64x2 *)(critbits + 2 * (rs0 + nseg)) = u64x2{b2, b3};
+      }
+      const uint32_t rank_v = rank_below(b1, rank_below(b0, 0));
+      const uint32_t rank_q = rank_below(b3, rank_below(b2, 0)) + (uint32_t)((b2 >> lane) & 1u);
+      crow += 2 * (uint32_t)pitch;
+      rs0 += 2 * nseg;
+      // codes outside the ext are 0, so those entries get T_NONE
+      sl[ent_at(0, li, lj)] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
+      sl[ent_at(1, li, lj)] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
     }
-    // ranks among the row segment's critical cells before this lane's
-    const uint32_t rank_v = rank_below(b1, rank_below(b0, 0));
-    const uint32_t rank_q = rank_below(b3, rank_below(b2, 0)) + (uint32_t)((b2 >> lane) & 1u);
-    crow += 2 * (uint32_t)pitch;
-    rs0 += 2 * nseg;
-    // codes outside the ext are 0, so those entries get T_NONE
-    sl[ent_at(0, li, lj)] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
-    sl[ent_at(1, li, lj)] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
     vc00 = vc01;
     vc10 = vc11;
     hc00 = hc01;
@@ -265,7 +301,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   // halo, and the next wave's first row its walk also evaluates) are
   // interior, so boundary block rows run most of their waves without
   // boundary tests; the keys it reads are then all inside the ext.
-  const int lj0 = (tid >> 6) * RW;
+  const int lj0 = wv * RW;
   const int wj0 = j0 + wv * RW;
   const bool wave_in = cols_in && (2 * wj0 - 1 > g.ry0) && (2 * (wj0 + RW) + 1 < g.ry1);
   if (wave_in)
@@ -310,16 +346,19 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     constexpr int PER = 2 * NV / 256;
     // links are byte offsets into sl, below the terminal bit; a terminal
     // masked by ENT_MASK is still an offset inside sl (T_ROOT's row and rank
-    // bits stay below 0x2000, T_EXIT's side and position below 0x2100,
-    // T_NONE's are 0), so the read for a terminal entry is harmless
-    constexpr uint32_t ENT_MASK = 0x3FFEu;
-    static_assert(4 * ENT_N <= 0x4000 && 0x2000 + 0x100 < 2 * 2 * ENT_N, "entry offsets and masked terminals");
+    // bits, T_EXIT's 0x2000 with side and position, T_NONE's 0), so the read
+    // for a terminal entry is harmless
+    constexpr uint32_t ENT_MASK = 4 * ENT_N <= 0x2000 ? 0x1FFEu : 0x3FFEu;
+    static_assert(4 * ENT_N <= ENT_MASK + 2 && ((((2u * TY - 1) << 7) | 127u) & ENT_MASK) < 4 * ENT_N &&
+                      ((T_EXIT | 0xFFu) & ENT_MASK) < 4 * ENT_N,
+                  "entry offsets and masked terminals");
     const char *slb = (const char *)sl;
     // this thread's entries: (li, lj) = (tid & 63, (tid >> 6) + 4 m) of
-    // forest q, for k = 8 q + m; their slots are a fixed stride apart
+    // forest q, for k = MPER q + m; their slots are a fixed stride apart
+    constexpr int MPER = TY / 4;
     const int e0 = ent_at(0, tid & 63, tid >> 6);
-    auto slot = [&](int k) { return e0 + (k >> 3) * ENT_N + (k & 7) * 4 * ENT_W; };
-    static_assert(PER == 16 && TY == 32 && TX == 64, "entry walk layout");
+    auto slot = [&](int k) { return e0 + (k / MPER) * ENT_N + (k % MPER) * 4 * ENT_W; };
+    static_assert(PER == 2 * MPER && TY % 4 == 0 && TX == 64, "entry walk layout");
     uint32_t cur[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) cur[k] = sl[slot(k)];
@@ -574,6 +613,13 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
   return wbase + x - v;
 }
 
+// Critical cells of (cell row, segment) item i: the popcount of its two
+// critical words.
+__device__ __forceinline__ uint32_t seg_count(const TileDesc &T, int64_t i) {
+  const u64x2 w = ((const u64x2 *)T.critbits)[i];
+  return (uint32_t)(__popcll(w[0]) + __popcll(w[1]));
+}
+
 __global__ __launch_bounds__(256) void k_scan_reduce(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
                                                      int ntiles) {
   __shared__ uint32_t s_wave[4];
@@ -586,7 +632,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const TileDesc *__restrict_
   uint32_t s = 0;
   for (int k = 0; k < SCAN_CHUNK / 256; ++k) {
     int64_t i = base + k * 256 + threadIdx.x;
-    if (i < n) s += T.seg_counts[i];
+    if (i < n) s += seg_count(T, i);
   }
   uint32_t tot;
   block_exclusive_scan_256(s, s_wave, tot);
@@ -624,7 +670,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const TileDesc *__restrict__ 
 #pragma unroll
   for (int k = 0; k < SCAN_CHUNK / 256; ++k) {
     int64_t i = base + k;
-    v[k] = (i < n) ? T.seg_counts[i] : 0;
+    v[k] = (i < n) ? seg_count(T, i) : 0;
     s += v[k];
   }
   uint32_t tot;

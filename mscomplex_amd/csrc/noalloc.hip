@@ -78,7 +78,7 @@ extern "C" int mscx_tile_work_noalloc(const int32_t rect[4], const int32_t ext[4
     ensure_init();
     Runtime &rt = runtime();
     NoallocHead h{};
-    size_t o[8];
+    size_t o[7];
     const size_t body = fixed_layout(rect, ext, h.desc, o);
     MSCX_REQUIRE(NOALLOC_HEAD + body <= NOALLOC_SCRATCH_BYTES, MSCX_EINVAL,
                  "tile too large for the no-allocation scratch (" + std::to_string(NOALLOC_HEAD + body) +
@@ -96,13 +96,12 @@ extern "C" int mscx_tile_work_noalloc(const int32_t rect[4], const int32_t ext[4
     d.verts = dev_verts;
     d.vpitch = row_pitch;
     d.codes = (uint8_t *)(b + o[0]);
-    d.seg_counts = (uint32_t *)(b + o[1]);
-    d.seg_offs = (uint32_t *)(b + o[2]);
-    d.scan_part = (uint32_t *)(b + o[3]);
-    d.critbits = (uint64_t *)(b + o[4]);
-    d.own = (uint32_t *)(b + o[5]);
-    d.loc = (uint32_t *)(b + o[6]);
-    d.ring = (uint32_t *)(b + o[7]);
+    d.seg_offs = (uint32_t *)(b + o[1]);
+    d.scan_part = (uint32_t *)(b + o[2]);
+    d.critbits = (uint64_t *)(b + o[3]);
+    d.own = (uint32_t *)(b + o[4]);
+    d.loc = (uint32_t *)(b + o[5]);
+    d.ring = (uint32_t *)(b + o[6]);
     const int64_t nblk = (int64_t)d.nseg * ((d.g.vy + GRAD_TY - 1) / GRAD_TY);
     d.ringidx = d.ring + nblk * RING_W;
     d.nkrit = &dh->k;

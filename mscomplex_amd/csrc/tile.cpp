@@ -26,8 +26,8 @@ void check_rects(const int32_t *rect, const int32_t *ext) {
 
 // Geometry of a tile over ext and the layout of its one device allocation:
 // codes, counts, offsets, scan partials, critical flags, owners, local
-// terminals, ring words.  Returns the allocation size; offsets in o[8].
-size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[8]) {
+// terminals, ring words.  Returns the allocation size; offsets in o[7].
+size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[7]) {
   TileGeom &g = d.g;
   g.vx = (ext[2] - ext[0]) / 2 + 1;
   g.vy = (ext[3] - ext[1]) / 2 + 1;
@@ -56,12 +56,11 @@ size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, si
   };
   o[0] = take((size_t)g.ch * d.pitch);
   o[1] = take(nrs * 4);
-  o[2] = take(nrs * 4);
-  o[3] = take(nparts * 4 + 4);
-  o[4] = take(nrs * 16);
-  o[5] = take((size_t)g.ch * d.own_pitch * 4);
-  o[6] = take((size_t)g.vy * d.loc_pitch * 4);
-  o[7] = take((size_t)nblk * RING_W * 4);
+  o[2] = take(nparts * 4 + 4);
+  o[3] = take(nrs * 16);
+  o[4] = take((size_t)g.ch * d.own_pitch * 4);
+  o[5] = take((size_t)g.vy * d.loc_pitch * 4);
+  o[6] = take((size_t)nblk * RING_W * 4);
   size_t o_ringidx = take((size_t)nblk * RING_W * 4);
   (void)o_ringidx;
   return off;
@@ -69,26 +68,25 @@ size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, si
 
 size_t tile_fixed_bytes(const int32_t rect[4], const int32_t ext[4]) {
   TileDesc d{};
-  size_t o[8];
+  size_t o[7];
   return fixed_layout(rect, ext, d, o);
 }
 
 static void alloc_fixed(mscx_tile *t) {
   TileDesc &d = t->d;
-  size_t o[8];
+  size_t o[7];
   const size_t bytes = fixed_layout(t->rect, t->ext, d, o);
   const int64_t nblk = (int64_t)d.nseg * ((d.g.vy + GRAD_TY - 1) / GRAD_TY);
   t->fixed_bytes = bytes;
   t->fixed = device_pool().alloc(bytes);
   char *base = (char *)t->fixed;
   d.codes = (uint8_t *)(base + o[0]);
-  d.seg_counts = (uint32_t *)(base + o[1]);
-  d.seg_offs = (uint32_t *)(base + o[2]);
-  d.scan_part = (uint32_t *)(base + o[3]);
-  d.critbits = (uint64_t *)(base + o[4]);
-  d.own = (uint32_t *)(base + o[5]);
-  d.loc = (uint32_t *)(base + o[6]);
-  d.ring = (uint32_t *)(base + o[7]);
+  d.seg_offs = (uint32_t *)(base + o[1]);
+  d.scan_part = (uint32_t *)(base + o[2]);
+  d.critbits = (uint64_t *)(base + o[3]);
+  d.own = (uint32_t *)(base + o[4]);
+  d.loc = (uint32_t *)(base + o[5]);
+  d.ring = (uint32_t *)(base + o[6]);
   d.ringidx = d.ring + nblk * RING_W;
 }
 

@@ -36,10 +36,10 @@ size_t tile_fixed_bytes(const int32_t rect[4], const int32_t ext[4]);
 // Throws MSCX_EINVAL unless (rect, ext) is a valid tile.
 void check_rects(const int32_t *rect, const int32_t *ext);
 // Geometry of a tile over ext in d and the byte offsets of its device
-// arrays in one block: o[0] codes, o[1] segment counts, o[2] their offsets,
-// o[3] scan partials, o[4] critical bits, o[5] owners, o[6] local
-// terminals, o[7] ring words (ringidx follows them); returns the block size.
-size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[8]);
+// arrays in one block: o[0] codes, o[1] segment offsets, o[2] scan
+// partials, o[3] critical bits, o[4] owners, o[5] local terminals, o[6]
+// ring words (ringidx follows them); returns the block size.
+size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, size_t o[7]);
 
 // Byte offsets of the crit / fn / conn / pair arrays for K critical cells.
 struct CritLayout {

@@ -67,10 +67,9 @@ struct TileDesc {
   int64_t vpitch;
   // gradient codes, ch rows of `pitch` bytes
   uint8_t *codes;
-  // collation: per (cell row, segment) critical counts, their exclusive
-  // scan, and the critical flags as two 64-bit words (cells 2l and 2l+1 of
-  // the segment are bit l of word 0 and word 1)
-  uint32_t *seg_counts;  // ch * nseg
+  // collation: per (cell row, segment) the critical flags as two 64-bit
+  // words (cells 2l and 2l+1 of the segment are bit l of word 0 and word 1)
+  // and the exclusive scan of their counts
   uint32_t *seg_offs;    // ch * nseg
   uint64_t *critbits;    // ch * nseg * 2
   uint32_t *scan_part;   // per scan block partial sums
