@@ -234,8 +234,10 @@ def batched_bench(n=1024, seed=7):
     print(f"[bench] batched: generated {n} grids in {t_gen:.1f}s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     sample = mx.uneven_mix(64, 64, 2048, seed + 1, aspect_max=4.0, thresholds=0.01)
-    policy, table = mx.autotune_tiling([g for g, _ in sample],
-                                       [0.0 if i % 2 else t for i, (_, t) in enumerate(sample)])
+    sample_ths = [0.0 if i % 2 else t for i, (_, t) in enumerate(sample)]
+    policy, table = mx.autotune_tiling([g for g, _ in sample], sample_ths)
+    # kernel block height per tile-height class, timed on the GPU stages
+    rows_table, rows_meas = mx.autotune_block_rows([g for g, _ in sample], sample_ths, tiling_policy=policy)
     t_tune = time.perf_counter() - t0
     tiling = [policy(g.shape[1], g.shape[0]) for g in grids]
     bufs = [mx.DeviceBuffer.from_array(g) for g in grids]
@@ -285,6 +287,8 @@ def batched_bench(n=1024, seed=7):
             "tiling_policy": {"side<256": policy(255, 255), "side<1024": policy(1023, 1023),
                               "side>=1024": policy(2048, 2048)},
             "autotune_table": {f"{b}:{c}": round(v, 4) for (b, c), v in table.items()},
+            "block_rows_policy": {"tile_rows<=" + str(b): r for b, r in rows_table} or "32 rows for every class",
+            "block_rows_gpu_stage_ms": {f"tile_rows<{b}:{r}": round(v, 3) for (b, r), v in rows_meas.items()},
             "autotune_s": round(t_tune, 2), "generate_s": round(t_gen, 2)}
 
 

@@ -72,6 +72,17 @@ int mscx_profile_read(double *ms, uint32_t *counts, int n);
 int mscx_set_graphs(int on, int *previous);
 int mscx_graph_stats(uint64_t *builds, uint64_t *replays);
 
+/* Block tile height of the gradient, trace and owner kernels (no reference
+ * counterpart; the reference has no device kernels).  A tile whose window
+ * has vy vertex rows takes rows[k] of the first class k with
+ * vy <= max_rows[k] (at most 16 classes, bounds increasing) and 32 rows when
+ * no class matches; rows[k] is 32 or 16.  Blocks are 64 vertex columns wide
+ * (one per lane of a wavefront).  Tiles created afterwards use the table;
+ * n = 0 restores the default.  Results do not depend on it.  The batched
+ * form's autotuner (mscomplex_amd.autotune_block_rows) measures both
+ * heights per class and installs its choice here. */
+int mscx_set_block_rows(int32_t n, const int32_t *max_rows, const int32_t *rows);
+
 /* Counters of the manifold pass's disc tracing since the library loaded (no
  * reference counterpart): disc slots traced on the GPU, slots traced on the
  * host (a disc whose initial entries grow more chains than one GPU thread

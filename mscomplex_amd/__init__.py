@@ -115,6 +115,18 @@ def graph_stats():
     return int(c.value), int(r.value)
 
 
+def set_block_rows(table=None):
+    """Block tile height (32 or 16 vertex rows) of the gradient, trace and
+    owner kernels by tile height: table = [(max_rows, rows), ...] with
+    increasing bounds; a tile of vy vertex rows takes the first entry with
+    vy <= max_rows, 32 rows when none matches.  None restores the default.
+    Applies to tiles created afterwards; results do not depend on it."""
+    table = list(table or [])
+    b = np.array([t[0] for t in table], np.int32)
+    r = np.array([t[1] for t in table], np.int32)
+    check(lib().mscx_set_block_rows(len(table), ptr(b) if table else None, ptr(r) if table else None))
+
+
 def disc_trace_stats():
     """(disc slots traced on the GPU, slots traced on the host, calls whose
     GPU lists were dropped) since the library loaded."""
@@ -619,6 +631,59 @@ def _pick_tiling(res, margin):
         if choice is None or el < choice[0] * (1.0 - margin):
             choice = (el, tl)
     return choice[1] if choice else (0, 0)
+
+
+def autotune_block_rows(grids, thresholds, tiling_policy=None, classes=(128, 512, 1 << 30), candidates=(32, 16),
+                        reps=2, margin=0.03):
+    """Kernel block tile height per tile-height class for batched runs,
+    chosen by timing the GPU stages.
+
+    Grids are classed by the vertex rows of their leaf tiles under
+    tiling_policy ((sx, sy) -> (lx, ly); default no tiling).  For each class
+    and each candidate height, the class's grids run as one batch (without
+    manifolds) with per-stage HIP-event timing; the GPU stage time (gradient,
+    scan, compaction, trace, owners, incidence) is the best of `reps` runs.
+    The first candidate (the library's default) is kept unless another is
+    faster by more than `margin`.  The chosen table is installed with
+    set_block_rows; returns (table, measured ms per (class bound, height))."""
+    tiling_policy = tiling_policy or (lambda sx, sy: (0, 0))
+    by_class = {}
+    for g, t in zip(grids, thresholds):
+        sy, sx = g.shape
+        lx, ly = tiling_policy(sx, sy)
+        rows = (sy - 1) // (1 << ly) + 3
+        bound = next(c for c in classes if rows < c)
+        by_class.setdefault(bound, []).append((g, t, lx, ly))
+    meas, table = {}, []
+    try:
+        for bound in classes:
+            members = by_class.get(bound)
+            if not members:
+                continue
+            res = []
+            for rows in candidates:
+                set_block_rows([(1 << 30, rows)])
+                best = float("inf")
+                for _ in range(reps):
+                    profile(True)
+                    Batch([m[0] for m in members], [m[2] for m in members], [m[3] for m in members],
+                          [m[1] for m in members], discs=False)
+                    st = profile_read()
+                    best = min(best, sum(st[k][0] for k in STAGES[:6]))
+                meas[(bound, rows)] = best
+                res.append((best, rows))
+            choice = res[0]
+            for el, rows in res[1:]:
+                if el < choice[0] * (1.0 - margin):
+                    choice = (el, rows)
+            table.append((bound - 1, choice[1]))
+    finally:
+        profile(False)
+    # drop trailing classes that keep the default height
+    while table and table[-1][1] == candidates[0]:
+        table.pop()
+    set_block_rows(table)
+    return table, meas
 
 
 def autotune_tiling(grids, thresholds, candidates=((0, 0), (1, 1), (2, 2)), classes=(256, 1024, 1 << 30),

@@ -29,6 +29,7 @@ _SIGS = {
     "mscx_profile_read": (c_int, [P, P, c_int]),
     "mscx_set_graphs": (c_int, [c_int, P]),
     "mscx_graph_stats": (c_int, [P, P]),
+    "mscx_set_block_rows": (c_int, [c_int32, P, P]),
     "mscx_disc_trace_stats": (c_int, [P]),
     "mscx_merge_stats": (c_int, [P]),
     "mscx_dev_alloc": (c_int, [c_size_t, ctypes.POINTER(P)]),

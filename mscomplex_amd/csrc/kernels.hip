@@ -46,13 +46,14 @@ __device__ __forceinline__ uint32_t pack_cell(int x, int y) { return (uint32_t)(
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 // ------------------------------------------------- gradient + local trace
-// One block = a GRAD_TX x GRAD_TY vertex tile (cells 2*GRAD_TX x 2*GRAD_TY);
+// One block = a GRAD_TX x TY vertex tile (cells 2*GRAD_TX x 2*TY; TY is the
+// tile's block height, GRAD_TY or GRAD_TY_LOW rows);
 // a lane owns one vertex column and, per vertex (i, j), the four cells
 // (2i, 2j), (2i+1, 2j), (2i, 2j+1), (2i+1, 2j+1).
 //  1. keys of the tile's vertices and a 1/2-vertex halo go to LDS (all
 //     global loads of a lane are issued before the first LDS store; interior
 //     blocks load without bounds tests)
-//  2. each wave walks GRAD_TY/4 rows with a 4x4 window of keys in registers
+//  2. each wave walks TY/4 rows with a 4x4 window of keys in registers
 //     (rows j-1..j+2, columns i-1..i+2; one new LDS row per step) and
 //     evaluates the pairing candidates it needs there (vertex_cand /
 //     hedge_cand / vedge_cand, carrying the next row's over), then the four
@@ -73,8 +74,15 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 // (li + 1, lj + 1) and whose one-entry border holds the exit terminals: the
 // border slot a step from an edge entry lands on is T_EXIT with that side
 // and position, so a link never needs to test whether it leaves the tile.
-constexpr int ENT_W = GRAD_TX + 2, ENT_H = GRAD_TY + 2, ENT_N = ENT_W * ENT_H;
-__device__ __forceinline__ int ent_at(int q, int li, int lj) { return q * ENT_N + (lj + 1) * ENT_W + li + 1; }
+constexpr int ENT_W = GRAD_TX + 2;
+template <int TY>
+constexpr int ent_n() {
+  return ENT_W * (TY + 2);
+}
+template <int TY>
+__device__ __forceinline__ int ent_at(int q, int li, int lj) {
+  return q * ent_n<TY>() + (lj + 1) * ENT_W + li + 1;
+}
 
 // Trace link of a vertex (q = 0) or quad (q = 1) entry (li, lj) from its
 // code: when critical, its own root terminal (its cell row in the block,
@@ -83,24 +91,26 @@ __device__ __forceinline__ int ent_at(int q, int li, int lj) { return q * ENT_N 
 // step by +1, -1, +ENT_W, -ENT_W), as a byte offset into the entry array (a
 // border slot when the step leaves the tile).  Every value is computed
 // before it is selected (no divergent branches).
+template <int TY>
 __device__ __forceinline__ uint32_t link_of(uint8_t c, int li, int lj, int q, uint32_t rank) {
   static_assert(ENT_W < 128, "step table holds +-ENT_W in a signed byte");
   constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)ENT_W << 16) | ((uint32_t)(256 - ENT_W) << 24);
   const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
   const uint32_t s = (uint32_t)code_dir(c) - 1u;  // side 0..3 when paired
   const int step = __builtin_amdgcn_sbfe((int)STEP, (int)(8 * s), 8);
-  const uint32_t link = 2u * (uint32_t)(ent_at(q, li, lj) + step);  // LDS byte offset
+  const uint32_t link = 2u * (uint32_t)(ent_at<TY>(q, li, lj) + step);  // LDS byte offset
   const uint32_t paired = (c & CF_PAIRED) ? link : T_NONE;
   return (c & CF_CRITICAL) ? root : paired;
 }
 
 // link_of for an entry strictly inside the interior rect, from its pair
 // direction d (0: critical, its root terminal).
+template <int TY>
 __device__ __forceinline__ uint32_t link_dir(int d, int li, int lj, int q, uint32_t rank) {
   constexpr uint32_t STEP = 0x01u | (0xFFu << 8) | ((uint32_t)ENT_W << 16) | ((uint32_t)(256 - ENT_W) << 24);
   const uint32_t root = T_ROOT | ((uint32_t)(2 * lj + q) << 7) | rank;
   const int step = __builtin_amdgcn_sbfe((int)STEP, 8 * (d - 1), 8);
-  const uint32_t link = 2u * (uint32_t)(ent_at(q, li, lj) + step);
+  const uint32_t link = 2u * (uint32_t)(ent_at<TY>(q, li, lj) + step);
   return d ? link : root;
 }
 
@@ -113,16 +123,16 @@ __device__ __forceinline__ uint32_t interior_codes2(int da, int db) {
   return __builtin_amdgcn_perm(HI, LO, (uint32_t)da | ((uint32_t)db << 8)) & 0xFFFFu;
 }
 
-// Step 2 for one wave: rows lj0 .. lj0 + GRAD_TY/4 - 1 of the block tile,
+// Step 2 for one wave: rows lj0 .. lj0 + TY/4 - 1 of the block tile,
 // with the row-step rules on a register window.  A lane computes the new
 // candidates of its own column (row_own) and takes the next column's from
 // the neighbouring lane; lane 63's next column lies outside the wave, so its
 // candidates for all the wave's rows are computed once up front, one row per
 // lane, and read with readlane.
-template <bool INTERIOR>
+template <int TY, bool INTERIOR>
 __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, const uint32_t *__restrict__ sk,
                                           uint16_t *__restrict__ sl, int i0, int j0, int bx, int lane, int lj0) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, RW = TY / 4;
+  constexpr int TX = GRAD_TX, KW = TX + 3, RW = TY / 4;
   const int li = lane, i = i0 + li;
   // column i0 + TX: vertex candidate at row j0 + lj0 + L and vertical-edge
   // candidate at the same row, lane L in 0..RW (keys from LDS rows
@@ -197,8 +207,8 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
       const uint32_t rank_q = rank_below(b3, rank_below(b2 >> 1, (uint32_t)(b2 & 1u)));
       crow += 2 * (uint32_t)pitch;
       rs0 += 2 * nseg;
-      sl[ent_at(0, li, lj)] = (uint16_t)link_dir(dv, li, lj, 0, rank_v);
-      sl[ent_at(1, li, lj)] = (uint16_t)link_dir(dq, li, lj, 1, rank_q);
+      sl[ent_at<TY>(0, li, lj)] = (uint16_t)link_dir<TY>(dv, li, lj, 0, rank_v);
+      sl[ent_at<TY>(1, li, lj)] = (uint16_t)link_dir<TY>(dq, li, lj, 1, rank_q);
     } else {
       uint8_t cv, chh, ce, cq;
       row_cell_codes<INTERIOR>(K, g, vc00, vc10, hc00, vc01, hc01, ec00, ec10, cv, chh, ce, cq);
@@ -217,8 +227,8 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
       crow += 2 * (uint32_t)pitch;
       rs0 += 2 * nseg;
       // codes outside the ext are 0, so those entries get T_NONE
-      sl[ent_at(0, li, lj)] = (uint16_t)link_of(cv, li, lj, 0, rank_v);
-      sl[ent_at(1, li, lj)] = (uint16_t)link_of(cq, li, lj, 1, rank_q);
+      sl[ent_at<TY>(0, li, lj)] = (uint16_t)link_of<TY>(cv, li, lj, 0, rank_v);
+      sl[ent_at<TY>(1, li, lj)] = (uint16_t)link_of<TY>(cq, li, lj, 1, rank_q);
     }
     vc00 = vc01;
     vc10 = vc11;
@@ -234,10 +244,10 @@ __device__ __forceinline__ void grad_rows(const TileDesc &T, const TileGeom &g, 
 
 // Keys of the block's vertices and their one-vertex halo into LDS: all of a
 // lane's global loads are issued before its first LDS store.
-template <bool INTERIOR>
+template <int TY, bool INTERIOR>
 __device__ __forceinline__ void load_keys(const TileDesc &T, const TileGeom &g, uint32_t *__restrict__ sk, int i0,
                                           int j0, int lane, int wv) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY, KW = TX + 3, KH = TY + 3;
+  constexpr int TX = GRAD_TX, KW = TX + 3, KH = TY + 3;
   constexpr int NR = (KH + 3) / 4;  // rows per wave
   const GLB float *verts = glb((const float *)T.verts);
   const int64_t vpitch = T.vpitch;
@@ -262,20 +272,17 @@ __device__ __forceinline__ void load_keys(const TileDesc &T, const TileGeom &g, 
   }
 }
 
-__global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                                    int ntiles, uint32_t *err) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY;
-  constexpr int KW = TX + 3, KH = TY + 3;  // keys of vertices [i0-1, i0+TX+1] x [j0-1, j0+TY+1]
-  constexpr int NV = TX * TY;              // entries per forest
-  constexpr int RW = TY / 4;               // vertex rows per wave
-  __shared__ uint32_t sk[KH * KW];
-  __shared__ uint16_t sl[2 * ENT_N];
-
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, ntiles, b);
-  const TileDesc T = tiles[t];
+// One block of the gradient kernel for a tile of TY-row block tiles; sk and
+// sl are the launch's LDS arrays, sized for the tallest block.
+template <int TY>
+__device__ __forceinline__ void grad_block(const TileDesc &T, int lb, uint32_t *__restrict__ sk,
+                                           uint16_t *__restrict__ sl, uint32_t *err) {
+  constexpr int TX = GRAD_TX;
+  constexpr int NV = TX * TY;  // entries per forest
+  constexpr int RW = TY / 4;   // vertex rows per wave
+  constexpr int ENT_N = ent_n<TY>();
+  constexpr int RING_Q = ring_q(TY), RING_W = ring_w(TY);
   const TileGeom g = T.g;
-  const int lb = b - prefix[t];
   const int nbx = T.nseg;
   const int bx = lb % nbx, by = lb / nbx;
   const int i0 = bx * TX, j0 = by * TY;
@@ -290,9 +297,9 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   const bool interior = cols_in && (2 * j0 - 1 > g.ry0) && (2 * (j0 + TY) + 1 < g.ry1);
   // 1. keys (0 outside the ext window; the rules never read those)
   if (interior)
-    load_keys<true>(T, g, sk, i0, j0, lane, wv);
+    load_keys<TY, true>(T, g, sk, i0, j0, lane, wv);
   else
-    load_keys<false>(T, g, sk, i0, j0, lane, wv);
+    load_keys<TY, false>(T, g, sk, i0, j0, lane, wv);
   __syncthreads();
 
   // 2. codes, critical flags per row segment, trace links
@@ -305,15 +312,15 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
   const int wj0 = j0 + wv * RW;
   const bool wave_in = cols_in && (2 * wj0 - 1 > g.ry0) && (2 * (wj0 + RW) + 1 < g.ry1);
   if (wave_in)
-    grad_rows<true>(T, g, sk, sl, i0, j0, bx, lane, lj0);
+    grad_rows<TY, true>(T, g, sk, sl, i0, j0, bx, lane, lj0);
   else
-    grad_rows<false>(T, g, sk, sl, i0, j0, bx, lane, lj0);
+    grad_rows<TY, false>(T, g, sk, sl, i0, j0, bx, lane, lj0);
   // rows past the ext edge hold no entries
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
     if (j0 + lj < g.vy) continue;
-    sl[ent_at(0, lane, lj)] = (uint16_t)T_NONE;
-    sl[ent_at(1, lane, lj)] = (uint16_t)T_NONE;
+    sl[ent_at<TY>(0, lane, lj)] = (uint16_t)T_NONE;
+    sl[ent_at<TY>(1, lane, lj)] = (uint16_t)T_NONE;
   }
   // the border: exit terminals of side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y) at
   // the position along that side (the entry's row for x sides, column for
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     // this thread's entries: (li, lj) = (tid & 63, (tid >> 6) + 4 m) of
     // forest q, for k = MPER q + m; their slots are a fixed stride apart
     constexpr int MPER = TY / 4;
-    const int e0 = ent_at(0, tid & 63, tid >> 6);
+    const int e0 = ent_at<TY>(0, tid & 63, tid >> 6);
     auto slot = [&](int k) { return e0 + (k / MPER) * ENT_N + (k % MPER) * 4 * ENT_W; };
     static_assert(PER == 2 * MPER && TY % 4 == 0 && TX == 64, "entry walk layout");
     uint32_t cur[PER];
@@ -392,17 +399,18 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
     const int lj = wv * RW + r;
     const int j = j0 + lj;
     if (j >= g.vy) break;
-    T.loc[(int64_t)j * T.loc_pitch + i0 + lane] = (uint32_t)sl[ent_at(0, lane, lj)] | ((uint32_t)sl[ent_at(1, lane, lj)] << 16);
+    T.loc[(int64_t)j * T.loc_pitch + i0 + lane] =
+        (uint32_t)sl[ent_at<TY>(0, lane, lj)] | ((uint32_t)sl[ent_at<TY>(1, lane, lj)] << 16);
   }
   uint32_t *ring = T.ring + (int64_t)lb * RING_W;
   for (int p = tid; p < RING_W; p += 256) {
     const int q = p >= RING_Q ? 1 : 0;
     const int pp = p - q * RING_Q;
     int li, lj;
-    ring_at(pp, li, lj);
+    ring_at(pp, TY, li, lj);
     uint32_t out = OWN_NONE;
     if (i0 + li + q < g.vx && j0 + lj + q < g.vy) {
-      const uint32_t w = sl[ent_at(q, li, lj)];
+      const uint32_t w = sl[ent_at<TY>(q, li, lj)];
       const uint32_t kind = term_kind(w);
       if (kind == 0) {
         // root: ext-local cell row, segment (= block column), rank in the segment
@@ -413,13 +421,29 @@ __global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__
         const int tbx = bx + (s == 0) - (s == 1), tby = by + (s == 2) - (s == 3);
         // a path never leaves the ext window, so the neighbour exists
         if (tbx >= 0 && tby >= 0 && tbx < nbx && tby * TY < g.vy)
-          out = RING_MARK | (uint32_t)((tby * nbx + tbx) * RING_W + q * RING_Q + ring_entry_base(s) + pos);
+          out = RING_MARK | (uint32_t)((tby * nbx + tbx) * RING_W + q * RING_Q + ring_entry_base(s, TY) + pos);
         else
           atomicOr(err, 8u);
       }
     }
     ring[p] = out;
   }
+}
+
+__global__ __launch_bounds__(256) void k_grad_local(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                                    int ntiles, uint32_t *err) {
+  // keys of vertices [i0-1, i0+TX+1] x [j0-1, j0+TY+1] and the trace entries
+  // of the tallest block tile
+  __shared__ uint32_t sk[(GRAD_TY + 3) * (GRAD_TX + 3)];
+  __shared__ uint16_t sl[2 * ent_n<GRAD_TY>()];
+  static_assert(GRAD_TY_LOW < GRAD_TY, "tallest block");
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, ntiles, b);
+  const TileDesc T = tiles[t];
+  if (T.ty == GRAD_TY_LOW)
+    grad_block<GRAD_TY_LOW>(T, b - prefix[t], sk, sl, err);
+  else
+    grad_block<GRAD_TY>(T, b - prefix[t], sk, sl, err);
 }
 
 // Ring resolution: every perimeter entry follows ring words (one per block
@@ -430,9 +454,9 @@ __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict
   const int b = blockIdx.x;
   const int t = find_tile(prefix, ntiles, b);
   const TileDesc T = tiles[t];
-  const int nblk = T.nseg * ((T.g.vy + GRAD_TY - 1) / GRAD_TY);
+  const int nblk = (int)tile_blocks(T.g, T.nseg, T.ty);
   const int64_t e = (int64_t)(b - prefix[t]) * 256 + threadIdx.x;
-  if (e >= (int64_t)nblk * RING_W) return;
+  if (e >= (int64_t)nblk * ring_w(T.ty)) return;
   uint32_t w = T.ring[e];
   for (int steps = 0; w != OWN_NONE && (w & RING_MARK); ++steps) {
     if (steps > nblk) {
@@ -452,18 +476,13 @@ __global__ __launch_bounds__(256) void k_ring_resolve(const TileDesc *__restrict
 // the resolved ring word it leads to); an edge takes the owner of the vertex
 // or quad it is paired with, a critical cell owns itself (its rank comes from
 // ballots over the codes of its row).
-__global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
-                                               int ntiles) {
-  constexpr int TX = GRAD_TX, TY = GRAD_TY, NV = TX * TY, RW = TY / 4;
-  __shared__ uint32_t so[2 * TY];
-  __shared__ uint32_t dv[NV], dq[NV];
-  __shared__ uint32_t exit_base[4];
-
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, ntiles, b);
-  const TileDesc T = tiles[t];
+template <int TY>
+__device__ __forceinline__ void owner_block(const TileDesc &T, int lb, uint32_t *__restrict__ so,
+                                            uint32_t *__restrict__ dv, uint32_t *__restrict__ dq,
+                                            uint32_t *__restrict__ exit_base) {
+  constexpr int TX = GRAD_TX, RW = TY / 4;
+  constexpr int RING_Q = ring_q(TY), RING_W = ring_w(TY);
   const TileGeom g = T.g;
-  const int lb = b - prefix[t];
   const int nbx = T.nseg;
   const int bx = lb % nbx, by = lb / nbx;
   const int i0 = bx * TX, j0 = by * TY;
@@ -493,7 +512,7 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
   if (tid < 4) {
     const int tbx = bx + (tid == 0) - (tid == 1), tby = by + (tid == 2) - (tid == 3);
     const bool ok = tbx >= 0 && tby >= 0 && tbx < nbx && tby * TY < g.vy;
-    exit_base[tid] = ok ? (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)ring_entry_base(tid) : 0xFFFFFFFFu;
+    exit_base[tid] = ok ? (uint32_t)(tby * nbx + tbx) * RING_W + (uint32_t)ring_entry_base(tid, TY) : 0xFFFFFFFFu;
   }
   __syncthreads();
   // ring word at offset off of forest q in block (tbx, tby), or UINT32_MAX
@@ -529,12 +548,12 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int lj = wv * RW + r;
-    const uint32_t x = lane == TX - 1 ? ring_index(bx + 1, by, 0, ring_entry_base(0) + lj)
-                                      : (lane == 0 ? ring_index(bx - 1, by, 1, ring_entry_base(1) + lj) : 0xFFFFFFFFu);
+    const uint32_t x = lane == TX - 1 ? ring_index(bx + 1, by, 0, ring_entry_base(0, TY) + lj)
+                                      : (lane == 0 ? ring_index(bx - 1, by, 1, ring_entry_base(1, TY) + lj) : 0xFFFFFFFFu);
     xs[r] = x != 0xFFFFFFFFu ? ringidx[x] : OWN_NONE;
   }
-  const uint32_t xt = wv == 0 ? ring_val(bx, by - 1, 1, ring_entry_base(3) + lane) : OWN_NONE;
-  const uint32_t xb = wv == 3 ? ring_val(bx, by + 1, 0, ring_entry_base(2) + lane) : OWN_NONE;
+  const uint32_t xt = wv == 0 ? ring_val(bx, by - 1, 1, ring_entry_base(3, TY) + lane) : OWN_NONE;
+  const uint32_t xb = wv == 3 ? ring_val(bx, by + 1, 0, ring_entry_base(2, TY) + lane) : OWN_NONE;
 #pragma unroll
   for (int k = 0; k < 2 * RW; ++k) {
     const int q = k & 1, lj = wv * RW + (k >> 1);
@@ -586,6 +605,21 @@ __global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tile
     *(GLB u32x2 *)orow = u32x2{ov, oh};
     if (ein) *(GLB u32x2 *)(orow + own_pitch) = u32x2{oe, oq};
   }
+}
+
+__global__ __launch_bounds__(256) void k_owner(const TileDesc *__restrict__ tiles, const int *__restrict__ prefix,
+                                               int ntiles) {
+  // LDS of the tallest block tile
+  __shared__ uint32_t so[2 * GRAD_TY];
+  __shared__ uint32_t dv[GRAD_TX * GRAD_TY], dq[GRAD_TX * GRAD_TY];
+  __shared__ uint32_t exit_base[4];
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, ntiles, b);
+  const TileDesc T = tiles[t];
+  if (T.ty == GRAD_TY_LOW)
+    owner_block<GRAD_TY_LOW>(T, b - prefix[t], so, dv, dq, exit_base);
+  else
+    owner_block<GRAD_TY>(T, b - prefix[t], so, dv, dq, exit_base);
 }
 
 // ---------------------------------------------------------------------- scan
@@ -1165,7 +1199,7 @@ void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, 
 
 static int blocks_of(int kind, const TileDesc &T) {
   const TileGeom &g = T.g;
-  const int64_t nblk = (int64_t)T.nseg * ((g.vy + GRAD_TY - 1) / GRAD_TY);
+  const int64_t nblk = tile_blocks(g, T.nseg, T.ty);
   switch (kind) {
     case KB_GRADIENT:
       return (int)nblk;
@@ -1174,7 +1208,7 @@ static int blocks_of(int kind, const TileDesc &T) {
       return (int)((n + SCAN_CHUNK - 1) / SCAN_CHUNK);
     }
     case KB_RING:
-      return (int)((nblk * RING_W + 255) / 256);
+      return (int)((nblk * ring_w(T.ty) + 255) / 256);
     case KB_COMPACT: {
       int64_t n = (int64_t)g.ch * T.nseg;
       return (int)((n + 4 * COMPACT_ITEMS - 1) / (4 * COMPACT_ITEMS));

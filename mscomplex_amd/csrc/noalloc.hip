@@ -102,8 +102,7 @@ extern "C" int mscx_tile_work_noalloc(const int32_t rect[4], const int32_t ext[4
     d.own = (uint32_t *)(b + o[4]);
     d.loc = (uint32_t *)(b + o[5]);
     d.ring = (uint32_t *)(b + o[6]);
-    const int64_t nblk = (int64_t)d.nseg * ((d.g.vy + GRAD_TY - 1) / GRAD_TY);
-    d.ringidx = d.ring + nblk * RING_W;
+    d.ringidx = d.ring + ring_words_max(d.g, d.nseg);
     d.nkrit = &dh->k;
     d.kcap = 0;
     int nb[KB_COUNT] = {};

@@ -5,10 +5,25 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 namespace mscx {
 
 static Scratch g_batch;  // descriptors, block prefixes, K array, flags
+
+// Block tile height by tile height (mscx_set_block_rows): the first entry
+// whose bound is at least the tile's vertex rows; GRAD_TY otherwise.
+static std::mutex g_rows_mu;
+static std::vector<std::pair<int32_t, int32_t>> g_rows;
+
+int block_rows_for(int vy) {
+  std::lock_guard<std::mutex> lk(g_rows_mu);
+  for (const auto &e : g_rows)
+    if (vy <= e.first) return e.second;
+  return GRAD_TY;
+}
 
 void check_rects(const int32_t *rect, const int32_t *ext) {
   for (int k = 0; k < 4; ++k) {
@@ -47,7 +62,8 @@ size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, si
   d.loc_pitch = d.nseg * GRAD_TX;
   const int64_t nrs = (int64_t)g.ch * d.nseg;
   const int64_t nparts = (nrs + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  const int64_t nblk = (int64_t)d.nseg * ((g.vy + GRAD_TY - 1) / GRAD_TY);
+  d.ty = block_rows_for(g.vy);
+  const int64_t nring = ring_words_max(g, d.nseg);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t at = off;
@@ -60,8 +76,8 @@ size_t fixed_layout(const int32_t rect[4], const int32_t ext[4], TileDesc &d, si
   o[3] = take(nrs * 16);
   o[4] = take((size_t)g.ch * d.own_pitch * 4);
   o[5] = take((size_t)g.vy * d.loc_pitch * 4);
-  o[6] = take((size_t)nblk * RING_W * 4);
-  size_t o_ringidx = take((size_t)nblk * RING_W * 4);
+  o[6] = take((size_t)nring * 4);
+  size_t o_ringidx = take((size_t)nring * 4);
   (void)o_ringidx;
   return off;
 }
@@ -76,7 +92,6 @@ static void alloc_fixed(mscx_tile *t) {
   TileDesc &d = t->d;
   size_t o[7];
   const size_t bytes = fixed_layout(t->rect, t->ext, d, o);
-  const int64_t nblk = (int64_t)d.nseg * ((d.g.vy + GRAD_TY - 1) / GRAD_TY);
   t->fixed_bytes = bytes;
   t->fixed = device_pool().alloc(bytes);
   char *base = (char *)t->fixed;
@@ -87,7 +102,7 @@ static void alloc_fixed(mscx_tile *t) {
   d.own = (uint32_t *)(base + o[4]);
   d.loc = (uint32_t *)(base + o[5]);
   d.ring = (uint32_t *)(base + o[6]);
-  d.ringidx = d.ring + nblk * RING_W;
+  d.ringidx = d.ring + ring_words_max(d.g, d.nseg);
 }
 
 CritLayout::CritLayout(size_t K) {
@@ -428,6 +443,23 @@ int mscx_tile_download_critical(const mscx_tile *t, float *fn, uint32_t *conn4, 
     if (fn) memcpy(fn, f.data(), f.size() * 4);
     if (conn4) memcpy(conn4, cn.data(), cn.size() * 4);
     if (pair_idx) memcpy(pair_idx, p.data(), p.size() * 4);
+  });
+}
+
+int mscx_set_block_rows(int32_t n, const int32_t *max_rows, const int32_t *rows) {
+  return mscx::guarded([&] {
+    MSCX_REQUIRE(n >= 0 && n <= 16, MSCX_EINVAL, "at most 16 block-height classes");
+    MSCX_REQUIRE(n == 0 || (max_rows && rows), MSCX_EINVAL, "null class array");
+    std::vector<std::pair<int32_t, int32_t>> t;
+    for (int32_t k = 0; k < n; ++k) {
+      MSCX_REQUIRE(rows[k] == mscx::GRAD_TY || rows[k] == mscx::GRAD_TY_LOW, MSCX_EINVAL,
+                   "block height must be " + std::to_string(mscx::GRAD_TY) + " or " +
+                       std::to_string(mscx::GRAD_TY_LOW) + " rows");
+      MSCX_REQUIRE(k == 0 || max_rows[k] > max_rows[k - 1], MSCX_EINVAL, "class bounds must increase");
+      t.emplace_back(max_rows[k], rows[k]);
+    }
+    std::lock_guard<std::mutex> lk(mscx::g_rows_mu);
+    mscx::g_rows.swap(t);
   });
 }
 

@@ -31,6 +31,10 @@ namespace mscx {
 // launch per stage, then synchronises.  Sets K / worked on every tile.
 void run_tiles(mscx_tile *const *tiles, int n);
 
+// Block tile height (GRAD_TY or GRAD_TY_LOW rows) a tile of vy vertex rows
+// takes, from the table mscx_set_block_rows installs.
+int block_rows_for(int vy);
+
 // Size of the device allocation a tile over (rect, ext) makes at creation.
 size_t tile_fixed_bytes(const int32_t rect[4], const int32_t ext[4]);
 // Throws MSCX_EINVAL unless (rect, ext) is a valid tile.

@@ -10,39 +10,52 @@
 namespace mscx {
 
 constexpr int GRAD_TX = 64;       // vertices per gradient/trace block tile in x (one per lane)
-constexpr int GRAD_TY = 32;       // vertex rows per block tile
+constexpr int GRAD_TY = 32;       // vertex rows per block tile (the default shape)
+constexpr int GRAD_TY_LOW = 16;   // the other block tile height the kernels carry
 constexpr int SEG = 2 * GRAD_TX;  // cells per collation segment: one block tile row
 constexpr int SCAN_CHUNK = 2048;  // counts per scan block
 
-// Perimeter ("ring") of a block tile, per forest (vertices, then quads at
-// RING_Q): four side arrays, the bottom row (lj = 0) at [0, TX), the top row
-// (lj = TY-1) at [TX, 2TX), the left column (li = 0) at [2TX, 2TX+TY) and
-// the right column (li = TX-1) at [2TX+TY, 2TX+2TY); a corner has a word in
-// two arrays, both with the same value.  A gradient path that leaves a block
-// through side s at position p enters its neighbour on the array of the
-// opposite side, at ring_entry_base(s) + p.
-constexpr int RING_Q = 2 * GRAD_TX + 2 * GRAD_TY;  // ring words per forest
-constexpr int RING_W = 2 * RING_Q;                 // ring words per block
+// Perimeter ("ring") of a block tile of ty rows, per forest (vertices, then
+// quads at ring_q(ty)): four side arrays, the bottom row (lj = 0) at
+// [0, TX), the top row (lj = ty-1) at [TX, 2TX), the left column (li = 0) at
+// [2TX, 2TX+ty) and the right column (li = TX-1) at [2TX+ty, 2TX+2ty); a
+// corner has a word in two arrays, both with the same value.  A gradient
+// path that leaves a block through side s at position p enters its
+// neighbour on the array of the opposite side, at ring_entry_base(s, ty) + p.
+MSCX_HD constexpr int ring_q(int ty) { return 2 * GRAD_TX + 2 * ty; }  // ring words per forest
+MSCX_HD constexpr int ring_w(int ty) { return 2 * ring_q(ty); }        // ring words per block
 
 // side arrays entered through side s = 0 (+x), 1 (-x), 2 (+y), 3 (-y): the
 // neighbour's left column, right column, bottom row, top row
-MSCX_HD int ring_entry_base(int s) {
-  return s == 0 ? 2 * GRAD_TX : s == 1 ? 2 * GRAD_TX + GRAD_TY : s == 2 ? 0 : GRAD_TX;
+MSCX_HD int ring_entry_base(int s, int ty) {
+  return s == 0 ? 2 * GRAD_TX : s == 1 ? 2 * GRAD_TX + ty : s == 2 ? 0 : GRAD_TX;
 }
-MSCX_HD void ring_at(int p, int &li, int &lj) {
+MSCX_HD void ring_at(int p, int ty, int &li, int &lj) {
   if (p < GRAD_TX) {
     li = p;
     lj = 0;
   } else if (p < 2 * GRAD_TX) {
     li = p - GRAD_TX;
-    lj = GRAD_TY - 1;
-  } else if (p < 2 * GRAD_TX + GRAD_TY) {
+    lj = ty - 1;
+  } else if (p < 2 * GRAD_TX + ty) {
     li = 0;
     lj = p - 2 * GRAD_TX;
   } else {
     li = GRAD_TX - 1;
-    lj = p - (2 * GRAD_TX + GRAD_TY);
+    lj = p - (2 * GRAD_TX + ty);
   }
+}
+
+// Block tiles of a tile whose vertex window is g, with nseg block columns,
+// at ty rows per block; and the ring words its allocation holds (enough for
+// either block height, so the height can be chosen after allocation).
+MSCX_HD int64_t tile_blocks(const TileGeom &g, int nseg, int ty) {
+  return (int64_t)nseg * ((g.vy + ty - 1) / ty);
+}
+MSCX_HD int64_t ring_words_max(const TileGeom &g, int nseg) {
+  const int64_t a = tile_blocks(g, nseg, GRAD_TY) * ring_w(GRAD_TY);
+  const int64_t b = tile_blocks(g, nseg, GRAD_TY_LOW) * ring_w(GRAD_TY_LOW);
+  return a > b ? a : b;
 }
 
 // Local terminal of a vertex or quad entry after the block-local trace
@@ -77,7 +90,9 @@ struct TileDesc {
   // local terminals: per vertex position, vertex terminal | quad terminal << 16
   uint32_t *loc;
   int loc_pitch;  // words per vertex row (nseg * GRAD_TX)
-  // ring words per block (RING_W each): after the gradient kernel a root
+  // block tile height (GRAD_TY or GRAD_TY_LOW vertex rows)
+  int ty;
+  // ring words per block (ring_w(ty) each): after the gradient kernel a root
   // (ext-local cell row << 15 | segment << 7 | rank in the segment), OWN_NONE
   // or RING_MARK | ring index of the entry the path continues at; ringidx
   // holds the resolved critical index

@@ -62,3 +62,13 @@ def test_autotune_prefers_the_coarser_tiling_within_the_margin():
     assert pick([(0.5709, (0, 0)), (0.5702, (1, 1)), (0.6376, (2, 2))], 0.05) == (0, 0)
     assert pick([(1.0, (0, 0)), (0.90, (1, 1)), (0.80, (2, 2))], 0.05) == (2, 2)
     assert pick([(1.0, (0, 0)), (0.90, (1, 1)), (0.88, (2, 2))], 0.05) == (1, 1)
+
+
+def test_block_rows_table_is_validated():
+    # no device work: the table only selects the kernels' block height
+    mx.set_block_rows([(127, 16), (511, 32)])
+    mx.set_block_rows(None)
+    for bad in ([(100, 24)], [(511, 16), (127, 32)], [(i, 16) for i in range(1, 18)]):
+        with pytest.raises(mx.MscxError):
+            mx.set_block_rows(bad)
+    mx.set_block_rows(None)
