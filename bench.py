@@ -546,7 +546,8 @@ def main():
                                f"{g1[1] - g0[1]} graph replays in the timed steps",
                      "algorithmic_bytes": g_bytes, "avg_launch_ms": round(g_ms, 4),
                      "limiter": "VALU issue: the gradient rules' integer compares and selects "
-                                "(~180 VALU per vertex row step of a lane), not HBM bandwidth"},
+                                "(~124 VALU per interior vertex row step of a lane, ~1830 per wave with "
+                                "the in-block pointer jumping), not HBM bandwidth"},
         "gradient_trace": {"kernels": ["k_grad_local", "k_ring_resolve", "k_owner"],
                            "algorithmic_bytes": g_bytes + t_bytes, "ms": round(gt_ms, 4),
                            "achieved_GBs": round(gt_ach, 1), "frac": round(gt_ach / HBM_PEAK_GBS, 4)},
