@@ -13,6 +13,7 @@
 
 #include "complex.hpp"
 #include "devsort.hpp"
+#include "dmerge.hpp"
 #include <chrono>
 
 #include "runtime.hpp"
@@ -257,10 +258,14 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     OneEvent() { MSCX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
     ~OneEvent() { (void)hipEventDestroy(e); }
   } ev_cnt;
-  // 3. disc tracing on the GPU (the saddles' 1-manifolds): every slot's
-  //    list is counted first, while the host builds the label tables; the
-  //    counts are scanned into each complex's offsets once the labels and
-  //    their grouping are under way, and the lists are written after them
+  // 3. disc tracing on the GPU (the saddles' 1-manifolds), on a stream of
+  //    its own so that its latency-bound passes overlap the label pass and
+  //    the grouping: every slot's list is counted first, while the host
+  //    builds the label tables; the counts are scanned into each complex's
+  //    offsets once the labels and their grouping are under way, and the
+  //    lists are written after them
+  StreamLease trace_lease;
+  hipStream_t ts = trace_lease.s;
   tp = now_wall();
   std::vector<size_t> nslot(n), slot_base(n + 1, 0), init_base(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -280,7 +285,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   const size_t t_cnt = align_up(t_init + 4 * NI, 256);
   const size_t t_off = align_up(t_cnt + 4 * S, 256);
   const size_t t_bytes = align_up(t_off + 4 * S, 256);
-  PoolBlock trblk{device_pool().alloc(t_bytes), s};
+  PoolBlock trblk{device_pool().alloc(t_bytes), ts};
   char *tdev = (char *)trblk.p;
   char *th = (char *)lease.s->trace.reserve_host(t_bytes);
   DiscTraceDesc *td = (DiscTraceDesc *)(th + t_desc);
@@ -315,14 +320,14 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
   const int tblocks = plan_disc_trace_blocks(td, n, (int *)(th + t_pref));
   *terr_h = 0;
-  MSCX_HIP(hipMemcpyAsync(tdev, th, t_cnt, hipMemcpyHostToDevice, s));
-  launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, false, s);
+  MSCX_HIP(hipMemcpyAsync(tdev, th, t_cnt, hipMemcpyHostToDevice, ts));
+  launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, false, ts);
   MSCX_HIP(hipGetLastError());
   // counts (and the error word) back
   uint32_t *cnt_h = (uint32_t *)(th + t_cnt);
-  MSCX_HIP(hipMemcpyAsync(cnt_h, tdev + t_cnt, 4 * S, hipMemcpyDeviceToHost, s));
-  MSCX_HIP(hipMemcpyAsync(terr_h, tdev + t_err, 4, hipMemcpyDeviceToHost, s));
-  MSCX_HIP(hipEventRecord(ev_cnt.e, s));
+  MSCX_HIP(hipMemcpyAsync(cnt_h, tdev + t_cnt, 4 * S, hipMemcpyDeviceToHost, ts));
+  MSCX_HIP(hipMemcpyAsync(terr_h, tdev + t_err, 4, hipMemcpyDeviceToHost, ts));
+  MSCX_HIP(hipEventRecord(ev_cnt.e, ts));
   double paths_s = now_wall() - tp;
   tp = now_wall();
   // 4. label tables per critical index (host) + label pass (device)
@@ -464,16 +469,16 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     g_trace_host += S;
   }
   const size_t NO = out_base[n];
-  PoolBlock outblk{device_pool().alloc(std::max<size_t>(4 * NO, 256)), s};
+  PoolBlock outblk{device_pool().alloc(std::max<size_t>(4 * NO, 256)), ts};
   uint32_t *out_h = nullptr;
   if (gpu_trace) {
     for (int i = 0; i < n; ++i) td[i].out = (uint32_t *)outblk.p + out_base[i];
-    MSCX_HIP(hipMemcpyAsync(tdev, th, t_err, hipMemcpyHostToDevice, s));
-    MSCX_HIP(hipMemcpyAsync(tdev + t_off, th + t_off, 4 * S, hipMemcpyHostToDevice, s));
-    launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, true, s);
+    MSCX_HIP(hipMemcpyAsync(tdev, th, t_err, hipMemcpyHostToDevice, ts));
+    MSCX_HIP(hipMemcpyAsync(tdev + t_off, th + t_off, 4 * S, hipMemcpyHostToDevice, ts));
+    launch_disc_trace((const DiscTraceDesc *)tdev, (const int *)(tdev + t_pref), n, tblocks, true, ts);
     MSCX_HIP(hipGetLastError());
     out_h = (uint32_t *)lease.s->lists.reserve_host(std::max<size_t>(4 * NO, 4));
-    if (NO) MSCX_HIP(hipMemcpyAsync(out_h, outblk.p, 4 * NO, hipMemcpyDeviceToHost, s));
+    if (NO) MSCX_HIP(hipMemcpyAsync(out_h, outblk.p, 4 * NO, hipMemcpyDeviceToHost, ts));
   }
   tm.add_host(ST_DISC_PATHS, paths_s + now_wall() - tp);
 
@@ -504,10 +509,13 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
       group_ns += (int64_t)(1e9 * (now_wall() - w1));
     });
   } catch (...) {
-    (void)hipStreamSynchronize(s);  // no copy may still be landing in the staging buffers
+    // no copy may still be landing in the staging buffers
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(ts);
     throw;
   }
   MSCX_HIP(hipStreamSynchronize(s));
+  MSCX_HIP(hipStreamSynchronize(ts));
   if (gpu_trace) {
     const double w0 = now_wall();
     par_for(n, [&](int i) {

@@ -998,14 +998,10 @@ __device__ __forceinline__ int dt_succ(const DiscTraceDesc &D, const GLB uint8_t
   return ns;
 }
 
+// One thread follows slot s's chains (count pass: its length, or the wide
+// mark; write pass: its list).
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restrict__ descs,
-                                                    const int *__restrict__ prefix, int n) {
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, n, b);
-  const DiscTraceDesc &D = descs[t];
-  const uint32_t s = (uint32_t)(b - prefix[t]) * 256u + threadIdx.x;
-  if (s >= D.nslot) return;
+__device__ void trace_slot(const DiscTraceDesc &D, uint32_t s) {
   const GLB uint8_t *codes = glb(D.codes);
   const uint32_t i0 = D.init_off[s], ni = D.init_off[s + 1] - i0;
   const uint32_t *init = D.init_cells + i0;
@@ -1067,24 +1063,15 @@ __global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restr
   if (!WRITE) D.count[s] = len;
 }
 
-// The wide slots (more chains than a thread follows): the block's slots are
-// those of k_disc_trace's block; each wave takes the wide slots among its 64
-// one after another, with a lane per chain.  The list grows exactly as in
+// The wide slots (more chains than a thread follows): each wave takes the
+// wide slots among its 64 one after another, with a lane per chain.  The list grows exactly as in
 // k_disc_trace: the initial entries, then round by round every live chain's
 // current cell in chain order; a round's positions are the ranks of the live
 // chains (ballot and mbcnt over each group of 64 chains).  The chains' cells
 // live in LDS (DISC_TRACE_WIDE_CAP per wave).
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_disc_trace_wide(const DiscTraceDesc *__restrict__ descs,
-                                                         const int *__restrict__ prefix, int n) {
+__device__ void trace_wide(const DiscTraceDesc &D, uint32_t s0, int lane, uint32_t *chains) {
   constexpr int CAP = DISC_TRACE_WIDE_CAP;
-  __shared__ uint32_t chains_all[4][CAP];
-  const int b = blockIdx.x;
-  const int t = find_tile(prefix, n, b);
-  const DiscTraceDesc &D = descs[t];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint32_t *chains = chains_all[wv];
-  const uint32_t s0 = (uint32_t)(b - prefix[t]) * 256u + (uint32_t)(wv * 64);
   const uint32_t my = s0 + (uint32_t)lane;
   const uint32_t mc = my < D.nslot ? D.count[my] : 0u;
   // count pass: the slots the thread pass marked; write pass: the slots the
@@ -1173,6 +1160,24 @@ __global__ __launch_bounds__(256) void k_disc_trace_wide(const DiscTraceDesc *__
   }
 }
 
+// One launch per pass: a wave's lanes trace their own slots, then the wave
+// traces the wide slots among them, so a pass lasts as long as its longest
+// chain rather than the longest thread chain plus the longest wide one.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_disc_trace(const DiscTraceDesc *__restrict__ descs,
+                                                    const int *__restrict__ prefix, int n) {
+  __shared__ uint32_t chains_all[4][DISC_TRACE_WIDE_CAP];
+  const int b = blockIdx.x;
+  const int t = find_tile(prefix, n, b);
+  const DiscTraceDesc &D = descs[t];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t s0 = (uint32_t)(b - prefix[t]) * 256u + (uint32_t)(wv * 64);
+  if (s0 >= D.nslot) return;  // uniform across the wave
+  if (s0 + (uint32_t)lane < D.nslot) trace_slot<WRITE>(D, s0 + (uint32_t)lane);
+  // each lane reads back only the mark its own count pass wrote
+  trace_wide<WRITE>(D, s0, lane, chains_all[wv]);
+}
+
 int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix) {
   int acc = 0;
   for (int i = 0; i < n; ++i) {
@@ -1186,13 +1191,10 @@ int plan_disc_trace_blocks(const DiscTraceDesc *d, int n, int *prefix) {
 void launch_disc_trace(const DiscTraceDesc *d_desc, const int *d_prefix, int n, int nblocks, bool write,
                        hipStream_t s) {
   if (!nblocks) return;
-  if (write) {
+  if (write)
     hipLaunchKernelGGL(k_disc_trace<true>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
-    hipLaunchKernelGGL(k_disc_trace_wide<true>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
-  } else {
+  else
     hipLaunchKernelGGL(k_disc_trace<false>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
-    hipLaunchKernelGGL(k_disc_trace_wide<false>, dim3(nblocks), dim3(256), 0, s, d_desc, d_prefix, n);
-  }
 }
 
 // -------------------------------------------------------------- launchers
