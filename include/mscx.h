@@ -84,11 +84,14 @@ int mscx_graph_stats(uint64_t *builds, uint64_t *replays);
 int mscx_set_block_rows(int32_t n, const int32_t *max_rows, const int32_t *rows);
 
 /* Counters of the manifold pass's disc tracing since the library loaded (no
- * reference counterpart): disc slots traced on the GPU, slots traced on the
- * host (a disc whose initial entries grow more chains than one GPU thread
- * follows), and calls whose GPU lists were dropped for the host's because
- * the chains were not a gradient's V-paths (never expected). */
-int mscx_disc_trace_stats(uint64_t out[3]);
+ * reference counterpart): [0] disc slots traced on the GPU, [1] slots
+ * traced on the host (a disc whose initial entries grow more chains than a
+ * wavefront's chain table holds), [2] calls whose GPU lists were dropped for
+ * the host's because the chains were not a gradient's V-paths (never
+ * expected), [3] GPU slots traced by a whole wavefront (more chains than
+ * one lane follows), [4] the longest list traced on the GPU (cells; a
+ * tracing pass lasts about as long as its chain of dependent reads). */
+int mscx_disc_trace_stats(uint64_t out[5]);
 /* Counters of the device merge tree since the library loaded (no reference
  * counterpart): merges run on the GPU (up and down), grids whose tree went
  * back to the host path (a case the kernels do not follow), and host

@@ -129,8 +129,9 @@ def set_block_rows(table=None):
 
 def disc_trace_stats():
     """(disc slots traced on the GPU, slots traced on the host, calls whose
-    GPU lists were dropped) since the library loaded."""
-    o = np.zeros(3, np.uint64)
+    GPU lists were dropped, GPU slots traced by a whole wavefront, longest
+    list traced on the GPU) since the library loaded."""
+    o = np.zeros(5, np.uint64)
     check(lib().mscx_disc_trace_stats(ptr(o)))
     return tuple(int(v) for v in o)
 

@@ -30,7 +30,7 @@ namespace mscx {
 // allocate nothing.
 // disc slots traced on the GPU and on the host, and calls whose GPU lists
 // were dropped because the chains were no gradient's V-paths
-static std::atomic<uint64_t> g_trace_gpu{0}, g_trace_host{0}, g_trace_dropped{0};
+static std::atomic<uint64_t> g_trace_gpu{0}, g_trace_host{0}, g_trace_dropped{0}, g_trace_wide{0}, g_trace_longest{0};
 
 static double now_wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -439,15 +439,21 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   if (gpu_trace) {
     // each complex's offsets by slot; the device offsets are the same per leaf
     uint32_t *off_h = (uint32_t *)(th + t_off);
+    std::atomic<uint64_t> wide(0), longest(0);
     par_for(n, [&](int i) {
       std::vector<uint32_t> &o = cs[i]->sd_off;
       o.assign(nslot[i] + 1, 0);
       const uint32_t *c = cnt_h + slot_base[i];
       const DiscPrep &P = dp[i];
+      uint64_t nw = 0, lmax = 0;
       for (size_t k = 0; k < nslot[i]; ++k) {
         uint32_t len = c[k];
         if (len == DISC_TRACE_WIDE) throw std::logic_error("disc slot left unmeasured by the wide tracing");
-        if (len != DISC_TRACE_HOST && len >= DISC_TRACE_WIDE_LEN) len -= DISC_TRACE_WIDE_LEN;
+        if (len != DISC_TRACE_HOST && len >= DISC_TRACE_WIDE_LEN) {
+          len -= DISC_TRACE_WIDE_LEN;
+          ++nw;
+        }
+        if (len != DISC_TRACE_HOST) lmax = std::max<uint64_t>(lmax, len);
         if (len == DISC_TRACE_HOST) {
           std::vector<uint32_t> lst;
           disc_trace_one(P.init_cells.data() + P.init_off[k], P.init_off[k + 1] - P.init_off[k], (int)(k & 1),
@@ -458,7 +464,13 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
         o[k + 1] = o[k] + len;
       }
       memcpy(off_h + slot_base[i], o.data(), 4 * nslot[i]);
+      wide += nw;
+      for (uint64_t m = longest.load(); lmax > m && !longest.compare_exchange_weak(m, lmax);) {
+      }
     });
+    g_trace_wide += wide.load();
+    for (uint64_t m = g_trace_longest.load(); longest.load() > m && !g_trace_longest.compare_exchange_weak(m, longest.load());) {
+    }
     for (int i = 0; i < n; ++i) out_base[i + 1] = out_base[i] + cs[i]->sd_off[nslot[i]];
     size_t nh = 0;
     for (int i = 0; i < n; ++i) nh += host_slots[i].size();
@@ -534,11 +546,13 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
 
 }  // namespace mscx
 
-extern "C" int mscx_disc_trace_stats(uint64_t out[3]) {
+extern "C" int mscx_disc_trace_stats(uint64_t out[5]) {
   return mscx::guarded([&] {
     MSCX_REQUIRE(out, MSCX_EINVAL, "null argument");
     out[0] = mscx::g_trace_gpu.load();
     out[1] = mscx::g_trace_host.load();
     out[2] = mscx::g_trace_dropped.load();
+    out[3] = mscx::g_trace_wide.load();
+    out[4] = mscx::g_trace_longest.load();
   });
 }

@@ -103,9 +103,30 @@ def test_discs_are_traced_on_the_gpu():
     before = mx.disc_trace_stats()
     run = mx.GridRun(g, lx=1, ly=1, threshold=0.01)
     after = mx.disc_trace_stats()
-    gpu, host, dropped = (a - b for a, b in zip(after, before))
-    print(f"[pins] disc slots: {gpu} on the GPU, {host} on the host, {dropped} calls dropped", flush=True)
+    gpu, host, dropped, wide = (a - b for a, b in zip(after[:4], before[:4]))
+    print(f"[pins] disc slots: {gpu} on the GPU ({wide} by a wavefront), {host} on the host, {dropped} calls "
+          f"dropped; longest list {after[4]}", flush=True)
     assert dropped == 0 and gpu > 0 and gpu > 10 * host
+    assert wide > 0, "no slot took the wavefront tracing"
     orc = oracle.Pipeline(g, lx=1, ly=1, threshold=0.01)
     for i in range(run.num_pieces()):
         assert np.array_equal(run.piece(i).dump(), orc.dump(4, i)), i
+
+
+def test_discs_traced_by_wavefronts_and_the_host():
+    """White noise at 3%: some saddle discs grow more chains than a lane
+    follows (a wavefront traces them) and a few more than a wavefront's
+    chain table holds (the host traces those); every list equals the
+    oracle's."""
+    import oracle
+    from tests.helpers import rng_field
+    g = rng_field("white", 300, 300, 3)
+    print("[pins] white 300^2 at 3%: wavefront and host disc slots", flush=True)
+    before = mx.disc_trace_stats()
+    run = mx.GridRun(g, levels=0, threshold=0.03)
+    after = mx.disc_trace_stats()
+    gpu, host, dropped, wide = (a - b for a, b in zip(after[:4], before[:4]))
+    print(f"[pins] disc slots: {gpu} on the GPU ({wide} by a wavefront), {host} on the host", flush=True)
+    assert dropped == 0 and wide > 0 and host > 0
+    orc = oracle.Pipeline(g, levels=0, threshold=0.03)
+    assert np.array_equal(run.piece(0).dump(), orc.dump(4, 0))
