@@ -915,12 +915,17 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
   auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
   CellMap &redir = out.redir;  // cell -> slot in out.redirect (later assignments overwrite)
   redir = CellMap();
-  redir.reserve(m.size() / 4 + 16);
+  if (with_maps) redir.reserve(m.size() / 4 + 16);
   out.redirect.clear();
+  out.redir_node.assign(m.size(), UINT32_MAX);
   out.minmap.clear();
   out.maxmap.clear();
   out.surv_saddles.clear();
-  auto set_redirect = [&](uint32_t cell, uint32_t val) {
+  // node `key` (cell m.cell[key]) redirected to node `to`
+  auto set_redirect = [&](uint32_t key, uint32_t to) {
+    out.redir_node[key] = to;
+    if (!with_maps) return;
+    const uint32_t cell = m.cell[key], val = m.cell[to];
     uint32_t s = redir.find(cell);
     if (s == UINT32_MAX) {
       redir.set(cell, (uint32_t)out.redirect.size());
@@ -960,10 +965,9 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
           if (pass == 1) continue;
           const ConnSet &conn = (cell_dim(pc) == 0) ? m.des[i] : m.asc[i];
           if (conn.size() != 1) throw std::logic_error("I should be connected to exactly one surv extrema");
-          uint32_t ec = m.cell[conn[0]];
           if (m.pair[pi] != i) throw std::logic_error("My pair does not know Im paired with him");
-          if (in_r(c)) set_redirect(c, ec);
-          if (in_r(pc)) set_redirect(pc, ec);
+          if (in_r(c)) set_redirect(i, conn[0]);
+          if (in_r(pc)) set_redirect(pi, conn[0]);
         } else if (dim == 2) {
           for (uint32_t s : m.des[i])
             if (in_r(pc)) push(s, 0, pc);

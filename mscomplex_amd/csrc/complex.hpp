@@ -274,17 +274,18 @@ struct CodesView {
   uint8_t at(int x, int y) const { return p[(int64_t)(y - oy) * pitch + (x - ox)]; }
 };
 struct DiscPrep {
-  // (cell, owner), sorted by cell when built with the maps; redir finds a
-  // cell's entry either way
+  // (cell, owner), sorted by cell (only when built with the maps); redir
+  // finds a cell's entry
   std::vector<std::pair<uint32_t, uint32_t>> redirect;
   CellMap redir;
+  // per node: the node its cell's owner is redirected to (the last
+  // assignment wins), UINT32_MAX when none; the redirected cells are always
+  // nodes of the complex (a boundary-cancellable saddle and its pair), and
+  // so are the targets (the surviving extremum each saddle is connected to)
+  std::vector<uint32_t> redir_node;
   // (cell, cp index) of the surviving minima / maxima, sorted by cell (only
   // when built with the maps)
   std::vector<std::pair<uint32_t, uint32_t>> minmap, maxmap;
-  uint32_t redirect_of(uint32_t cell) const {
-    const uint32_t s = redir.find(cell);
-    return s == UINT32_MAX ? UINT32_MAX : redirect[s].second;
-  }
   std::vector<uint32_t> surv_saddles;
   // the entries postMergeFillDiscs appends to each disc before tracing, in
   // its order (CSR by slot 2 i + j), and per slot whether it is traced (the

@@ -142,46 +142,51 @@ static std::vector<Chunk> chunks_of(const std::vector<size_t> &count, size_t chu
 // that starts at 0).
 static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, const DiscPrep &dp,
                               const mscx_tile *t, uint32_t *owned, uint32_t *self, uint32_t kb, uint32_t ke) {
-  auto in_r = [&](uint32_t c) { return c != UINT32_MAX && rect.contains(cx_of(c), cy_of(c)); };
-  auto owner_of = [&](uint32_t c) -> uint32_t {  // own[c] for a cell c inside the ext
-    uint32_t i = m.idx.find(c);
-    if (i != UINT32_MAX && i < K) return c;
+  // Every cell the reference's label() looks at here is a node's cell (a
+  // critical point of the leaf, a redirected saddle or its pair, or the
+  // extremum a redirect points to), so its lookups in the cell map and the
+  // redirect map are taken by node index.
+  auto in_r = [&](uint32_t c) { return rect.contains(cx_of(c), cy_of(c)); };
+  // own[] of node j's cell inside the ext, as a node index: a critical point
+  // of the leaf owns itself; any other node's cell is read from the owner
+  // image (UINT32_MAX: no owner)
+  auto owner_node = [&](uint32_t j) -> uint32_t {
+    if (j < K) return j;
+    const uint32_t c = m.cell[j];
     uint32_t oi = 0;
     const int64_t off = (int64_t)(cy_of(c) - t->ext[1]) * t->d.own_pitch + (cx_of(c) - t->ext[0]);
     MSCX_HIP(hipMemcpy(&oi, t->d.own + off, 4, hipMemcpyDeviceToHost));
-    return oi == OWN_NONE ? UINT32_MAX : m.cell[oi];
+    return oi == OWN_NONE ? UINT32_MAX : oi;
   };
-  auto own2 = [&](uint32_t c) {
-    uint32_t r = dp.redirect_of(c);
-    return r != UINT32_MAX ? r : owner_of(c);
+  // own2 of node j's cell (one redirect, else the owner)
+  auto own2 = [&](uint32_t j) {
+    const uint32_t r = dp.redir_node[j];
+    return r != UINT32_MAX ? r : owner_node(j);
   };
-  // minmap / maxmap lookups (the surviving extrema by cell) through the
-  // complex's cell map
-  auto label = [&](uint32_t o, uint32_t edim) -> uint32_t {
-    if (in_r(o)) o = own2(o);
-    if (o == UINT32_MAX || cx_of(o) == -1 || cy_of(o) == -1) return UINT32_MAX;
-    uint32_t k = surviving_extremum(m, o, edim);
-    return k == UINT32_MAX ? LAB_ERR : k;
+  // label() of the reference on node j's cell: the surviving extremum (of
+  // dimension edim) that owns it after one redirect, LAB_ERR if the owner is
+  // no surviving extremum, UINT32_MAX if the cell has no owner
+  auto label = [&](uint32_t j, uint32_t edim) -> uint32_t {
+    if (in_r(m.cell[j])) j = own2(j);
+    if (j == UINT32_MAX) return UINT32_MAX;
+    return (cell_dim(m.cell[j]) == edim && !(m.flags[j] & CPF_BCANC)) ? j : LAB_ERR;
   };
   // a redirect at a vertex or quad that the leaf does not hold as critical
   // would change the label of a non-critical cell, which the tables cannot
   // express
   if (kb == 0)
-    for (const auto &r : dp.redirect) {
-      if (cell_dim(r.first) == 1 || !in_r(r.first)) continue;
-      uint32_t i = m.idx.find(r.first);
-      if (i == UINT32_MAX || i >= K) throw std::logic_error("owner redirect at a cell that is not critical in the leaf");
-    }
+    for (uint32_t j = K; j < m.size(); ++j)
+      if (dp.redir_node[j] != UINT32_MAX && cell_dim(m.cell[j]) != 1 && in_r(m.cell[j]))
+        throw std::logic_error("owner redirect at a cell that is not critical in the leaf");
   for (uint32_t k = kb; k < ke; ++k) {
-    const uint32_t c = m.cell[k];
-    const uint32_t dim = cell_dim(c);
+    const uint32_t dim = cell_dim(m.cell[k]);
     if (dim == 1) {  // saddles own no vertex or quad
       owned[k] = self[k] = LAB_ERR;
       continue;
     }
-    owned[k] = label(c, dim);
-    const uint32_t r = dp.redirect_of(c);
-    self[k] = label(r != UINT32_MAX ? r : c, dim);
+    owned[k] = label(k, dim);
+    const uint32_t r = dp.redir_node[k];
+    self[k] = label(r != UINT32_MAX ? r : k, dim);
   }
 }
 
