@@ -269,15 +269,13 @@ void restore_complex(ComplexSpill &in, Complex &m) {
 // raster order; each saddle connects to the owners of its facets (des) and
 // cofacets (asc).  Arc multisets are order-free, so they are built in two
 // passes (counts, then fills in index order, already sorted).
-void complex_build(Complex &c, const Rect &rect, const Rect &ext, const std::vector<uint32_t> &crit,
-                   const std::vector<float> &fn, const std::vector<uint32_t> &conn4,
-                   const std::vector<uint32_t> &pair) {
-  const uint32_t K = (uint32_t)crit.size();
+void complex_build(Complex &c, const Rect &rect, const Rect &ext, uint32_t K, const uint32_t *crit,
+                   const float *fn, const uint32_t *conn4, const uint32_t *pair) {
   c.rect = rect;
   c.ext = ext;
   c.reserve_nodes(Complex::with_headroom(K));
-  c.cell.assign(crit.begin(), crit.end());
-  c.fn.assign(fn.begin(), fn.end());
+  c.cell.assign(crit, crit + K);
+  c.fn.assign(fn, fn + K);
   c.pair.assign(K, UINT32_MAX);
   c.flags.assign(K, 0);
   c.asc.assign(K, ConnSet());
@@ -323,7 +321,7 @@ void complex_from_tile(const mscx_tile *t, Complex &out) {
   std::vector<float> fn;
   tile_critical(t, crit, fn, conn, pr);
   Rect r{t->rect[0], t->rect[1], t->rect[2], t->rect[3]}, e{t->ext[0], t->ext[1], t->ext[2], t->ext[3]};
-  complex_build(out, r, e, crit, fn, conn, pr);
+  complex_build(out, r, e, (uint32_t)crit.size(), crit.data(), fn.data(), conn.data(), pr.data());
 }
 
 static constexpr uint64_t kMaxArcProduct = 1ull << 20;
@@ -1309,8 +1307,7 @@ int mscx_complex_from_critical(const int32_t rect[4], const int32_t ext[4], uint
     }
     std::unique_ptr<mscx_complex> h(new mscx_complex);
     Rect r{rect[0], rect[1], rect[2], rect[3]}, e{ext[0], ext[1], ext[2], ext[3]};
-    complex_build(h->c, r, e, std::vector<uint32_t>(crit, crit + K), std::vector<float>(fn, fn + K),
-                  std::vector<uint32_t>(conn4, conn4 + 4 * (size_t)K), std::vector<uint32_t>(pair_idx, pair_idx + K));
+    complex_build(h->c, r, e, K, crit, fn, conn4, pair_idx);
     *out = h.release();
   });
 }

@@ -236,10 +236,10 @@ struct Complex {
 
 // GridDataset::writeout_connectivity from a worked tile.
 void complex_from_tile(const mscx_tile *t, Complex &out);
-// Same, from host copies of the tile's critical data.
-void complex_build(Complex &c, const Rect &rect, const Rect &ext, const std::vector<uint32_t> &crit,
-                   const std::vector<float> &fn, const std::vector<uint32_t> &conn4,
-                   const std::vector<uint32_t> &pair);
+// Same, from host copies of the tile's critical data (K critical cells;
+// conn4 holds 4 words per cell).
+void complex_build(Complex &c, const Rect &rect, const Rect &ext, uint32_t K, const uint32_t *crit,
+                   const float *fn, const uint32_t *conn4, const uint32_t *pair);
 
 void cancel_pairs(Complex &m, uint32_t cp0, uint32_t cp1, std::vector<uint32_t> *new_edges);
 void uncancel_pairs(Complex &m, uint32_t cp1, uint32_t cp2);

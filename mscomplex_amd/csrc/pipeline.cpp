@@ -189,57 +189,91 @@ size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int
   return b;
 }
 
-// Host staging of the critical-point block of one compute call.
-static std::mutex g_crit_mu;
-static Scratch g_crit_host;
-
 // Host leaf complexes (writeout_connectivity) of the grids from one host
 // copy of the tiles' critical block.
+// Host staging for the tiles' critical block: one per concurrent caller
+// (a batch's leaves are built from it while its trees run).
+struct CritStage {
+  Scratch s;
+};
+static std::mutex g_crit_mu;
+static std::vector<std::unique_ptr<CritStage>> g_crit_free;
+struct CritLease {
+  std::unique_ptr<CritStage> st;
+  CritLease() {
+    std::lock_guard<std::mutex> lk(g_crit_mu);
+    if (g_crit_free.empty()) {
+      st.reset(new CritStage);
+    } else {
+      st = std::move(g_crit_free.back());
+      g_crit_free.pop_back();
+    }
+  }
+  ~CritLease() {
+    std::lock_guard<std::mutex> lk(g_crit_mu);
+    g_crit_free.push_back(std::move(st));
+  }
+};
+
+// Every tile's critical data lives in one block: one copy to host.
+static const char *copy_crit_block(const std::vector<mscx_grid *> &grids, CritLease &lease) {
+  const mscx_tile *t0 = grids[0]->tiles[0];
+  const size_t bytes = t0->crit_block_bytes;
+  char *host = (char *)lease.st->s.reserve_host(bytes);
+  MSCX_HIP(hipMemcpyAsync(host, t0->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
+  MSCX_HIP(hipStreamSynchronize(runtime().stream));
+  return host;
+}
+
+static void alloc_host_leaves(mscx_grid *G) {
+  G->pieces.assign(G->leaves.size(), nullptr);
+  for (size_t i = 0; i < G->leaves.size(); ++i) G->pieces[i] = new mscx_complex;
+}
+
+// Leaf i's complex (writeout_connectivity) from the host copy of the block.
+static void build_host_leaf(mscx_grid *G, int i, const char *host) {
+  const mscx_tile *t = G->tiles[i];
+  const char *b = host + t->crit_off;
+  const CritLayout L(t->K);
+  complex_build(G->pieces[i]->c, G->leaves[i].rect, G->leaves[i].ext, t->K, (const uint32_t *)b,
+                (const float *)(b + L.fn), (const uint32_t *)(b + L.conn), (const uint32_t *)(b + L.pair));
+}
+
 static void build_host_leaves(std::vector<mscx_grid *> &grids) {
   const int ng = (int)grids.size();
   if (!ng) return;
-  // every tile's critical data lives in one block: one copy to host
-  std::lock_guard<std::mutex> lk(g_crit_mu);
-  const mscx_tile *t0 = grids[0]->tiles[0];
-  const size_t bytes = t0->crit_block_bytes;
-  char *host = (char *)g_crit_host.reserve_host(bytes);
-  MSCX_HIP(hipMemcpyAsync(host, t0->crit_block.get(), bytes, hipMemcpyDeviceToHost, runtime().stream));
-  MSCX_HIP(hipStreamSynchronize(runtime().stream));
-
-  std::vector<std::pair<int, int>> leafs;  // (grid, leaf)
+  CritLease lease;
+  const char *host = copy_crit_block(grids, lease);
+  std::vector<std::pair<int, int>> leafs;  // (grid, leaf), largest first
   for (int gi = 0; gi < ng; ++gi) {
-    mscx_grid *G = grids[gi];
-    G->pieces.assign(G->leaves.size(), nullptr);
-    for (size_t i = 0; i < G->leaves.size(); ++i) {
-      G->pieces[i] = new mscx_complex;
-      leafs.emplace_back(gi, (int)i);
-    }
+    alloc_host_leaves(grids[gi]);
+    for (size_t i = 0; i < grids[gi]->leaves.size(); ++i) leafs.emplace_back(gi, (int)i);
   }
-  parallel_for((int)leafs.size(), [&](int k) {
-    mscx_grid *G = grids[leafs[k].first];
-    const int i = leafs[k].second;
-    const mscx_tile *t = G->tiles[i];
-    const char *b = host + t->crit_off;
-    const CritLayout L(t->K);
-    const uint32_t K = t->K;
-    std::vector<uint32_t> crit((const uint32_t *)b, (const uint32_t *)b + K);
-    std::vector<float> fn((const float *)(b + L.fn), (const float *)(b + L.fn) + K);
-    std::vector<uint32_t> conn((const uint32_t *)(b + L.conn), (const uint32_t *)(b + L.conn) + 4 * (size_t)K);
-    std::vector<uint32_t> pr((const uint32_t *)(b + L.pair), (const uint32_t *)(b + L.pair) + K);
-    complex_build(G->pieces[i]->c, G->leaves[i].rect, G->leaves[i].ext, crit, fn, conn, pr);
+  std::stable_sort(leafs.begin(), leafs.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+    return grids[a.first]->tiles[a.second]->K > grids[b.first]->tiles[b.second]->K;
   });
+  parallel_for((int)leafs.size(), [&](int k) { build_host_leaf(grids[leafs[k].first], leafs[k].second, host); });
 }
 
-void grids_compute(std::vector<mscx_grid *> &grids) {
+// GPU stages of every leaf of every grid and the leaf complexes.  With
+// `defer`, the host leaves are allocated but left for the caller to build
+// from the returned host copy of the critical block (held by the lease).
+static const char *grids_compute_impl(std::vector<mscx_grid *> &grids, CritLease *defer) {
   const int ng = (int)grids.size();
-  if (!ng) return;
+  if (!ng) return nullptr;
   const double t0 = now_s();
   std::vector<mscx_tile *> tiles;
   for (mscx_grid *G : grids) tiles.insert(tiles.end(), G->tiles.begin(), G->tiles.end());
   run_tiles(tiles.data(), (int)tiles.size());
   std::vector<mscx_grid *> host, dev;
   for (mscx_grid *G : grids) (G->dmerge ? dev : host).push_back(G);
-  build_host_leaves(host);
+  const char *staged = nullptr;
+  if (!defer) {
+    build_host_leaves(host);
+  } else if (!host.empty()) {
+    staged = copy_crit_block(grids, *defer);
+    for (mscx_grid *G : host) alloc_host_leaves(G);
+  }
   // device merge tree: the leaves are built in HBM from the critical block
   std::vector<std::pair<int, int>> leafs;
   for (int gi = 0; gi < (int)dev.size(); ++gi) {
@@ -261,7 +295,10 @@ void grids_compute(std::vector<mscx_grid *> &grids) {
   });
   const double t1 = now_s();
   for (mscx_grid *G : grids) G->t[0] += t1 - t0;
+  return staged;
 }
+
+void grids_compute(std::vector<mscx_grid *> &grids) { grids_compute_impl(grids, nullptr); }
 
 // The device tree could not be followed: the tree runs on the host from
 // host leaves built from the tiles' critical block.
@@ -619,10 +656,20 @@ namespace mscx {
 static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status, std::vector<std::string> *error) {
   const int ng = (int)grids.size();
   if (!ng) return;
-  grids_compute(grids);
+  // A batch builds each grid's host leaves in the task that then runs its
+  // tree, so that a large leaf's build overlaps other grids' trees instead
+  // of holding every tree back.
+  CritLease lease;
+  const char *staged = grids_compute_impl(grids, ng > 1 ? &lease : nullptr);
   std::vector<char> ok(ng, 1);
   auto tree = [&](int gi, bool inner) {
     try {
+      mscx_grid *G = grids[gi];
+      if (staged && !G->dmerge) {
+        const double t0 = now_s();
+        for (int i = 0; i < (int)G->leaves.size(); ++i) build_host_leaf(G, i, staged);
+        G->t[0] += now_s() - t0;
+      }
       grid_merge_up(grids[gi], inner);
       grid_simplify_down(grids[gi], inner);
     } catch (const std::exception &e) {
