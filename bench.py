@@ -204,6 +204,12 @@ def worst_e2e_bench(side=16384, threshold=0.0):
     import subprocess
     out = {}
     for levels in (0, 256):
+        if levels:
+            # the previous child's ~45 GB of host memory (and its device
+            # allocations) are still being returned for a few seconds after it
+            # exits; a child started at once measured its merge up 2.6-4.3 s
+            # instead of 0.36 s
+            time.sleep(10)
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--worst-e2e-child", str(side), str(levels),
                             str(threshold)], capture_output=True, text=True, timeout=1500)
         line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
