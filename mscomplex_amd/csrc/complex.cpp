@@ -84,15 +84,17 @@ uint32_t CellMap::get_or_zero(uint32_t key) {
 // ------------------------------------------------------------------ Complex
 
 DiscSpan Complex::disc(uint32_t i, int j) const {
-  const std::vector<uint32_t> &v = j == 0 ? asc_disc[i] : des_disc[i];
-  if (!v.empty()) return DiscSpan{v.data(), v.size()};
+  if (i < asc_disc.size()) {
+    const std::vector<uint32_t> &v = j == 0 ? asc_disc[i] : des_disc[i];
+    if (!v.empty()) return DiscSpan{v.data(), v.size()};
+  }
   const size_t sl = 2 * (size_t)i + j;
   if (sl + 1 < sd_off.size() && sd_off[sl + 1] > sd_off[sl])
     return DiscSpan{sd_cells.data() + sd_off[sl], sd_off[sl + 1] - sd_off[sl]};
   uint32_t d = cell_dim(cell[i]);
   if (!xd_off.empty() && i + 1 < xd_off.size() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
     return DiscSpan{xd_cells.data() + xd_off[i], xd_off[i + 1] - xd_off[i]};
-  return DiscSpan{v.data(), 0};
+  return DiscSpan{nullptr, 0};
 }
 
 void Complex::reserve_nodes(size_t n) {
@@ -102,8 +104,6 @@ void Complex::reserve_nodes(size_t n) {
   flags.reserve(n);
   asc.reserve(n);
   des.reserve(n);
-  asc_disc.reserve(n);
-  des_disc.reserve(n);
 }
 
 uint32_t Complex::replicate(const Complex &src, uint32_t j) {
@@ -115,8 +115,10 @@ uint32_t Complex::replicate(const Complex &src, uint32_t j) {
   flags.push_back(src.flags[j]);
   asc.emplace_back();
   des.emplace_back();
-  asc_disc.emplace_back();
-  des_disc.emplace_back();
+  if (!asc_disc.empty()) {
+    asc_disc.emplace_back();
+    des_disc.emplace_back();
+  }
   return n;
 }
 
@@ -152,6 +154,14 @@ void Complex::dump(std::vector<uint32_t> &out) const {
 }
 
 // ---------------------------------------------------------------- spill
+
+// Node i's disc list j as held per node (loaded from an archive), or an
+// empty list when the complex holds none per node.
+static const std::vector<uint32_t> &node_disc(const Complex &m, uint32_t i, int j) {
+  static const std::vector<uint32_t> none;
+  if (i >= m.asc_disc.size()) return none;
+  return j == 0 ? m.asc_disc[i] : m.des_disc[i];
+}
 
 // Nodes per record chunk of a spilled complex, and the size from which its
 // chunks are written and read on several host threads.
@@ -196,20 +206,21 @@ void spill_complex(Complex &m, ComplexSpill &out) {
     std::vector<uint32_t> &w = out.words[c];
     size_t len = 0;
     for (uint32_t i = out.begin[c]; i < out.begin[c + 1]; ++i)
-      len += 8 + m.asc[i].size() + m.des[i].size() + m.asc_disc[i].size() + m.des_disc[i].size();
+      len += 8 + m.asc[i].size() + m.des[i].size() + node_disc(m, i, 0).size() + node_disc(m, i, 1).size();
     w.reserve(len);
     for (uint32_t i = out.begin[c]; i < out.begin[c + 1]; ++i) {
       uint32_t fb;
       memcpy(&fb, &m.fn[i], 4);
       const uint32_t rec[8] = {m.cell[i], fb, m.pair[i], m.flags[i], m.asc[i].size(), m.des[i].size(),
-                               (uint32_t)m.asc_disc[i].size(), (uint32_t)m.des_disc[i].size()};
+                               (uint32_t)node_disc(m, i, 0).size(), (uint32_t)node_disc(m, i, 1).size()};
       w.insert(w.end(), rec, rec + 8);
       w.insert(w.end(), m.asc[i].begin(), m.asc[i].end());
       w.insert(w.end(), m.des[i].begin(), m.des[i].end());
-      w.insert(w.end(), m.asc_disc[i].begin(), m.asc_disc[i].end());
-      w.insert(w.end(), m.des_disc[i].begin(), m.des_disc[i].end());
+      w.insert(w.end(), node_disc(m, i, 0).begin(), node_disc(m, i, 0).end());
+      w.insert(w.end(), node_disc(m, i, 1).begin(), node_disc(m, i, 1).end());
     }
   });
+  out.node_discs = !m.asc_disc.empty();
   out.idx = std::move(m.idx);
   out.xd_off = std::move(m.xd_off);
   out.xd_cells = std::move(m.xd_cells);
@@ -230,8 +241,10 @@ void restore_complex(ComplexSpill &in, Complex &m) {
   c.flags.resize(n);
   c.asc.resize(n);
   c.des.resize(n);
-  c.asc_disc.resize(n);
-  c.des_disc.resize(n);
+  if (in.node_discs) {
+    c.asc_disc.resize(n);
+    c.des_disc.resize(n);
+  }
   const int nc = (int)in.words.size();
   spill_parallel(nc, [&](int k) {
     const uint32_t *p = in.words[k].data();
@@ -280,8 +293,6 @@ void complex_build(Complex &c, const Rect &rect, const Rect &ext, uint32_t K, co
   c.flags.assign(K, 0);
   c.asc.assign(K, ConnSet());
   c.des.assign(K, ConnSet());
-  c.asc_disc.assign(K, {});
-  c.des_disc.assign(K, {});
   c.idx.reserve(K);
   for (uint32_t k = 0; k < K; ++k) c.idx.insert(crit[k], k);
   for (uint32_t k = 0; k < K; ++k) {
@@ -519,8 +530,6 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   out->flags.resize(n);
   out->asc.resize(n);
   out->des.resize(n);
-  out->asc_disc.resize(n);
-  out->des_disc.resize(n);
   for (int i = 0; i < 2; ++i) {
     const Complex &m = *arr[i];
     const uint32_t *ai = at[i].data();
@@ -1224,8 +1233,6 @@ void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::stri
   c.flags.resize(ncp);
   c.asc.resize(ncp);
   c.des.resize(ncp);
-  c.asc_disc.resize(ncp);
-  c.des_disc.resize(ncp);
   c.idx.reserve(ncp);
   for (uint32_t i = 0; i < ncp; ++i) {
     const uint32_t *r = take(8);
@@ -1245,8 +1252,12 @@ void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::stri
       }
     }
     const uint32_t *ad = take(nad);
-    if (nad) c.asc_disc[i].assign(ad, ad + nad);
     const uint32_t *dd = take(ndd);
+    if ((nad || ndd) && c.asc_disc.empty()) {
+      c.asc_disc.resize(ncp);
+      c.des_disc.resize(ncp);
+    }
+    if (nad) c.asc_disc[i].assign(ad, ad + nad);
     if (ndd) c.des_disc[i].assign(dd, dd + ndd);
     if (c.idx.find_or_insert(c.cell[i], i) != UINT32_MAX) throw bad();
   }

@@ -204,6 +204,8 @@ struct Complex {
   std::vector<uint32_t> pair;
   std::vector<uint8_t> flags;
   std::vector<ConnSet> asc, des;
+  // per-node disc lists as loaded from an archive: both empty when the
+  // complex holds none per node, else one list per node
   std::vector<std::vector<uint32_t>> asc_disc, des_disc;
   // discs of surviving extrema (vertices of minima, quads of maxima) as
   // raster-ordered lists in CSR form by critical-point index
@@ -332,6 +334,7 @@ void complex_from_words(const uint32_t *w, size_t n, Complex &m, const std::stri
 struct ComplexSpill {
   Rect rect{}, ext{};
   uint32_t n = 0;
+  bool node_discs = false;  // the complex held per-node disc lists
   std::vector<uint32_t> begin;  // first node of each chunk, then n
   std::vector<std::vector<uint32_t>> words;
   CellMap idx;

@@ -990,8 +990,6 @@ void line_pass(DComplex &d, const Rect &ixn, bool up, hipStream_t s) {
   c.flags.resize(nmeta);
   c.asc.resize(nmeta);
   c.des.resize(nmeta);
-  c.asc_disc.resize(nmeta);
-  c.des_disc.resize(nmeta);
   c.idx.reserve(nmeta);
   for (uint32_t i = 0; i < nmeta; ++i) {
     c.cell[i] = mrec[3 * (size_t)i];
@@ -1364,8 +1362,6 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
   c.flags.resize(n);
   c.asc.resize(n);
   c.des.resize(n);
-  c.asc_disc.resize(n);
-  c.des_disc.resize(n);
   MSCX_HIP(hipStreamSynchronize(s));
   auto parse = [&](uint32_t lo, uint32_t hi) {
     for (uint32_t i = lo; i < hi; ++i) {
