@@ -294,7 +294,12 @@ void complex_build(Complex &c, const Rect &rect, const Rect &ext, uint32_t K, co
   c.asc.assign(K, ConnSet());
   c.des.assign(K, ConnSet());
   c.idx.reserve(K);
-  for (uint32_t k = 0; k < K; ++k) c.idx.insert(crit[k], k);
+  // the home slots are random: start loading them a few keys ahead
+  constexpr uint32_t AHEAD = 16;
+  for (uint32_t k = 0; k < K; ++k) {
+    if (k + AHEAD < K) c.idx.prefetch(crit[k + AHEAD]);
+    c.idx.insert(crit[k], k);
+  }
   for (uint32_t k = 0; k < K; ++k) {
     if (pair[k] != UINT32_MAX) {
       c.flags[k] |= CPF_BCANC;
@@ -313,8 +318,14 @@ void complex_build(Complex &c, const Rect &rect, const Rect &ext, uint32_t K, co
     if (a0 != UINT32_MAX) c.asc[k].push_sorted(a0);
     if (a1 != UINT32_MAX) c.asc[k].push_sorted(a1);
   }
-  // extremum side: saddles visited in increasing index, so appends stay sorted
+  // extremum side: saddles visited in increasing index, so appends stay
+  // sorted (the extrema's sets are random accesses: start loading those of
+  // a saddle a few nodes ahead)
   for (uint32_t k = 0; k < K; ++k) {
+    if (k + AHEAD < K && cell_dim(crit[k + AHEAD]) == 1) {
+      for (uint32_t v : c.des[k + AHEAD]) __builtin_prefetch(&c.asc[v]);
+      for (uint32_t v : c.asc[k + AHEAD]) __builtin_prefetch(&c.des[v]);
+    }
     if (cell_dim(crit[k]) != 1) continue;
     for (uint32_t v : c.des[k]) {
       if (cell_dim(crit[v]) != 0) throw std::logic_error("must connect i,i+1 cp (or vice versa)");
