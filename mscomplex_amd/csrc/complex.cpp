@@ -167,32 +167,6 @@ static const std::vector<uint32_t> &node_disc(const Complex &m, uint32_t i, int 
 // chunks are written and read on several host threads.
 static constexpr uint32_t kSpillChunk = 1u << 20;
 
-template <class F>
-static void spill_parallel(int n, F f) {
-  const int nt = std::min(n, host_threads());
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next(0);
-  std::vector<std::exception_ptr> errs(nt);
-  auto body = [&](int t) {
-    for (int i = next++; i < n; i = next++) {
-      try {
-        f(i);
-      } catch (...) {
-        if (!errs[t]) errs[t] = std::current_exception();
-      }
-    }
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) th.emplace_back(body, t);
-  body(0);
-  for (auto &x : th) x.join();
-  for (auto &e : errs)
-    if (e) std::rethrow_exception(e);
-}
-
 void spill_complex(Complex &m, ComplexSpill &out) {
   const uint32_t n = m.size();
   out.rect = m.rect;
@@ -202,7 +176,7 @@ void spill_complex(Complex &m, ComplexSpill &out) {
   out.begin.resize(nc + 1);
   for (uint32_t c = 0; c <= nc; ++c) out.begin[c] = std::min(n, c * kSpillChunk);
   out.words.assign(nc, {});
-  spill_parallel((int)nc, [&](int c) {
+  parallel_for((int)nc, [&](int c) {
     std::vector<uint32_t> &w = out.words[c];
     size_t len = 0;
     for (uint32_t i = out.begin[c]; i < out.begin[c + 1]; ++i)
@@ -246,7 +220,7 @@ void restore_complex(ComplexSpill &in, Complex &m) {
     c.des_disc.resize(n);
   }
   const int nc = (int)in.words.size();
-  spill_parallel(nc, [&](int k) {
+  parallel_for(nc, [&](int k) {
     const uint32_t *p = in.words[k].data();
     for (uint32_t i = in.begin[k]; i < in.begin[k + 1]; ++i) {
       c.cell[i] = p[0];
@@ -509,31 +483,50 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   // j, UINT32_MAX for a cancelled node that is not replicated): msc1's
   // non-cancelled nodes in order, then msc2's nodes that are not msc1's
   // non-cancelled cells, in order -- the indices the reference's loop
-  // gives.  The union's arrays are then filled by index, its cell map is
-  // built on one thread while two others copy the arcs into disjoint ranges
-  // of destination nodes, and the pair indices are looked up last, in the
-  // reference's order (a multiset's content does not depend on the order of
-  // its inserts, and each destination set is written by one thread).
+  // gives.  The union's arrays are then filled by index; its cell map is
+  // built on one thread while the others build the arc sets of disjoint
+  // ranges of destination nodes (each set from its one or two source
+  // nodes: a multiset's content does not depend on the order of its
+  // inserts); the pair indices are looked up last, in the reference's order.
+  const int nt = host_threads();
+  const bool par = nt > 2 && (size_t)m1.size() + m2.size() >= 16384;
+  constexpr int CHUNK = 4096;
+  auto chunks = [](uint32_t n) { return (int)((n + CHUNK - 1) / CHUNK); };
   std::vector<uint32_t> at[2];
   uint32_t n = 0;
   at[0].assign(m1.size(), UINT32_MAX);
   for (uint32_t j = 0; j < m1.size(); ++j)
     if (!(m1.flags[j] & CPF_CANCELLED)) at[0][j] = n++;
   const uint32_t n1 = n;
-  at[1].assign(m2.size(), UINT32_MAX);
+  // m2's cell among the union's cells so far, i.e. among m1's non-cancelled
+  // ones (m1's map may also hold zero entries for cells it never held, hence
+  // the check of the cell itself); the lookups are independent
+  at[1].resize(m2.size());
+  auto lookup = [&](int c) {
+    const uint32_t b = (uint32_t)c * CHUNK, e = std::min<uint32_t>(m2.size(), b + CHUNK);
+    for (uint32_t j = b; j < e; ++j) {
+      if (j + 16 < e) m1.idx.prefetch(m2.cell[j + 16]);
+      const uint32_t k = m1.idx.find(m2.cell[j]);
+      at[1][j] = (k != UINT32_MAX && k < m1.size() && m1.cell[k] == m2.cell[j]) ? at[0][k] : UINT32_MAX;
+    }
+  };
+  parallel_for(chunks(m2.size()), lookup, par ? nt : 1);
+  std::vector<uint32_t> src[2];  // the source node of each destination, per complex
+  src[0].resize(n1);
+  for (uint32_t j = 0; j < m1.size(); ++j)
+    if (at[0][j] != UINT32_MAX) src[0][at[0][j]] = j;
+  std::vector<uint32_t> shared(n1, UINT32_MAX);  // m2's node on m1's cell d
   for (uint32_t j = 0; j < m2.size(); ++j) {
-    if (j + 16 < m2.size()) m1.idx.prefetch(m2.cell[j + 16]);
-    // m2's cell among the union's cells so far, i.e. among m1's
-    // non-cancelled ones (m1's map may also hold zero entries for cells it
-    // never held, hence the check of the cell itself)
-    const uint32_t k = m1.idx.find(m2.cell[j]);
-    if (k != UINT32_MAX && k < m1.size() && m1.cell[k] == m2.cell[j] && at[0][k] != UINT32_MAX) {
-      at[1][j] = at[0][k];
+    if (at[1][j] != UINT32_MAX) {
+      shared[at[1][j]] = j;
       continue;
     }
     if (m2.flags[j] & CPF_CANCELLED) continue;
     at[1][j] = n++;
   }
+  src[1].resize(n - n1);
+  for (uint32_t j = 0; j < m2.size(); ++j)
+    if (at[1][j] >= n1 && at[1][j] != UINT32_MAX) src[1][at[1][j] - n1] = j;
   out->reserve_nodes(Complex::with_headroom(n));
   out->cell.resize(n);
   out->fn.resize(n);
@@ -541,18 +534,18 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   out->flags.resize(n);
   out->asc.resize(n);
   out->des.resize(n);
-  for (int i = 0; i < 2; ++i) {
-    const Complex &m = *arr[i];
-    const uint32_t *ai = at[i].data();
-    for (uint32_t j = 0; j < m.size(); ++j) {
-      const uint32_t d = ai[j];
-      // m2's nodes on m1's cells keep m1's data
-      if (d == UINT32_MAX || (i == 1 && d < n1)) continue;
+  // m2's nodes on m1's cells keep m1's data
+  auto fill = [&](int c) {
+    const uint32_t b = (uint32_t)c * CHUNK, e = std::min<uint32_t>(n, b + CHUNK);
+    for (uint32_t d = b; d < e; ++d) {
+      const Complex &m = d < n1 ? m1 : m2;
+      const uint32_t j = d < n1 ? src[0][d] : src[1][d - n1];
       out->cell[d] = m.cell[j];
       out->fn[d] = m.fn[j];
       out->flags[d] = m.flags[j];
     }
-  }
+  };
+  parallel_for(chunks(n), fill, par ? nt : 1);
   auto build_map = [&] {
     out->idx.reserve((size_t)m1.size() + m2.size());
     for (uint32_t d = 0; d < n; ++d) {
@@ -560,66 +553,57 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
       if (out->idx.find_or_insert(out->cell[d], d) != UINT32_MAX) throw std::logic_error("this cp is present in msc");
     }
   };
-  // arcs of m_i's node j into out; `lo_half` selects the destinations below
-  // n1 (m1's nodes) or from n1 on
-  auto copy_arcs = [&](int i, bool lo_half) {
+  // the non-cancelled arcs of m_i's node j into the sets of destination d;
+  // m2's arcs between two cells of the shared line are dropped
+  auto copy_node = [&](int i, uint32_t j, uint32_t d) {
     const Complex &m = *arr[i];
+    if (m.flags[j] & CPF_CANCELLED) return;
     const uint32_t *ai = at[i].data();
-    for (uint32_t j = 0; j < m.size(); ++j) {
-      if (m.flags[j] & CPF_CANCELLED) continue;
-      const uint32_t di = ai[j];
-      if ((di < n1) != lo_half) continue;
-      const bool cmn = (i == 1) && ixn.contains(cx_of(m.cell[j]), cy_of(m.cell[j]));
-      for (int k = 0; k < 2; ++k) {
-        const ConnSet &src = k == 0 ? m.asc[j] : m.des[j];
-        ConnSet &dst = k == 0 ? out->asc[di] : out->des[di];
-        for (uint32_t v : src) {
-          if (cmn) {
-            const uint32_t vc = m.cell[v];
-            if (ixn.contains(cx_of(vc), cy_of(vc))) continue;
-          }
-          if (m.flags[v] & CPF_CANCELLED) continue;
-          dst.insert(ai[v]);
+    const bool cmn = (i == 1) && ixn.contains(cx_of(m.cell[j]), cy_of(m.cell[j]));
+    for (int k = 0; k < 2; ++k) {
+      const ConnSet &from = k == 0 ? m.asc[j] : m.des[j];
+      ConnSet &dst = k == 0 ? out->asc[d] : out->des[d];
+      for (uint32_t v : from) {
+        if (cmn) {
+          const uint32_t vc = m.cell[v];
+          if (ixn.contains(cx_of(vc), cy_of(vc))) continue;
         }
+        if (m.flags[v] & CPF_CANCELLED) continue;
+        dst.insert(ai[v]);
       }
     }
   };
-  auto lo = [&] {
-    copy_arcs(0, true);
-    copy_arcs(1, true);
+  auto arcs = [&](int c) {
+    const uint32_t b = (uint32_t)c * CHUNK, e = std::min<uint32_t>(n, b + CHUNK);
+    for (uint32_t d = b; d < e; ++d) {
+      if (d < n1) {
+        copy_node(0, src[0][d], d);
+        if (shared[d] != UINT32_MAX) copy_node(1, shared[d], d);
+      } else {
+        copy_node(1, src[1][d - n1], d);
+      }
+    }
   };
-  auto hi = [&] { copy_arcs(1, false); };
-  if (n >= 16384 && host_threads() > 2) {
-    std::exception_ptr e1, e2;
-    std::thread t1([&] {
+  if (par) {
+    std::exception_ptr em;
+    std::thread tm([&] {
       try {
         build_map();
       } catch (...) {
-        e1 = std::current_exception();
+        em = std::current_exception();
       }
     });
-    std::thread t2([&] {
-      try {
-        hi();
-      } catch (...) {
-        e2 = std::current_exception();
-      }
-    });
-    std::exception_ptr e0;
     try {
-      lo();
+      parallel_for(chunks(n), arcs, nt - 1);
     } catch (...) {
-      e0 = std::current_exception();
+      tm.join();
+      throw;
     }
-    t1.join();
-    t2.join();
-    if (e1) std::rethrow_exception(e1);
-    if (e0) std::rethrow_exception(e0);
-    if (e2) std::rethrow_exception(e2);
+    tm.join();
+    if (em) std::rethrow_exception(em);
   } else {
     build_map();
-    lo();
-    hi();
+    for (int c = 0; c < chunks(n); ++c) arcs(c);
   }
   // map::operator[] on each boundary-cancellable node's pair cell (which
   // inserts a zero entry when the pair is absent, as the reference's lookup
@@ -689,11 +673,18 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
     // at[j]: m's index of P's cell j (UINT32_MAX when absent), kept equal to
     // m.idx.find(P.cell[j]) as nodes are replicated into m
     at.resize(P.size());
-    constexpr uint32_t AHEAD = 16;
-    for (uint32_t j = 0; j < P.size(); ++j) {
-      if (j + AHEAD < P.size()) m.idx.prefetch(P.cell[j + AHEAD]);
-      at[j] = m.idx.find(P.cell[j]);
-    }
+    constexpr uint32_t AHEAD = 16, CHUNK = 8192;
+    const uint32_t np = P.size();
+    parallel_for(
+        (int)((np + CHUNK - 1) / CHUNK),
+        [&](int c) {
+          const uint32_t b = (uint32_t)c * CHUNK, e = std::min(np, b + CHUNK);
+          for (uint32_t j = b; j < e; ++j) {
+            if (j + AHEAD < e) m.idx.prefetch(P.cell[j + AHEAD]);
+            at[j] = m.idx.find(P.cell[j]);
+          }
+        },
+        nt);
     auto rep = [&](uint32_t x) {
       const uint32_t r = m.replicate(P, x);
       m.fn.push_back(P.fn[x]);
@@ -754,23 +745,14 @@ void merge_down(Complex &P, Complex &m1, Complex &m2) {
       }
     }
     if (nt > 1 && fill.size() >= 8192 && distinct) {
-      std::atomic<size_t> next(0);
+      // rewrite() throws only when an allocation fails
       const size_t nf = fill.size(), chunk = 512;
-      std::vector<std::exception_ptr> errs(nt);
-      auto work = [&](int t) {
-        try {  // rewrite() throws only when an allocation fails
-          for (size_t b = next.fetch_add(chunk); b < nf; b = next.fetch_add(chunk))
-            for (size_t q = b; q < std::min(nf, b + chunk); ++q) rewrite(fill[q]);
-        } catch (...) {
-          errs[t] = std::current_exception();
-        }
-      };
-      std::vector<std::thread> th;
-      for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
-      work(0);
-      for (auto &t : th) t.join();
-      for (auto &e : errs)
-        if (e) std::rethrow_exception(e);
+      parallel_for(
+          (int)((nf + chunk - 1) / chunk),
+          [&](int c) {
+            for (size_t q = (size_t)c * chunk; q < std::min(nf, (size_t)(c + 1) * chunk); ++q) rewrite(fill[q]);
+          },
+          nt);
     } else {
       for (uint32_t j : fill) rewrite(j);
     }

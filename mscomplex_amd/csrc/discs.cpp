@@ -70,48 +70,6 @@ struct DiscLease {
   DiscLease &operator=(const DiscLease &) = delete;
 };
 
-// Dynamic parallel loop over [0, n) on the host threads; of several failing
-// items the lowest index's exception is rethrown after all have joined.
-template <class F>
-static void par_for(int n, F f) {
-  const int nt = std::min(n, host_threads());
-  if (n <= 1 || nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next(0);
-  std::vector<std::exception_ptr> errs(nt);
-  std::vector<int> err_at(nt, n);
-  const int cap = thread_cap();
-  auto body = [&](int t) {
-    ThreadCap keep(cap);
-    for (int i = next++; i < n; i = next++) {
-      try {
-        f(i);
-      } catch (...) {
-        if (i < err_at[t]) {
-          err_at[t] = i;
-          errs[t] = std::current_exception();
-        }
-      }
-    }
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) {
-    try {
-      th.emplace_back(body, t);
-    } catch (...) {
-      break;
-    }
-  }
-  body(0);
-  for (auto &x : th) x.join();
-  int first = -1;
-  for (int t = 0; t < nt; ++t)
-    if (errs[t] && (first < 0 || err_at[t] < err_at[first])) first = t;
-  if (first >= 0) std::rethrow_exception(errs[first]);
-}
-
 // Work items of a per-leaf loop split into chunks of at most `chunk`
 // elements: (leaf, begin, end), the largest leaves' chunks first, so that a
 // few large leaves spread over the threads and small ones fill the gaps.
@@ -250,7 +208,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   for (int i = 0; i < n; ++i) nk[i] = tiles[i]->K;
   {
     const std::vector<Chunk> whole = chunks_of(nk, ~(size_t)0);  // one item per leaf, largest first
-    par_for((int)whole.size(), [&](int q) {
+    parallel_for((int)whole.size(), [&](int q) {
       const int i = whole[q].leaf;
       disc_prepare(*cs[i], rect[i], cv[i], dp[i], false);
     });
@@ -295,7 +253,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   char *th = (char *)lease.s->trace.reserve_host(t_bytes);
   DiscTraceDesc *td = (DiscTraceDesc *)(th + t_desc);
   uint32_t *terr_h = (uint32_t *)(th + t_err);
-  par_for(n, [&](int i) {
+  parallel_for(n, [&](int i) {
     const DiscPrep &P = dp[i];
     memcpy(th + t_ioff + 4 * (slot_base[i] + i), P.init_off.data(), 4 * (nslot[i] + 1));
     memcpy(th + t_trc + slot_base[i], P.traced.data(), nslot[i]);
@@ -353,7 +311,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   {
     // entries of a leaf's tables are independent: large leaves are split
     const std::vector<Chunk> items = chunks_of(nk, 1 << 16);
-    par_for((int)items.size(), [&](int q) {
+    parallel_for((int)items.size(), [&](int q) {
       const int i = items[q].leaf;
       const uint32_t K = tiles[i]->K;
       uint32_t *owned = (uint32_t *)(stage + o_tab + tab_off[i]);
@@ -445,7 +403,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     // each complex's offsets by slot; the device offsets are the same per leaf
     uint32_t *off_h = (uint32_t *)(th + t_off);
     std::atomic<uint64_t> wide(0), longest(0);
-    par_for(n, [&](int i) {
+    parallel_for(n, [&](int i) {
       std::vector<uint32_t> &o = cs[i]->sd_off;
       o.assign(nslot[i] + 1, 0);
       const uint32_t *c = cnt_h + slot_base[i];
@@ -505,7 +463,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   tp = now_wall();
   std::atomic<int64_t> wait_ns(0), trace_ns(0), group_ns(0);
   try {
-    par_for(2 * n, [&](int q) {
+    parallel_for(2 * n, [&](int q) {
       const double w0 = now_wall();
       if (q >= n) {
         if (gpu_trace) return;
@@ -535,7 +493,7 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   MSCX_HIP(hipStreamSynchronize(ts));
   if (gpu_trace) {
     const double w0 = now_wall();
-    par_for(n, [&](int i) {
+    parallel_for(n, [&](int i) {
       std::vector<uint32_t> &sc = cs[i]->sd_cells;
       sc.assign(out_h + out_base[i], out_h + out_base[i + 1]);
       for (const auto &hs : host_slots[i]) std::copy(hs.second.begin(), hs.second.end(), sc.begin() + cs[i]->sd_off[hs.first]);

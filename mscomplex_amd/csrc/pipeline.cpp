@@ -29,54 +29,6 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Dynamic parallel loop over [0, n) on up to `nt` threads.  When items
-// fail, the exception of the lowest failing index is rethrown after all
-// threads have joined: the failure a sequential loop would have met first,
-// whatever the threads' timing (items after a failure still run; callers
-// discard their results).
-template <class F>
-static void parallel_for(int n, F f, int nt = 0) {
-  if (nt <= 0) nt = host_threads();
-  nt = std::min(nt, n);
-  if (n <= 1 || nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next(0);
-  std::vector<std::thread> th;
-  std::vector<std::exception_ptr> errs(nt);
-  std::vector<int> err_at(nt, n);
-  const int cap = thread_cap();
-  auto body = [&](int t) {
-    ThreadCap keep(cap);
-    for (int i = next++; i < n; i = next++) {
-      try {
-        f(i);
-      } catch (...) {
-        if (i < err_at[t]) {
-          err_at[t] = i;
-          errs[t] = std::current_exception();
-        }
-      }
-    }
-  };
-  // worker 0 is the calling thread; a worker that cannot be started leaves
-  // its share to the others
-  for (int t = 1; t < nt; ++t) {
-    try {
-      th.emplace_back(body, t);
-    } catch (...) {
-      break;
-    }
-  }
-  body(0);
-  for (auto &x : th) x.join();
-  int first = -1;
-  for (int t = 0; t < nt; ++t)
-    if (errs[t] && (first < 0 || err_at[t] < err_at[first])) first = t;
-  if (first >= 0) std::rethrow_exception(errs[first]);
-}
-
 // createPieces_quadtree (grid_datamanager.cpp:53-89): split along y into
 // 2^levels slabs; with lx/ly a 2-D tiling split along y while it has more
 // levels left than x, else along x.  Children share the split line; each
@@ -403,6 +355,13 @@ static void grid_simplify_down_dev(mscx_grid *G) {
 static void init_spill(mscx_grid *G);
 
 // mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
+// Host threads one merge of a tree level may use inside: the level's share
+// of the caller's threads when the level's merges run side by side, else 3
+// (a batch runs whole grids side by side).
+static int merge_threads(bool inner_parallel, int threads, int merges) {
+  return inner_parallel ? std::max(1, threads / std::max(1, merges)) : 3;
+}
+
 void grid_merge_up(mscx_grid *G, bool inner_parallel) {
   if (G->dmerge) {
     try {
@@ -418,11 +377,13 @@ void grid_merge_up(mscx_grid *G, bool inner_parallel) {
   G->pieces.resize(2 * nleaf - 1, nullptr);
   if (G->ooc) init_spill(G);
   for (int p = nleaf; p < 2 * nleaf - 1; ++p) G->pieces[p] = new mscx_complex;
+  const int T = host_threads();
   for (int lo = 0, width = nleaf; width > 1; lo += width, width /= 2)
     parallel_for(
         width / 2,
         [&](int k) {
           int j = lo + 2 * k;
+          ThreadCap cap(merge_threads(inner_parallel, T, width / 2));
           merge_up(restore_piece(G, j)->c, restore_piece(G, j + 1)->c, G->pieces[nleaf + j / 2]->c);
           // out of core: the children wait on the side until merge down
           spill_piece(G, j);
@@ -462,6 +423,7 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
   const double t3 = now_s();
   G->t[2] += t3 - t2;
   if (nleaf > 1) {
+    const int T = host_threads();
     std::vector<int> level{2 * nleaf - 2};
     while (!level.empty()) {
       parallel_for(
@@ -469,6 +431,7 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
           [&](int k) {
             int p = level[k];
             int c = 2 * (p - nleaf);
+            ThreadCap cap(merge_threads(inner_parallel, T, (int)level.size()));
             merge_down(restore_piece(G, p)->c, restore_piece(G, c)->c, restore_piece(G, c + 1)->c);
             spill_piece(G, p);  // out of core: the parent is final
           },

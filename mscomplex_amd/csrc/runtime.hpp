@@ -3,7 +3,11 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstddef>
+#include <exception>
+#include <thread>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -87,6 +91,54 @@ struct ThreadCap {
   ThreadCap(const ThreadCap &) = delete;
   ThreadCap &operator=(const ThreadCap &) = delete;
 };
+
+// Dynamic parallel loop over [0, n) on up to `nt` threads.  When items
+// fail, the exception of the lowest failing index is rethrown after all
+// threads have joined: the failure a sequential loop would have met first,
+// whatever the threads' timing (items after a failure still run; callers
+// discard their results).
+template <class F>
+void parallel_for(int n, F f, int nt = 0) {
+  if (nt <= 0) nt = host_threads();
+  nt = std::min(nt, n);
+  if (n <= 1 || nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> errs(nt);
+  std::vector<int> err_at(nt, n);
+  const int cap = thread_cap();
+  auto body = [&](int t) {
+    ThreadCap keep(cap);
+    for (int i = next++; i < n; i = next++) {
+      try {
+        f(i);
+      } catch (...) {
+        if (i < err_at[t]) {
+          err_at[t] = i;
+          errs[t] = std::current_exception();
+        }
+      }
+    }
+  };
+  // worker 0 is the calling thread; a worker that cannot be started leaves
+  // its share to the others
+  for (int t = 1; t < nt; ++t) {
+    try {
+      th.emplace_back(body, t);
+    } catch (...) {
+      break;
+    }
+  }
+  body(0);
+  for (auto &x : th) x.join();
+  int first = -1;
+  for (int t = 0; t < nt; ++t)
+    if (errs[t] && (first < 0 || err_at[t] < err_at[first])) first = t;
+  if (first >= 0) std::rethrow_exception(errs[first]);
+}
 
 // Growable device scratch and pinned host staging used by batched launches.
 struct Scratch {
