@@ -175,9 +175,10 @@ def test_degenerate_tilings(case):
 
 
 def test_large_merges_take_the_threaded_path():
-    # a root of more than 16384 nodes: merge_up builds the union's cell map
-    # on one thread while two others copy the arcs; the result must not
-    # depend on it
+    # a root of more than 16384 nodes: merge_up looks cells up and fills
+    # the union on several threads and builds its cell map on one thread
+    # while the others build the arc sets by ranges of destination nodes;
+    # the result must not depend on it
     data = rng_field("white", 300, 260, 4)
     compare(data, lx=1, ly=1, threshold=0.0)
     compare(data, levels=2, threshold=0.01)
