@@ -19,8 +19,9 @@ def whole(sx, sy):
 
 
 def test_h1_witness():
-    # on this 2x3 grid the CPU path pairs edge (1,2) with quad
-    # (1,3); the descending compare of the OpenCL path would pick (1,1).
+    # On this 2x3 grid the reference's CPU path pairs edge (1,2) with quad
+    # (1,3) (compareCells sorts ascending, grid_dataset.cpp); the descending
+    # compare of its OpenCL path would pick (1,1).
     data = np.array([[2, 3], [5, 6], [4, 1]], np.float32)
     f, p, o, c = oracle.tile_work(data, whole(2, 3))
     assert unpack(p[2, 1]) == (1, 3)
