@@ -53,8 +53,9 @@ int mscx_version(void);
  * parts as wall clock of the calls: 7 critical-point pass, 8 label tables,
  * 10 disc tracing on the GPU (with its transfers and scan), 13 grouping of
  * the extremum discs; and as host-thread time: 9 waiting for the labels,
- * 11 copying the traced lists, 12 grouping; and 14 the wall clock of the
- * gradient codes' download that starts the manifold pass. */
+ * 11 copying the traced lists, 12 grouping; 14 the wall clock of the
+ * gradient codes' download that starts the manifold pass; and 15 the wall
+ * clock of the leaves' host copies after a merge tree on the GPU. */
 int mscx_profile(int enable);
 int mscx_profile_read(double *ms, uint32_t *counts, int n);
 

@@ -146,7 +146,7 @@ def merge_stats():
 
 STAGES = ("gradient", "scan", "compact", "trace", "owner", "incidence", "host_complex",
           "disc_prep", "disc_tables", "disc_label_wait_thread_time", "disc_trace", "disc_lists_thread_time",
-          "disc_group_thread_time", "disc_group", "disc_codes")
+          "disc_group_thread_time", "disc_group", "disc_codes", "leaf_copies")
 
 
 def profile_read():

@@ -15,6 +15,8 @@
 #include <stdexcept>
 #include <thread>
 
+#include <sys/mman.h>
+
 #include "runtime.hpp"
 #include "tile.hpp"
 
@@ -28,6 +30,8 @@ void CellMap::reserve(size_t n) {
   if (want <= slots_.size()) return;
   std::vector<Slot> old;
   old.swap(slots_);
+  slots_.reserve(want);
+  advise_huge(slots_.data(), want * sizeof(Slot));
   slots_.assign(want, Slot{UINT32_MAX, 0});
   mask_ = want - 1;
   n_ = 0;
@@ -36,6 +40,11 @@ void CellMap::reserve(size_t n) {
 }
 
 void *CellMap::raw_table(size_t slots, size_t n) {
+  if (slots > slots_.capacity()) {
+    std::vector<Slot>().swap(slots_);
+    slots_.reserve(slots);
+    advise_huge(slots_.data(), slots * sizeof(Slot));
+  }
   slots_.resize(slots);
   mask_ = slots - 1;
   n_ = n;
@@ -97,13 +106,31 @@ DiscSpan Complex::disc(uint32_t i, int j) const {
   return DiscSpan{nullptr, 0};
 }
 
+// Large host arrays are backed by transparent huge pages where the system
+// allows it (madvise mode): their first touch then takes one fault per 2 MiB
+// instead of per 4 KiB.  Only whole 2 MiB pages inside the buffer are
+// advised; the advice is a hint and its failure is ignored.
+void advise_huge(const void *p, size_t bytes) {
+  constexpr uintptr_t H = (uintptr_t)2 << 20;
+  if (!p || bytes < 2 * H) return;
+  const uintptr_t b = ((uintptr_t)p + H - 1) & ~(H - 1), e = ((uintptr_t)p + bytes) & ~(H - 1);
+  if (e > b) (void)madvise((void *)b, e - b, MADV_HUGEPAGE);
+}
+
+template <class V>
+static void reserve_huge(V &v, size_t n) {
+  if (n <= v.capacity()) return;
+  v.reserve(n);
+  advise_huge(v.data(), v.capacity() * sizeof(typename V::value_type));
+}
+
 void Complex::reserve_nodes(size_t n) {
-  cell.reserve(n);
-  fn.reserve(n);
-  pair.reserve(n);
-  flags.reserve(n);
-  asc.reserve(n);
-  des.reserve(n);
+  reserve_huge(cell, n);
+  reserve_huge(fn, n);
+  reserve_huge(pair, n);
+  reserve_huge(flags, n);
+  reserve_huge(asc, n);
+  reserve_huge(des, n);
 }
 
 uint32_t Complex::replicate(const Complex &src, uint32_t j) {

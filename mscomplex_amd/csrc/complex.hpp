@@ -236,6 +236,9 @@ struct Complex {
   DiscSpan disc(uint32_t i, int j) const;
 };
 
+// Transparent huge pages for a large host buffer not yet touched (a hint).
+void advise_huge(const void *p, size_t bytes);
+
 // GridDataset::writeout_connectivity from a worked tile.
 void complex_from_tile(const mscx_tile *t, Complex &out);
 // Same, from host copies of the tile's critical data (K critical cells;

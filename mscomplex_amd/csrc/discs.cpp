@@ -480,6 +480,12 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
       const uint32_t b0 = lab_base[i], nn = m.size();
       m.xd_off.resize((size_t)nn + 1);
       for (uint32_t k = 0; k <= nn; ++k) m.xd_off[k] = hoff[b0 + k] - hoff[b0];
+      const size_t nc = hoff[b0 + nn] - hoff[b0];
+      if (nc > m.xd_cells.capacity()) {
+        std::vector<uint32_t>().swap(m.xd_cells);
+        m.xd_cells.reserve(nc);
+        advise_huge(m.xd_cells.data(), 4 * nc);
+      }
       m.xd_cells.assign(hcells + hoff[b0], hcells + hoff[b0 + nn]);
       group_ns += (int64_t)(1e9 * (now_wall() - w1));
     });
@@ -495,6 +501,12 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
     const double w0 = now_wall();
     parallel_for(n, [&](int i) {
       std::vector<uint32_t> &sc = cs[i]->sd_cells;
+      const size_t nc = out_base[i + 1] - out_base[i];
+      if (nc > sc.capacity()) {
+        std::vector<uint32_t>().swap(sc);
+        sc.reserve(nc);
+        advise_huge(sc.data(), 4 * nc);
+      }
       sc.assign(out_h + out_base[i], out_h + out_base[i + 1]);
       for (const auto &hs : host_slots[i]) std::copy(hs.second.begin(), hs.second.end(), sc.begin() + cs[i]->sd_off[hs.first]);
     });

@@ -1356,13 +1356,41 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
   MSCX_HIP(hipMemcpyAsync(hw, w, (size_t)words * 4, hipMemcpyDeviceToHost, s));
   MSCX_HIP(hipMemcpyAsync(ho, sz, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   c.reserve_nodes(Complex::with_headroom(n));
+  // on one thread the nodes are appended as they are parsed (every array
+  // written once); on several, the arrays are sized first and parsed by ranges
+  const int nt = n >= (1u << 16) ? std::min(host_threads(), 16) : 1;
+  if (nt == 1) {
+    dc.mark("alloc");
+    MSCX_HIP(hipStreamSynchronize(s));
+    dc.mark("copies");
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t *r = hw + ho[i];
+      c.cell.push_back(r[0]);
+      float f;
+      memcpy(&f, &r[1], 4);
+      c.fn.push_back(f);
+      c.pair.push_back(r[2]);
+      c.flags.push_back((uint8_t)r[3]);
+      const uint32_t na = r[4], nd = r[5];
+      c.asc.emplace_back();
+      c.des.emplace_back();
+      ConnSet &a = c.asc.back(), &d = c.des.back();
+      for (uint32_t q = 0; q < na; ++q) a.push_sorted(r[6 + q]);
+      for (uint32_t q = 0; q < nd; ++q) d.push_sorted(r[6 + na + q]);
+    }
+    dc.mark("parse");
+    out = std::move(c);
+    return;
+  }
   c.cell.resize(n);
   c.fn.resize(n);
   c.pair.resize(n);
   c.flags.resize(n);
   c.asc.resize(n);
   c.des.resize(n);
+  dc.mark("alloc");
   MSCX_HIP(hipStreamSynchronize(s));
+  dc.mark("copies");
   auto parse = [&](uint32_t lo, uint32_t hi) {
     for (uint32_t i = lo; i < hi; ++i) {
       const uint32_t *r = hw + ho[i];
@@ -1375,16 +1403,14 @@ void dcomplex_download(const DComplex &d, Complex &out, hipStream_t s) {
       for (uint32_t q = 0; q < nd; ++q) c.des[i].push_sorted(r[6 + na + q]);
     }
   };
-  const int nt = n >= (1u << 16) ? std::min(host_threads(), 16) : 1;
-  if (nt > 1) {
+  {
     std::vector<std::thread> th;
     const uint32_t per = (n + nt - 1) / nt;
     for (int t = 1; t < nt; ++t) th.emplace_back(parse, std::min(n, t * per), std::min(n, (t + 1) * per));
     parse(0, std::min(n, per));
     for (auto &x : th) x.join();
-  } else {
-    parse(0, n);
   }
+  dc.mark("parse");
   out = std::move(c);
 }
 

@@ -341,14 +341,20 @@ static void grid_simplify_down_dev(mscx_grid *G) {
     }
   }
   // the manifold pass works on host copies of the leaves
-  if (!(G->flags & MSCX_RUN_NO_DISCS))
+  if (!(G->flags & MSCX_RUN_NO_DISCS)) {
+    const double tc = now_s();
+    const int T = host_threads();
     parallel_for(nleaf, [&](int i) {
       if (G->pieces[i]) return;
+      // the leaves are copied side by side: each with its share of threads
+      ThreadCap cap(std::max(1, T / nleaf));
       StreamLease sl;
       std::unique_ptr<mscx_complex> c(new mscx_complex);
       dcomplex_download(*G->dp[i], c->c, sl.s);
       G->pieces[i] = c.release();
     });
+    stage_timer().add_host(ST_LEAF_COPIES, now_s() - tc);
+  }
   G->t[3] += now_s() - t3;
 }
 

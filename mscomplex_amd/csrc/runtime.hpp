@@ -182,11 +182,12 @@ DevicePool &device_pool();
 // GPU with its transfers and scan, 13 the extremum discs' grouping with the
 // lists' copy into the complexes; 9, 11 and 12 are host-thread times
 // (summed over the threads) of waiting for the labels, of the lists' copy
-// (or host tracing) and of grouping; 14 the codes' download to the host.
+// (or host tracing) and of grouping; 14 the codes' download to the host;
+// 15 the host copies of the leaves' complexes after a merge tree on the GPU.
 enum Stage {
   ST_GRADIENT = 0, ST_SCAN, ST_COMPACT, ST_TRACE, ST_OWNER, ST_INCIDENCE, ST_HOST_COMPLEX,
   ST_DISC_PREP, ST_DISC_TABLES, ST_DISC_LABELS, ST_DISC_PATHS, ST_DISC_TRACE_CPU, ST_DISC_GROUP_CPU, ST_DISC_GROUP,
-  ST_DISC_CODES, ST_COUNT
+  ST_DISC_CODES, ST_LEAF_COPIES, ST_COUNT
 };
 struct StageTimer {
   bool on = false;
