@@ -968,6 +968,9 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
   const size_t nslot = 2 * (size_t)m.size();
   out.init_off.assign(nslot + 1, 0);
   out.traced.assign(nslot, 0);
+  // the nodes that append entries (boundary-cancellable extrema, surviving
+  // saddles), in node order: the second pass visits only them
+  std::vector<uint32_t> pushers;
   for (int pass = 0; pass < 2; ++pass) {
     std::vector<uint32_t> cur;
     if (pass == 1) {
@@ -982,9 +985,12 @@ void disc_prepare(Complex &m, const Rect &rect, const CodesView &cv, DiscPrep &o
       else
         out.init_cells[cur[sl]++] = cell;
     };
-    for (uint32_t i = 0; i < m.size(); ++i) {
+    const uint32_t nvisit = pass == 0 ? m.size() : (uint32_t)pushers.size();
+    for (uint32_t v = 0; v < nvisit; ++v) {
+      const uint32_t i = pass == 0 ? v : pushers[v];
       uint32_t c = m.cell[i];
       uint32_t dim = cell_dim(c);
+      if (pass == 0 && ((m.flags[i] & CPF_BCANC) != 0) == (dim != 1)) pushers.push_back(i);
       if (m.flags[i] & CPF_BCANC) {
         uint32_t pi = m.pair[i];
         uint32_t pc = m.cell[pi];
