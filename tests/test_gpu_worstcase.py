@@ -3,8 +3,12 @@
 through the GPU tile stages (gradient, collation, trace, incidence) on one
 whole-grid tile and on the 8x8 tiling.
 
-The oracle does not finish at this size, so parity rests on properties that
-hold at any size and pin each stage against an independent computation:
+The oracle does not finish the whole grid at this size, but it does finish
+single leaves of the 8x8 tiling: three leaves of each field (two corners and
+an interior leaf, 2049 x 2049 vertices with halos) are compared with the
+oracle's GridDataset work word for word (flags, pairs, owners, critical
+list).  Beyond those, parity rests on properties that hold at any size and
+pin each stage against an independent computation:
   * gradient: every pairing is a mutual involution, and sampled windows
     (random interiors, the corners and the tile sides) equal the per-cell
     rules evaluated on the CPU (checked exhaustively against the oracle in
@@ -23,6 +27,7 @@ import numpy as np
 import pytest
 
 import mscomplex_amd as mx
+import oracle
 from tests.helpers import load_shim, split_pieces, window
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
@@ -167,3 +172,15 @@ def test_16384_8x8_tiles(levels):
     for k in (0, 9, 27, 36, 54, 63):
         rect, ext = pieces[k]
         _check_tile(tiles[k], window(data, ext), rect, ext, 100 + k, f"levels={levels} leaf {k}", samples=1500)
+    # full-size leaves against the oracle, word for word
+    for k in (0, 27, 63):
+        rect, ext = pieces[k]
+        print(f"[worst] levels={levels} leaf {k} vs the oracle", flush=True)
+        of, op, oo, oc = oracle.tile_work(window(data, ext), rect, ext)
+        f, p, o, c = tiles[k].arrays()
+        label = f"levels={levels} leaf {k}"
+        assert np.array_equal(c, oc), f"{label}: critical list differs ({c.size} vs {oc.size})"
+        assert np.array_equal(f, of), f"{label}: flags differ at {np.argwhere(f != of)[:5]}"
+        assert np.array_equal(p, op), f"{label}: pairs differ at {np.argwhere(p != op)[:5]}"
+        assert np.array_equal(o, oo), f"{label}: owners differ at {np.argwhere(o != oo)[:5]}"
+        del of, op, oo, oc, f, p, o, c
