@@ -360,7 +360,6 @@ static void grid_simplify_down_dev(mscx_grid *G) {
 
 static void init_spill(mscx_grid *G);
 
-// mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
 // Host threads one merge of a tree level may use inside: the level's share
 // of the caller's threads when the level's merges run side by side, else 3
 // (a batch runs whole grids side by side).
@@ -368,6 +367,7 @@ static int merge_threads(bool inner_parallel, int threads, int merges) {
   return inner_parallel ? std::max(1, threads / std::max(1, merges)) : 3;
 }
 
+// mergePiecesUp (grid_datamanager.cpp:258-306): parent of (j, j+1) is nleaf + j/2
 void grid_merge_up(mscx_grid *G, bool inner_parallel) {
   if (G->dmerge) {
     try {
