@@ -15,8 +15,6 @@
 #include <stdexcept>
 #include <thread>
 
-#include <sys/mman.h>
-
 #include "runtime.hpp"
 #include "tile.hpp"
 
@@ -104,17 +102,6 @@ DiscSpan Complex::disc(uint32_t i, int j) const {
   if (!xd_off.empty() && i + 1 < xd_off.size() && ((j == 0 && d == 0) || (j == 1 && d == 2)))
     return DiscSpan{xd_cells.data() + xd_off[i], xd_off[i + 1] - xd_off[i]};
   return DiscSpan{nullptr, 0};
-}
-
-// Large host arrays are backed by transparent huge pages where the system
-// allows it (madvise mode): their first touch then takes one fault per 2 MiB
-// instead of per 4 KiB.  Only whole 2 MiB pages inside the buffer are
-// advised; the advice is a hint and its failure is ignored.
-void advise_huge(const void *p, size_t bytes) {
-  constexpr uintptr_t H = (uintptr_t)2 << 20;
-  if (!p || bytes < 2 * H) return;
-  const uintptr_t b = ((uintptr_t)p + H - 1) & ~(H - 1), e = ((uintptr_t)p + bytes) & ~(H - 1);
-  if (e > b) (void)madvise((void *)b, e - b, MADV_HUGEPAGE);
 }
 
 template <class V>
@@ -840,7 +827,10 @@ void simplify(Complex &m, std::vector<std::pair<uint32_t, uint32_t>> &canc, doub
                                  : INT64_MAX;
   ExactHeap pq;
   pq.reserve(narcs * 2 + 16);
+  // the pops' random accesses span these two arrays: huge pages keep them
+  // within reach of the TLB
   slots.reserve(narcs * 4 + 16);
+  advise_huge(slots.data(), slots.capacity() * sizeof(P2));
 
   float max_val = std::numeric_limits<float>::min();
   float min_val = std::numeric_limits<float>::max();

@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "hugepages.hpp"
+
 struct mscx_tile;
 
 namespace mscx {
@@ -235,9 +237,6 @@ struct Complex {
   // the asc (j = 0) or des (j = 1) disc of critical point i
   DiscSpan disc(uint32_t i, int j) const;
 };
-
-// Transparent huge pages for a large host buffer not yet touched (a hint).
-void advise_huge(const void *p, size_t bytes);
 
 // GridDataset::writeout_connectivity from a worked tile.
 void complex_from_tile(const mscx_tile *t, Complex &out);

@@ -18,6 +18,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "hugepages.hpp"
+
 namespace mscx {
 
 class ExactHeap {
@@ -26,8 +28,16 @@ class ExactHeap {
   static uint32_t key(uint64_t e) { return (uint32_t)(e >> 32); }
   static uint32_t payload(uint64_t e) { return (uint32_t)e; }
 
+  // room for n elements; the storage is advised onto huge pages before its
+  // first touch (a pop's descent touches a random path of it)
   void reserve(size_t n) {
-    if (a_.size() < 2 * n + 8) a_.resize(2 * n + 8);
+    if (a_.size() >= 2 * n + 8) return;
+    std::vector<uint64_t> b;
+    b.reserve(2 * n + 8);
+    advise_huge(b.data(), b.capacity() * sizeof(uint64_t));
+    b.assign(a_.begin(), a_.end());
+    b.resize(2 * n + 8);
+    a_.swap(b);
   }
   bool empty() const { return n_ == 0; }
   size_t size() const { return n_; }
