@@ -621,13 +621,29 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   }
   // map::operator[] on each boundary-cancellable node's pair cell (which
   // inserts a zero entry when the pair is absent, as the reference's lookup
-  // does), m1's nodes then m2's, so that m2's lookup wins for a shared node
+  // does), m1's nodes then m2's, so that m2's lookup wins for a shared node.
+  // The lookups of one complex run in parallel; the cells they miss get
+  // their zero entries afterwards, in node order (a later miss of the same
+  // cell then finds that entry: 0 either way).
   for (int i = 0; i < 2; ++i) {
     const Complex &m = *arr[i];
     const uint32_t *ai = at[i].data();
-    for (uint32_t j = 0; j < m.size(); ++j)
-      if (!(m.flags[j] & CPF_CANCELLED) && (m.flags[j] & CPF_BCANC))
-        out->pair[ai[j]] = out->idx.get_or_zero(m.cell[m.pair[j]]);
+    const uint32_t nm = m.size();
+    std::vector<std::vector<uint32_t>> miss(chunks(nm));
+    auto look = [&](int c) {
+      const uint32_t b = (uint32_t)c * CHUNK, e = std::min<uint32_t>(nm, b + CHUNK);
+      for (uint32_t j = b; j < e; ++j) {
+        if ((m.flags[j] & CPF_CANCELLED) || !(m.flags[j] & CPF_BCANC)) continue;
+        const uint32_t v = out->idx.find(m.cell[m.pair[j]]);
+        if (v != UINT32_MAX)
+          out->pair[ai[j]] = v;
+        else
+          miss[c].push_back(j);
+      }
+    };
+    parallel_for(chunks(nm), look, par ? nt : 1);
+    for (const std::vector<uint32_t> &mc : miss)
+      for (uint32_t j : mc) out->pair[ai[j]] = out->idx.get_or_zero(m.cell[m.pair[j]]);
   }
   merge_up_line(*out, ixn);
 }
