@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -315,9 +316,14 @@ void disc_labels_host(const DiscPrep &dp, const Rect &rect, const Rect &ext, con
 void disc_group(Complex &m, const Rect &rect, const uint32_t *lv, const uint32_t *lq);
 
 
+// The gradient codes of n worked tiles copied to pinned host memory ahead
+// of their manifold pass (discs.cpp); the copies run on the auxiliary stream.
+struct CodesStage;
+std::shared_ptr<CodesStage> stage_codes(mscx_tile *const *tiles, int n);
 // postMergeFillDiscs for n worked tiles and their complexes, label pass on
-// the GPU (discs.cpp).
-void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n);
+// the GPU (discs.cpp); `pre` (may be null) holds the same tiles' codes
+// staged by stage_codes.
+void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n, CodesStage *pre = nullptr);
 
 void write_discs(const Complex &m, const std::string &prefix);
 // Host archive of a complex (dump words + checksum); load throws MSCX_EIO

@@ -37,7 +37,7 @@ static double now_wall() {
 }
 
 struct DiscScratch {
-  Scratch disc, stage, trace, lists;
+  Scratch codes, disc, stage, trace, lists;
 };
 
 // A block of the device pool that goes back once `s` has finished with it.
@@ -69,6 +69,49 @@ struct DiscLease {
   DiscLease(const DiscLease &) = delete;
   DiscLease &operator=(const DiscLease &) = delete;
 };
+
+// Byte offsets of the tiles' gradient codes in one host staging buffer.
+static std::vector<size_t> codes_offsets(mscx_tile *const *tiles, int n) {
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + align_up((size_t)tiles[i]->d.g.ch * tiles[i]->d.pitch, 256);
+  return off;
+}
+
+// The gradient codes of a set of worked tiles, copied to pinned host memory
+// on the auxiliary stream ahead of their manifold pass (the copies overlap
+// the host merge tree); the lease that holds the staging goes back to the
+// pool once the copies have finished.
+struct CodesStage {
+  DiscLease lease;
+  std::vector<mscx_tile *> tiles;
+  std::vector<size_t> off;
+  const uint8_t *host = nullptr;
+  hipEvent_t ev = nullptr;
+  ~CodesStage() {
+    if (ev) {
+      (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+    }
+  }
+};
+
+std::shared_ptr<CodesStage> stage_codes(mscx_tile *const *tiles, int n) {
+  if (n <= 0) return nullptr;
+  ensure_init();
+  for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i]->worked, MSCX_ESTATE, "tile has not been worked");
+  std::shared_ptr<CodesStage> st(new CodesStage);
+  st->tiles.assign(tiles, tiles + n);
+  st->off = codes_offsets(tiles, n);
+  char *h = (char *)st->lease.s->codes.reserve_host(st->off[n]);
+  hipStream_t s = runtime().aux;
+  for (int i = 0; i < n; ++i)
+    MSCX_HIP(hipMemcpyAsync(h + st->off[i], tiles[i]->d.codes, (size_t)tiles[i]->d.g.ch * tiles[i]->d.pitch,
+                            hipMemcpyDeviceToHost, s));
+  MSCX_HIP(hipEventCreateWithFlags(&st->ev, hipEventDisableTiming));
+  MSCX_HIP(hipEventRecord(st->ev, s));
+  st->host = (const uint8_t *)h;
+  return st;
+}
 
 // Work items of a per-leaf loop split into chunks of at most `chunk`
 // elements: (leaf, begin, end), the largest leaves' chunks first, so that a
@@ -148,26 +191,30 @@ static void disc_label_tables(const Complex &m, uint32_t K, const Rect &rect, co
   }
 }
 
-void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
+void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n, CodesStage *pre) {
   if (n <= 0) return;
   ensure_init();
   // The label pass runs on the auxiliary stream, so it overlaps the tile
   // stages of other grids on the work stream; the owner images and codes it
   // reads were completed (and synchronised) by run_tiles.  Concurrent calls
   // share the stream (a synchronisation may also wait for another call's
-  // copies) but not their staging.
-  DiscLease lease;
+  // copies) but not their staging.  Codes staged ahead for exactly these
+  // tiles are used as they are (with the staging's lease).
+  const bool staged = pre && pre->tiles.size() == (size_t)n && std::equal(pre->tiles.begin(), pre->tiles.end(), tiles);
+  std::unique_ptr<DiscLease> own_lease;
+  if (!staged) own_lease.reset(new DiscLease);
+  DiscLease &lease = staged ? pre->lease : *own_lease;
   Scratch &g_disc = lease.s->disc, &g_stage = lease.s->stage;
   hipStream_t s = runtime().aux;
   for (int i = 0; i < n; ++i) MSCX_REQUIRE(tiles[i]->worked, MSCX_ESTATE, "tile has not been worked");
 
   std::vector<Rect> rect(n), ext(n);
-  std::vector<size_t> code_off(n + 1, 0), nv(n), nq(n);
+  std::vector<size_t> nv(n), nq(n);
+  const std::vector<size_t> code_off = codes_offsets(tiles, n);
   for (int i = 0; i < n; ++i) {
     const mscx_tile *t = tiles[i];
     rect[i] = Rect{t->rect[0], t->rect[1], t->rect[2], t->rect[3]};
     ext[i] = Rect{t->ext[0], t->ext[1], t->ext[2], t->ext[3]};
-    code_off[i + 1] = code_off[i] + align_up((size_t)t->d.g.ch * t->d.pitch, 256);
     size_t nvx = (rect[i].x1 - rect[i].x0) / 2 + 1, nvy = (rect[i].y1 - rect[i].y0) / 2 + 1;
     nv[i] = nvx * nvy;
     nq[i] = (nvx - 1) * (nvy - 1);
@@ -178,8 +225,6 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   StageTimer &tm = stage_timer();
   double tp = now_wall();
   std::vector<DiscPrep> dp(n);
-  // sizes of the tables are known only after the critical-point pass, so
-  // the codes live in their own region at the start of the host scratch
   size_t lab_total = 0, nelem = 0, nlab = 0;
   std::vector<uint32_t> lab_base(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -189,17 +234,23 @@ void fill_discs_device(mscx_tile *const *tiles, Complex *const *cs, int n) {
   }
   nlab = lab_base[n];
   MSCX_REQUIRE(nlab < 0xFFFFFFFFull && nelem < (1ull << 40), MSCX_EINVAL, "too many cells for one manifold pass");
-  const size_t codes_bytes = code_off[n];
-  // host: codes | grouped cells (one word per vertex and quad) | disc offsets
-  const size_t h_cells = codes_bytes, h_off = align_up(h_cells + 4 * nelem, 256);
+  const uint8_t *codes_h;
+  if (staged) {
+    MSCX_HIP(hipEventSynchronize(pre->ev));
+    codes_h = pre->host;
+  } else {
+    char *h = (char *)lease.s->codes.reserve_host(code_off[n]);
+    for (int i = 0; i < n; ++i)
+      MSCX_HIP(hipMemcpyAsync(h + code_off[i], tiles[i]->d.codes, (size_t)tiles[i]->d.g.ch * tiles[i]->d.pitch,
+                              hipMemcpyDeviceToHost, s));
+    MSCX_HIP(hipStreamSynchronize(s));
+    codes_h = (const uint8_t *)h;
+  }
+  // host: grouped cells (one word per vertex and quad) | disc offsets
+  const size_t h_cells = 0, h_off = align_up(4 * nelem, 256);
   char *host = (char *)g_disc.reserve_host(h_off + 4 * (nlab + 1));
-  for (int i = 0; i < n; ++i)
-    MSCX_HIP(hipMemcpyAsync(host + code_off[i], tiles[i]->d.codes, (size_t)tiles[i]->d.g.ch * tiles[i]->d.pitch,
-                            hipMemcpyDeviceToHost, s));
-  MSCX_HIP(hipStreamSynchronize(s));
   std::vector<CodesView> cv(n);
-  for (int i = 0; i < n; ++i)
-    cv[i] = CodesView{(const uint8_t *)host + code_off[i], tiles[i]->d.pitch, ext[i].x0, ext[i].y0};
+  for (int i = 0; i < n; ++i) cv[i] = CodesView{codes_h + code_off[i], tiles[i]->d.pitch, ext[i].x0, ext[i].y0};
 
   tm.add_host(ST_DISC_CODES, now_wall() - tp);
   tp = now_wall();

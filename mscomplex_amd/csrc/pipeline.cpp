@@ -459,7 +459,7 @@ void grid_simplify_down(mscx_grid *G, bool inner_parallel) {
 
 // collectSubdomainManifolds / postMergeFillDiscs (:397-490) for every leaf
 // of every grid that asked for discs
-void grids_discs(std::vector<mscx_grid *> &grids) {
+void grids_discs(std::vector<mscx_grid *> &grids, CodesStage *pre) {
   const double t4 = now_s();
   std::vector<mscx_tile *> dt;
   std::vector<Complex *> dc;
@@ -470,7 +470,7 @@ void grids_discs(std::vector<mscx_grid *> &grids) {
       dc.push_back(&restore_piece(G, (int)i)->c);
     }
   }
-  fill_discs_device(dt.data(), dc.data(), (int)dt.size());
+  fill_discs_device(dt.data(), dc.data(), (int)dt.size(), pre);
   for (mscx_grid *G : grids)
     if (G->ooc && G->leaves.size() > 1)
       parallel_for((int)G->leaves.size(), [&](int i) { spill_piece(G, i); });
@@ -630,6 +630,10 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
   // of holding every tree back.
   CritLease lease;
   const char *staged = grids_compute_impl(grids, ng > 1 ? &lease : nullptr);
+  // one grid: its codes come to the host while the host runs its merge tree
+  std::shared_ptr<CodesStage> codes;
+  if (ng == 1 && !(grids[0]->flags & MSCX_RUN_NO_DISCS))
+    codes = stage_codes(grids[0]->tiles.data(), (int)grids[0]->leaves.size());
   std::vector<char> ok(ng, 1);
   auto tree = [&](int gi, bool inner) {
     try {
@@ -665,7 +669,7 @@ static void run_grids(std::vector<mscx_grid *> &grids, std::vector<int> *status,
   std::vector<mscx_grid *> done;
   for (int gi = 0; gi < ng; ++gi)
     if (ok[gi]) done.push_back(grids[gi]);
-  grids_discs(done);
+  grids_discs(done, codes.get());
 }
 
 }  // namespace mscx

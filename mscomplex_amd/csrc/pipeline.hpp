@@ -69,7 +69,7 @@ void grids_compute(std::vector<mscx_grid *> &grids);
 void grid_merge_up(mscx_grid *G, bool inner_parallel);
 void grid_simplify_down(mscx_grid *G, bool inner_parallel);
 // postMergeFillDiscs for every leaf of the given grids (label pass in one launch).
-void grids_discs(std::vector<mscx_grid *> &grids);
+void grids_discs(std::vector<mscx_grid *> &grids, CodesStage *pre = nullptr);
 // HBM a grid's tiles and input take (for admission against a budget);
 // the critical-point block is estimated at one critical cell in four.
 size_t grid_device_bytes(int32_t sx, int32_t sy, int32_t levels, int32_t lx, int32_t ly, bool with_input);
