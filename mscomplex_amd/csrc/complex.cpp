@@ -503,7 +503,7 @@ void merge_up(const Complex &m1, const Complex &m2, Complex &res) {
   // nodes: a multiset's content does not depend on the order of its
   // inserts); the pair indices are looked up last, in the reference's order.
   const int nt = host_threads();
-  const bool par = nt > 2 && (size_t)m1.size() + m2.size() >= 16384;
+  const bool par = nt >= 2 && (size_t)m1.size() + m2.size() >= 16384;
   constexpr int CHUNK = 4096;
   auto chunks = [](uint32_t n) { return (int)((n + CHUNK - 1) / CHUNK); };
   std::vector<uint32_t> at[2];
